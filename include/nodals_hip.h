@@ -1,0 +1,156 @@
+/*
+ * nodals_hip.h -- C ABI of the MI355X NodalS SIMPLE pipeline (libnodals_hip.so).
+ *
+ * Plain C types only: opaque handles, host pointers, sizes, int status.
+ * Every entry point returns 0 on success or a nonzero status:
+ *   1 invalid argument / unsupported option / malformed input
+ *   2 HIP runtime error            3 non-finite value
+ *   4 Krylov breakdown             5 not converged
+ *   6 internal error
+ * and leaves a message for nodals_last_error().  No C++ exception crosses the
+ * ABI.  A context owns all of its device memory and its HIP stream; calls on
+ * one context are not reentrant.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * path:line in the reference tree (JD63021/NodalS).
+ */
+#ifndef NODALS_HIP_H
+#define NODALS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nodals_h8 nodals_h8;   /* H8 laminar SIMPLE solver context  */
+typedef struct nodals_sys nodals_sys; /* explicit SPD system + AMG + Krylov */
+
+/* G4PCGResult (gpu/serial_cuda/h8_main_base.cu:568) plus the initial norm. */
+typedef struct {
+  int32_t its;
+  int32_t ok;
+  double rel;
+  double r0;
+} nodals_solve_result;
+
+/* Summary of one SIMPLE run: the fields of the NODALS_GPU_H8_FULLCONV /
+ * NODALS_GPU_H8_FIXED10 markers (gpu/serial_cuda/h8_main_base.cu:2190-2193). */
+typedef struct {
+  int32_t outer;
+  int32_t converged;
+  int32_t pass;
+  int32_t reserved;
+  double final_rel_cont;
+  double avg_pressure_its;
+  double pressure_drop;
+  double exact_pressure_drop;
+  double loop_ms;
+  double assembly_ms, momentum_ms, continuity_ms, pressure_ms, pupdate_ms;
+  double setup_ms;
+} nodals_h8_run_result;
+
+/* Last error message of the calling thread ("" if none). */
+const char* nodals_last_error(void);
+const char* nodals_version(void);
+
+/* ---------------------------------------------------------------- H8 ----
+ * Create from the H8 command line: the same options and validation as the
+ * reference binary (gpu/serial_cuda/h8_main_base.cu:1787-1911), argv[0]
+ * ignored.  Runs mesh input (load_foam_tet_mesh, foam_mesh_g2.hpp:55), host
+ * setup (build_g4_setup, g4_host.hpp:289; build_h2_fine_csr_host,
+ * h2b_fine_csr.cuh:445; build_sa_hierarchy, g5_sa_host.hpp:1545) and the
+ * device upload (upload_all, h8_main_base.cu:761-788). */
+int nodals_h8_create(int argc, const char* const* argv, nodals_h8** out);
+/* Host setup only (no HIP call): the same setup as nodals_h8_create without
+ * the device upload.  Only nodals_h8_info and the setup/topology fields of
+ * nodals_h8_get are valid on such a handle; other calls return status 1. */
+int nodals_h8_create_host(int argc, const char* const* argv, nodals_h8** out);
+void nodals_h8_destroy(nodals_h8* h);
+
+/* info[0..7] = cells, free velocity DOFs, momentum nnz, colors,
+ * fine pressure nnz, AMG explicit levels, terminal rows, fixed slots. */
+int nodals_h8_info(nodals_h8* h, int64_t info[8]);
+
+/* assemble_live (h8_main_base.cu:1137-1155) fused with bt_state + the
+ * momentum RHS kernel (h8_main_base.cu:2061-2062): convection on the current
+ * velocity, row-L1 relaxation, rAU, b = s + c + B^T p + delta u. */
+int nodals_h8_assemble(nodals_h8* h, double alpha_u, double rau_scale);
+
+/* fixed_momentum_work (h8_main_base.cu:1700-1715): mode "fgs1", "altgs1"
+ * or "sgs1"; multicolour Gauss-Seidel on all three components. */
+int nodals_h8_momentum_work(nodals_h8* h, const char* mode, double omega, int32_t outer);
+
+/* momentum_initial_rel_audit (h8_main_base.cu:1716-1726): ||b-Au||/||b||. */
+int nodals_h8_momentum_audit(nodals_h8* h, double rel[3]);
+
+/* Continuity residual fixedDiv + B u (h8_continuity_kernel,
+ * h8_precomputed_b.cuh:140); stores its negation as the pressure RHS and
+ * returns the FP64 2-norm (h8_main_base.cu:2070-2081). */
+int nodals_h8_continuity(nodals_h8* h, double* norm);
+
+/* Fine P0 Schur numeric refresh B diag(rAU) B^T with the current rAU
+ * (H2FineCSRDevice::refresh, h8_main_base.cu:322-328). */
+int nodals_h8_pressure_refresh(nodals_h8* h);
+
+/* pressure_pcg / pressure_richardson / pressure_cheb_poly
+ * (h8_main_base.cu:569-726) and FGMRES: solver = "pcg", "richardson",
+ * "cheb" or "fgmres", preconditioned by the AMG V-cycle. */
+int nodals_h8_pressure_solve(nodals_h8* h, const char* solver, double rtol, double atol, int32_t maxit,
+                             nodals_solve_result* res);
+
+/* p += alpha_p dp (h8_main_base.cu:2102). */
+int nodals_h8_pressure_update(nodals_h8* h, double alpha_p);
+
+/* The SIMPLE loop of h8_main_base.cu:2046-2119 with its marker lines. */
+int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res);
+
+/* Host copies in reference numbering.  Names (double unless noted):
+ *   u0 u1 u2 (free DOFs), b0 b1 b2, rau delta diag (free DOFs),
+ *   p cont dp fine_diag fine_l1 (cells), mom_val (momentum nnz, reference
+ *   CSR order), fine_val (fine nnz), and int32 topology: mom_row (int64),
+ *   mom_col, color, fine_row, fine_col, agg0.., cell_ref (8 per cell). */
+int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count);
+int nodals_h8_set(nodals_h8* h, const char* name, const void* host, int64_t count);
+
+/* Fine Schur SpMV y = A x on device-resident copies of host vectors, and a
+ * timed loop of `reps` launches of a named hot kernel ("fine_spmv",
+ * "refresh", "mom_gs", "conv_local", "mom_rows", "continuity"); ms_avg gets
+ * the mean launch time from HIP events on the context stream and
+ * bytes the algorithmic bytes of one launch. */
+int nodals_h8_fine_spmv(nodals_h8* h, const double* x, double* y);
+int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double* ms_avg, double* bytes);
+
+/* ---------------------------------------------------------------- SYS ---
+ * Explicit SPD system (FP64 CSR, int64 row pointers) with an SA hierarchy
+ * built by the coarse-level steps of build_sa_hierarchy
+ * (g5_sa_host.hpp:1570-1577) applied from level 0.  options: space separated
+ * key=value: precision=fp64|fp32|bf16 smoother=jacobi|l1jacobi|cheb2|mcgs
+ * sweeps=N omega=W order=symmetric|forward|backward terminal=N power_its=N. */
+int nodals_sys_create(int32_t n, const int64_t* row, const int32_t* col, const double* val, const char* options,
+                      nodals_sys** out);
+void nodals_sys_destroy(nodals_sys* s);
+/* info[0..3] = rows, nnz, AMG explicit levels, terminal rows;
+ * info[4+l] = rows of explicit level l (l < 12). */
+int nodals_sys_info(nodals_sys* s, int64_t info[16]);
+int nodals_sys_spmv(nodals_sys* s, const double* x, double* y);
+/* One preconditioner application z = M^-1 r. */
+int nodals_sys_precond(nodals_sys* s, const double* r, double* z);
+/* solver = "fgmres" | "pcg"; x0 = 0. */
+int nodals_sys_solve(nodals_sys* s, const double* b, double* x, const char* solver, double rtol, int32_t restart,
+                     int32_t maxit, nodals_solve_result* res);
+/* Solve with the RHS already resident, timed with HIP events around the
+ * whole Krylov solve on the context stream (ms). */
+int nodals_sys_solve_timed(nodals_sys* s, const char* solver, double rtol, int32_t restart, int32_t maxit,
+                           int32_t reps, double* ms_avg, nodals_solve_result* res);
+int nodals_sys_set_rhs(nodals_sys* s, const double* b);
+/* Mean launch time of `reps` SpMV launches on level l (-1: finest FP64
+ * operator; 0..: explicit AMG levels in the hierarchy precision), with the
+ * algorithmic bytes of one launch. */
+int nodals_sys_time_spmv(nodals_sys* s, int32_t level, int32_t reps, double* ms_avg, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NODALS_HIP_H */

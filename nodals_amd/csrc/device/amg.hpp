@@ -1,0 +1,280 @@
+// Device AMG V-cycle over a host-built SA hierarchy.
+//
+// Cycle structure of G4AMG::apply / explicit_vcycle
+// (gpu/serial_cuda/h8_main_base.cu:532-564): smoothers start from zero, the
+// post-smoother is applied as a correction to the current residual, the
+// terminal level is a dense inverse mat-vec.  Restriction is a gather SpMV
+// with the stored transpose P^T (no atomics), prolongation adds P x_c.
+//
+// Template parameters: TF is the storage type of the finest operator values
+// (double, float or bf16), TV the type of every vector and of the coarse
+// levels (double, or float for the mixed-precision hierarchy).
+#pragma once
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../host/amg_setup.hpp"
+#include "la_kernels.hpp"
+
+namespace nodals {
+namespace dev {
+
+struct SmootherOptions {
+  std::string kind = "jacobi";  // jacobi | l1jacobi | cheb2 | mcgs
+  double jacobiOmega = 0.7;
+  int jacobiFineSweeps = 1, jacobiCoarseSweeps = 1;
+  int chebDegree = 2;
+  double lambdaLowFraction = 0.05;
+  double mcgsOmega = 1.0;
+  int mcgsFineSweeps = 1, mcgsCoarseSweeps = 1;
+  std::string mcgsOrder = "symmetric";  // forward | backward | symmetric
+};
+
+// View of the finest operator, owned elsewhere (refreshed each SIMPLE step).
+template <class TF, class TV>
+struct FineView {
+  int n = 0;
+  int lanes = 16;
+  const int* row = nullptr;
+  const int* col = nullptr;
+  const TF* val = nullptr;
+  const TV* diag = nullptr;
+  const TV* l1 = nullptr;
+};
+
+template <class TV>
+struct DevColoring {
+  int ncolors = 0;
+  std::vector<int> off;
+  DBuf<int> rows;
+  void upload(const Coloring& C) {
+    ncolors = C.ncolors;
+    off = C.offset;
+    rows.upload(C.rows);
+  }
+};
+
+template <class TF, class TV>
+class DevAmg {
+ public:
+  struct Level {
+    int n = 0, lanes = 8;
+    DBuf<int> row, col;
+    DBuf<TV> val, diag, l1, b, x, r, tmp, corr;
+    double lambda = 1.0;
+    DevColoring<TV> color;
+  };
+  struct Xfer {
+    int nFine = 0, nCoarse = 0, lanesP = 8, lanesT = 8;
+    DBuf<int> prow, pcol, trow, tcol;
+    DBuf<TV> pval, tval;
+  };
+
+  void setup(const AmgHierarchy& H, int fineN, const SmootherOptions& o, const Coloring* fineColoring) {
+    opt_ = o;
+    fineN_ = fineN;
+    fineLambda_ = H.fineLambda;
+    const std::size_t nl = H.A.size();
+    require(nl >= 1 && H.P.size() == nl, "AMG hierarchy shape mismatch", kErrInternal);
+    L_.clear();
+    P_.clear();
+    L_.resize(nl);
+    P_.resize(nl);
+    for (std::size_t l = 0; l < nl; ++l) {
+      const CsrD& A = H.A[l];
+      Level& Lv = L_[l];
+      Lv.n = A.n;
+      Lv.lanes = lanesFor(A.n ? (double)A.col.size() / A.n : 1.0);
+      std::vector<int> r32(A.row.begin(), A.row.end());
+      Lv.row.upload(r32);
+      Lv.col.upload(A.col);
+      Lv.val.upload(castVec(A.val));
+      Lv.diag.upload(castVec(A.diag));
+      std::vector<double> l1((std::size_t)A.n);
+      for (int i = 0; i < A.n; ++i) {
+        double s = 0;
+        for (std::int64_t k = A.row[(std::size_t)i]; k < A.row[(std::size_t)i + 1]; ++k) s += std::abs(A.val[(std::size_t)k]);
+        l1[(std::size_t)i] = s;
+      }
+      Lv.l1.upload(castVec(l1));
+      for (DBuf<TV>* v : {&Lv.b, &Lv.x, &Lv.r, &Lv.tmp, &Lv.corr}) v->alloc((std::size_t)A.n);
+      Lv.x.upload(std::vector<TV>((std::size_t)A.n, TV(1)));
+      Lv.lambda = l < H.levelLambda.size() ? H.levelLambda[l] : 1.0;
+      if (opt_.kind == "mcgs" && l + 1 < nl) Lv.color.upload(greedyColoring(A.n, A.row.data(), A.col.data()));
+      uploadTransfer(H.P[l], P_[l]);
+    }
+    termN_ = H.terminalN;
+    tinv_.upload(castVec(H.terminalInv));
+    for (DBuf<TV>* v : {&fr_, &ftmp_, &fcorr_}) v->alloc((std::size_t)fineN);
+    if (opt_.kind == "mcgs") {
+      require(fineColoring != nullptr, "MCGS smoother requires a fine coloring", kErrInternal);
+      fineColor_.upload(*fineColoring);
+    }
+  }
+
+  void setFine(const FineView<TF, TV>& v) { fine_ = v; }
+  // tmp = A_l x on explicit level l (measurement of the coarse SpMV).
+  void levelSpmv(hipStream_t s, int l) {
+    Level& A = L_[(std::size_t)l];
+    spmv<TV, TV, false, false>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), A.x.data(), nullptr,
+                               A.tmp.data());
+  }
+  int levels() const { return (int)L_.size(); }
+  const Level& level(int l) const { return L_[(std::size_t)l]; }
+
+  // z = M^{-1} b on the finest level.
+  void apply(hipStream_t s, const TV* b, TV* z) {
+    smoothFine(s, b, z);
+    spmv<TF, TV, true, false>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, z, b, fr_.data());
+    Level& C = L_[0];
+    restrictTo(s, P_[0], fr_.data(), C.b.data());
+    vcycle(s, 0, C.b.data(), C.x.data());
+    prolongAdd(s, P_[0], C.x.data(), z);
+    spmv<TF, TV, true, false>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, z, b, fr_.data());
+    smoothFine(s, fr_.data(), fcorr_.data());
+    axpy<TV>(s, fine_.n, 1.0, fcorr_.data(), z);
+  }
+
+ private:
+  template <class TS>
+  static std::vector<TV> castVec(const std::vector<TS>& v) {
+    std::vector<TV> o(v.size());
+    for (std::size_t i = 0; i < v.size(); ++i) o[i] = (TV)v[i];
+    return o;
+  }
+
+  static void uploadTransfer(const Transfer& P, Xfer& X) {
+    X.nFine = P.nFine;
+    X.nCoarse = P.nCoarse;
+    std::vector<int> pr(P.row.begin(), P.row.end());
+    X.prow.upload(pr);
+    X.pcol.upload(P.col);
+    X.pval.upload(castVec(P.val));
+    X.lanesP = lanesFor(P.nFine ? (double)P.col.size() / P.nFine : 1.0);
+    // Transpose for gather restriction; within a coarse row, fine rows ascend.
+    std::vector<int> tr((std::size_t)P.nCoarse + 1, 0);
+    for (int c : P.col) ++tr[(std::size_t)c + 1];
+    for (int c = 0; c < P.nCoarse; ++c) tr[(std::size_t)c + 1] += tr[(std::size_t)c];
+    std::vector<int> tc(P.col.size());
+    std::vector<double> tv(P.col.size());
+    std::vector<int> cur(tr.begin(), tr.end() - 1);
+    for (int i = 0; i < P.nFine; ++i)
+      for (std::int64_t k = P.row[(std::size_t)i]; k < P.row[(std::size_t)i + 1]; ++k) {
+        const int c = P.col[(std::size_t)k];
+        const int z = cur[(std::size_t)c]++;
+        tc[(std::size_t)z] = i;
+        tv[(std::size_t)z] = P.val[(std::size_t)k];
+      }
+    X.trow.upload(tr);
+    X.tcol.upload(tc);
+    X.tval.upload(castVec(tv));
+    X.lanesT = lanesFor(P.nCoarse ? (double)tc.size() / P.nCoarse : 1.0);
+  }
+
+  void restrictTo(hipStream_t s, const Xfer& X, const TV* f, TV* c) {
+    spmv<TV, TV, false, false>(s, X.lanesT, X.nCoarse, X.trow.data(), X.tcol.data(), X.tval.data(), f, nullptr, c);
+  }
+  void prolongAdd(hipStream_t s, const Xfer& X, const TV* c, TV* f) {
+    spmv<TV, TV, false, true>(s, X.lanesP, X.nFine, X.prow.data(), X.pcol.data(), X.pval.data(), c, nullptr, f);
+  }
+
+  static double chebRoot(double hi, double lo, int k, int degree) {
+    const double centre = .5 * (hi + lo), radius = .5 * (hi - lo);
+    const double pi = 3.141592653589793238462643383279502884;
+    return centre - radius * std::cos(pi * (2.0 * k + 1.0) / (2.0 * degree));
+  }
+
+  // Generic smoother on an operator (row, col, val, diag, l1) from x = 0.
+  template <class TM>
+  void smooth(hipStream_t s, int n, int lanes, const int* row, const int* col, const TM* val, const TV* diag,
+              const TV* l1, const DevColoring<TV>* color, double lambda, int jacobiSweeps, int mcgsSweeps,
+              const TV* b, TV* x, TV* tmp, TV* r) {
+    const int grid = blocksFor(n);
+    if (opt_.kind == "jacobi" || opt_.kind == "l1jacobi") {
+      const TV* den = opt_.kind == "l1jacobi" ? l1 : diag;
+      hipLaunchKernelGGL(k_jacobi_zero<TV>, grid, 256, 0, s, n, opt_.jacobiOmega, b, den, x);
+      NODALS_LAUNCHED();
+      for (int sw = 1; sw < jacobiSweeps; ++sw) {
+        spmv<TM, TV, true, false>(s, lanes, n, row, col, val, x, b, r);
+        hipLaunchKernelGGL(k_jacobi_add<TV>, grid, 256, 0, s, n, opt_.jacobiOmega, r, den, x);
+        NODALS_LAUNCHED();
+      }
+    } else if (opt_.kind == "cheb2") {
+      const double lo = opt_.lambdaLowFraction * lambda;
+      require(lambda > lo && lo > 0, "Chebyshev interval invalid");
+      hipLaunchKernelGGL(k_jacobi_zero<TV>, grid, 256, 0, s, n, 1.0 / chebRoot(lambda, lo, 0, opt_.chebDegree), b,
+                         diag, x);
+      NODALS_LAUNCHED();
+      for (int k = 1; k < opt_.chebDegree; ++k) {
+        spmv<TM, TV, true, false>(s, lanes, n, row, col, val, x, b, r);
+        hipLaunchKernelGGL(k_jacobi_add<TV>, grid, 256, 0, s, n, 1.0 / chebRoot(lambda, lo, k, opt_.chebDegree), r,
+                           diag, x);
+        NODALS_LAUNCHED();
+      }
+    } else if (opt_.kind == "mcgs") {
+      fill<TV>(s, n, x, TV(0));
+      for (int sw = 0; sw < mcgsSweeps; ++sw) {
+        if (opt_.mcgsOrder == "forward" || opt_.mcgsOrder == "symmetric") colorPass(s, lanes, row, col, val, diag, b, x, *color, false);
+        if (opt_.mcgsOrder == "backward" || opt_.mcgsOrder == "symmetric") colorPass(s, lanes, row, col, val, diag, b, x, *color, true);
+      }
+    } else {
+      fail("unknown AMG smoother " + opt_.kind);
+    }
+    (void)tmp;
+  }
+
+  template <class TM>
+  void colorPass(hipStream_t s, int lanes, const int* row, const int* col, const TM* val, const TV* diag, const TV* b,
+                 TV* x, const DevColoring<TV>& C, bool reverse) {
+    for (int q = 0; q < C.ncolors; ++q) {
+      const int c = reverse ? C.ncolors - 1 - q : q;
+      const int begin = C.off[(std::size_t)c];
+      mcgsColor<TM, TV>(s, lanes, begin, C.off[(std::size_t)c + 1] - begin, C.rows.data(), row, col, val, diag, b, x,
+                        opt_.mcgsOmega);
+    }
+  }
+
+  void smoothFine(hipStream_t s, const TV* b, TV* x) {
+    smooth<TF>(s, fine_.n, fine_.lanes, fine_.row, fine_.col, fine_.val, fine_.diag, fine_.l1, &fineColor_,
+               fineLambda_, opt_.jacobiFineSweeps, opt_.mcgsFineSweeps, b, x, ftmp_.data(), ftmp_.data());
+  }
+
+  void smoothLevel(hipStream_t s, Level& A, const TV* b, TV* x) {
+    smooth<TV>(s, A.n, A.lanes, A.row.data(), A.col.data(), A.val.data(), A.diag.data(), A.l1.data(), &A.color,
+               A.lambda, opt_.jacobiCoarseSweeps, opt_.mcgsCoarseSweeps, b, x, A.tmp.data(), A.tmp.data());
+  }
+
+  void vcycle(hipStream_t s, int l, const TV* b, TV* x) {
+    Level& A = L_[(std::size_t)l];
+    if (l == (int)L_.size() - 1) {
+      hipLaunchKernelGGL(k_dense_mv<TV>, blocksFor((long long)A.n * 64), 256, 0, s, A.n, tinv_.data(), b, x);
+      NODALS_LAUNCHED();
+      return;
+    }
+    smoothLevel(s, A, b, x);
+    spmv<TV, TV, true, false>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), x, b, A.r.data());
+    Level& C = L_[(std::size_t)l + 1];
+    restrictTo(s, P_[(std::size_t)l + 1], A.r.data(), C.b.data());
+    vcycle(s, l + 1, C.b.data(), C.x.data());
+    prolongAdd(s, P_[(std::size_t)l + 1], C.x.data(), x);
+    spmv<TV, TV, true, false>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), x, b, A.r.data());
+    smoothLevel(s, A, A.r.data(), A.corr.data());
+    axpy<TV>(s, A.n, 1.0, A.corr.data(), x);
+  }
+
+  SmootherOptions opt_;
+  int fineN_ = 0;
+  double fineLambda_ = 1.0;
+  FineView<TF, TV> fine_;
+  DevColoring<TV> fineColor_;
+  std::vector<Level> L_;
+  std::vector<Xfer> P_;
+  DBuf<TV> tinv_;
+  int termN_ = 0;
+  DBuf<TV> fr_, ftmp_, fcorr_;
+};
+
+}  // namespace dev
+}  // namespace nodals

@@ -1,0 +1,380 @@
+// extern "C" entry points of include/nodals_hip.h.  Every call catches all
+// exceptions and maps them to a status code plus a thread-local message.
+#include "../../../include/nodals_hip.h"
+
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "h8_solver.hpp"
+#include "sys_solver.hpp"
+
+struct nodals_h8 {
+  std::unique_ptr<nodals::H8Solver> s;
+};
+struct nodals_sys {
+  std::unique_ptr<nodals::SysSolver> s;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_error.clear();
+    f();
+    return nodals::kOk;
+  } catch (const nodals::Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return nodals::kErrInternal;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nodals::kErrInternal;
+  } catch (...) {
+    g_error = "unknown exception";
+    return nodals::kErrInternal;
+  }
+}
+
+nodals::H8Solver& solver(nodals_h8* h) {
+  nodals::require(h != nullptr && h->s != nullptr, "null H8 handle");
+  return *h->s;
+}
+nodals::SysSolver& sys(nodals_sys* h) {
+  nodals::require(h != nullptr && h->s != nullptr, "null system handle");
+  return *h->s;
+}
+
+template <class T>
+void copyOut(const std::vector<T>& v, void* host, int64_t count) {
+  nodals::require(host != nullptr, "null host pointer");
+  nodals::require(count == (int64_t)v.size(),
+                  "count mismatch: expected " + std::to_string(v.size()) + " got " + std::to_string(count));
+  if (!v.empty()) std::memcpy(host, v.data(), v.size() * sizeof(T));
+}
+
+void fillResult(const nodals::dev::KrylovResult& r, nodals_solve_result* res) {
+  if (!res) return;
+  res->its = r.its;
+  res->ok = r.ok ? 1 : 0;
+  res->rel = r.rel;
+  res->r0 = r.r0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nodals_last_error(void) { return g_error.c_str(); }
+const char* nodals_version(void) { return "nodals_amd 0.1 (gfx950)"; }
+
+int nodals_h8_create(int argc, const char* const* argv, nodals_h8** out) {
+  return guarded([&]() {
+    nodals::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    const nodals::H8Options o = nodals::parseH8Args(argc, argv);
+    auto h = std::make_unique<nodals_h8>();
+    h->s = std::make_unique<nodals::H8Solver>(o);
+    *out = h.release();
+  });
+}
+
+int nodals_h8_create_host(int argc, const char* const* argv, nodals_h8** out) {
+  return guarded([&]() {
+    nodals::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    const nodals::H8Options o = nodals::parseH8Args(argc, argv);
+    auto h = std::make_unique<nodals_h8>();
+    h->s = std::make_unique<nodals::H8Solver>(o, false);
+    *out = h.release();
+  });
+}
+
+void nodals_h8_destroy(nodals_h8* h) {
+  try {
+    delete h;
+  } catch (...) {
+  }
+}
+
+int nodals_h8_info(nodals_h8* h, int64_t info[8]) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(info != nullptr, "null info");
+    info[0] = s.setup().nc;
+    info[1] = s.setup().nv;
+    info[2] = (int64_t)s.setup().mom.col.size();
+    info[3] = s.ncolors();
+    info[4] = (int64_t)s.fineTopo().col.size();
+    info[5] = (int64_t)s.hierarchy().A.size();
+    info[6] = s.hierarchy().terminalN;
+    info[7] = s.setup().nFixed;
+  });
+}
+
+int nodals_h8_assemble(nodals_h8* h, double alpha_u, double rau_scale) {
+  return guarded([&]() { solver(h).assemble(alpha_u, rau_scale); });
+}
+
+int nodals_h8_momentum_work(nodals_h8* h, const char* mode, double omega, int32_t outer) {
+  return guarded([&]() {
+    nodals::require(mode != nullptr, "null mode");
+    solver(h).momentumWork(mode, omega, outer);
+  });
+}
+
+int nodals_h8_momentum_audit(nodals_h8* h, double rel[3]) {
+  return guarded([&]() {
+    const auto q = solver(h).momentumAudit();
+    for (int d = 0; d < 3; ++d) rel[d] = q[(std::size_t)d];
+  });
+}
+
+int nodals_h8_continuity(nodals_h8* h, double* norm) {
+  return guarded([&]() {
+    const double n = solver(h).continuity();
+    if (norm) *norm = n;
+  });
+}
+
+int nodals_h8_pressure_refresh(nodals_h8* h) { return guarded([&]() { solver(h).refresh(); }); }
+
+int nodals_h8_pressure_solve(nodals_h8* h, const char* solverName, double rtol, double atol, int32_t maxit,
+                             nodals_solve_result* res) {
+  return guarded([&]() {
+    nodals::require(solverName != nullptr, "null solver name");
+    const auto r = solver(h).pressureSolve(solverName, rtol, atol, maxit);
+    fillResult(r, res);
+  });
+}
+
+int nodals_h8_pressure_update(nodals_h8* h, double alpha_p) {
+  return guarded([&]() { solver(h).pressureUpdate(alpha_p); });
+}
+
+int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res) {
+  return guarded([&]() {
+    const nodals::H8RunResult r = solver(h).run();
+    if (!res) return;
+    res->outer = r.outer;
+    res->converged = r.converged;
+    res->pass = r.pass;
+    res->reserved = 0;
+    res->final_rel_cont = r.finalRelCont;
+    res->avg_pressure_its = r.avgPressureIts;
+    res->pressure_drop = r.pressureDrop;
+    res->exact_pressure_drop = r.exactPressureDrop;
+    res->loop_ms = r.loopMs;
+    res->assembly_ms = r.assemblyMs;
+    res->momentum_ms = r.momentumMs;
+    res->continuity_ms = r.continuityMs;
+    res->pressure_ms = r.pressureMs;
+    res->pupdate_ms = r.pupdateMs;
+    res->setup_ms = r.setupMs;
+  });
+}
+
+int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(name != nullptr, "null name");
+    const std::string n = name;
+    if (n == "u0" || n == "u1" || n == "u2") copyOut(s.getVelocity(n[1] - '0'), host, count);
+    else if (n == "b0" || n == "b1" || n == "b2") copyOut(s.getBrhs(n[1] - '0'), host, count);
+    else if (n == "rau" || n == "delta" || n == "diag") copyOut(s.getRowField(n), host, count);
+    else if (n == "p" || n == "cont" || n == "dp" || n == "fine_diag" || n == "fine_l1") copyOut(s.getCellField(n), host, count);
+    else if (n == "mom_val") copyOut(s.getMomentumValues(), host, count);
+    else if (n == "fine_val") copyOut(s.getFineValues(), host, count);
+    else if (n == "mom_row") copyOut(s.setup().mom.row, host, count);
+    else if (n == "mom_col") copyOut(s.setup().mom.col, host, count);
+    else if (n == "mom_diff") copyOut(s.setup().mom.val, host, count);
+    else if (n == "color") copyOut(s.setup().coloring.color, host, count);
+    else if (n == "g2free") copyOut(s.setup().g2free, host, count);
+    else if (n == "fine_row") copyOut(s.fineTopo().row, host, count);
+    else if (n == "fine_col") copyOut(s.fineTopo().col, host, count);
+    else if (n == "init_rau") copyOut(s.setup().initRau, host, count);
+    else if (n == "static_rhs") {
+      std::vector<double> v;
+      for (int d = 0; d < 3; ++d) v.insert(v.end(), s.setup().staticRhs[(std::size_t)d].begin(), s.setup().staticRhs[(std::size_t)d].end());
+      copyOut(v, host, count);
+    } else if (n == "fixed_div") copyOut(s.setup().fixedDiv, host, count);
+    else if (n == "bcoef") copyOut(s.setup().bcoef, host, count);
+    else if (n == "cell_det" || n == "cell_invJ") {
+      std::vector<double> v;
+      for (const auto& cp : s.setup().cells) {
+        if (n == "cell_det") v.push_back(cp.det);
+        else v.insert(v.end(), cp.invJ, cp.invJ + 9);
+      }
+      copyOut(v, host, count);
+    } else if (n == "fixed_value") {
+      std::vector<double> v;
+      for (const auto& f : s.setup().fixedValue) v.insert(v.end(), f.begin(), f.end());
+      copyOut(v, host, count);
+    } else if (n == "pipe") {
+      const auto& P = s.setup().pipe;
+      copyOut(std::vector<double>{P.nu, P.hpDrop, P.profileScale, P.R, P.L}, host, count);
+    }
+    else if (n == "cell_ref") {
+      std::vector<int32_t> v;
+      for (const auto& cp : s.setup().cells) v.insert(v.end(), cp.ref, cp.ref + 8);
+      copyOut(v, host, count);
+    } else if (n == "cell_slot") {
+      std::vector<uint8_t> v;
+      for (const auto& cp : s.setup().cells) v.insert(v.end(), cp.rowSlot, cp.rowSlot + 64);
+      copyOut(v, host, count);
+    } else if (n.rfind("agg", 0) == 0) {
+      const std::size_t l = (std::size_t)std::stoi(n.substr(3));
+      nodals::require(l < s.hierarchy().agg.size(), "aggregate level out of range");
+      copyOut(s.hierarchy().agg[l].id, host, count);
+    } else if (n.rfind("P", 0) == 0 || n.rfind("A", 0) == 0) {
+      // P<l>_row|col|val and A<l>_row|col|val of the host hierarchy
+      const auto u = n.find('_');
+      nodals::require(u != std::string::npos, "unknown field " + n);
+      const std::size_t l = (std::size_t)std::stoi(n.substr(1, u - 1));
+      const std::string part = n.substr(u + 1);
+      const auto& H = s.hierarchy();
+      if (n[0] == 'P') {
+        nodals::require(l < H.P.size(), "transfer level out of range");
+        if (part == "row") copyOut(H.P[l].row, host, count);
+        else if (part == "col") copyOut(H.P[l].col, host, count);
+        else copyOut(H.P[l].val, host, count);
+      } else {
+        nodals::require(l < H.A.size(), "operator level out of range");
+        if (part == "row") copyOut(H.A[l].row, host, count);
+        else if (part == "col") copyOut(H.A[l].col, host, count);
+        else copyOut(H.A[l].val, host, count);
+      }
+    } else if (n == "fine_lambda") {
+      copyOut(std::vector<double>{s.hierarchy().fineLambda}, host, count);
+    } else if (n == "terminal_inv") {
+      copyOut(s.hierarchy().terminalInv, host, count);
+    } else if (n == "level_lambda") {
+      copyOut(s.hierarchy().levelLambda, host, count);
+    } else if (n == "cell_z") {
+      copyOut(s.setup().cellZ, host, count);
+    } else {
+      nodals::fail("unknown field " + n);
+    }
+  });
+}
+
+int nodals_h8_set(nodals_h8* h, const char* name, const void* host, int64_t count) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(name != nullptr && host != nullptr, "null argument");
+    const std::string n = name;
+    const double* d = static_cast<const double*>(host);
+    std::vector<double> v(d, d + count);
+    if (n == "u0" || n == "u1" || n == "u2") s.setVelocity(n[1] - '0', v);
+    else s.setCellField(n, v);
+  });
+}
+
+int nodals_h8_fine_spmv(nodals_h8* h, const double* x, double* y) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    const int nc = s.setup().nc;
+    NODALS_HIP(hipMemcpyAsync(s.devScratchA(), x, sizeof(double) * nc, hipMemcpyHostToDevice, s.stream()));
+    s.fineSpmv(s.devScratchA(), s.devScratchB());
+    NODALS_HIP(hipMemcpyAsync(y, s.devScratchB(), sizeof(double) * nc, hipMemcpyDeviceToHost, s.stream()));
+    s.sync();
+  });
+}
+
+int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double* ms_avg, double* bytes) {
+  return guarded([&]() {
+    nodals::require(kernel != nullptr, "null kernel name");
+    const double ms = solver(h).timeKernel(kernel, reps, bytes);
+    if (ms_avg) *ms_avg = ms;
+  });
+}
+
+int nodals_sys_create(int32_t n, const int64_t* row, const int32_t* col, const double* val, const char* options,
+                      nodals_sys** out) {
+  return guarded([&]() {
+    nodals::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    auto h = std::make_unique<nodals_sys>();
+    h->s = std::make_unique<nodals::SysSolver>(n, row, col, val, options ? options : "");
+    *out = h.release();
+  });
+}
+
+void nodals_sys_destroy(nodals_sys* s) {
+  try {
+    delete s;
+  } catch (...) {
+  }
+}
+
+int nodals_sys_info(nodals_sys* h, int64_t info[16]) {
+  return guarded([&]() {
+    auto& s = sys(h);
+    for (int k = 0; k < 16; ++k) info[k] = 0;
+    info[0] = s.n();
+    info[1] = s.nnz();
+    info[2] = s.levels();
+    info[3] = s.hierarchy().terminalN;
+    for (int l = 0; l < s.levels() && l < 12; ++l) info[4 + l] = s.hierarchy().A[(std::size_t)l].n;
+  });
+}
+
+int nodals_sys_spmv(nodals_sys* h, const double* x, double* y) {
+  return guarded([&]() {
+    auto& s = sys(h);
+    const auto r = s.spmvHost(std::vector<double>(x, x + s.n()));
+    std::memcpy(y, r.data(), sizeof(double) * r.size());
+  });
+}
+
+int nodals_sys_precond(nodals_sys* h, const double* r, double* z) {
+  return guarded([&]() {
+    auto& s = sys(h);
+    const auto v = s.precondHost(std::vector<double>(r, r + s.n()));
+    std::memcpy(z, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int nodals_sys_solve(nodals_sys* h, const double* b, double* x, const char* solverName, double rtol, int32_t restart,
+                     int32_t maxit, nodals_solve_result* res) {
+  return guarded([&]() {
+    auto& s = sys(h);
+    nodals::require(b && x && solverName, "null argument");
+    std::vector<double> xv;
+    const auto r = s.solve(std::vector<double>(b, b + s.n()), xv, solverName, rtol, restart, maxit);
+    std::memcpy(x, xv.data(), sizeof(double) * xv.size());
+    fillResult(r, res);
+  });
+}
+
+int nodals_sys_set_rhs(nodals_sys* h, const double* b) {
+  return guarded([&]() {
+    auto& s = sys(h);
+    s.setRhs(std::vector<double>(b, b + s.n()));
+  });
+}
+
+int nodals_sys_solve_timed(nodals_sys* h, const char* solverName, double rtol, int32_t restart, int32_t maxit,
+                           int32_t reps, double* ms_avg, nodals_solve_result* res) {
+  return guarded([&]() {
+    nodals::dev::KrylovResult r;
+    const double ms = sys(h).timedSolve(solverName, rtol, restart, maxit, reps, &r);
+    if (ms_avg) *ms_avg = ms;
+    fillResult(r, res);
+  });
+}
+
+int nodals_sys_time_spmv(nodals_sys* h, int32_t level, int32_t reps, double* ms_avg, double* bytes) {
+  return guarded([&]() {
+    const double ms = sys(h).timeSpmv(level, reps, bytes);
+    if (ms_avg) *ms_avg = ms;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+// H8 SIMPLE hot-path kernels for gfx950.
+//
+// Free velocity DOFs are renumbered on the device in colour order, so every
+// Gauss-Seidel colour is a contiguous row range and rows, right-hand sides
+// and diagonals stream coalesced.  Velocities and momentum right-hand sides
+// are stored xyz-interleaved so one gather fetches all three components.
+// Assembly is atomic-free and deterministic: cell-local 8x8 convection blocks
+// are written once per cell, and each momentum row gathers its terms in the
+// fixed (cell, basis) order recorded at setup.
+#pragma once
+
+#include "la_kernels.hpp"
+
+namespace nodals {
+namespace dev {
+
+__device__ __forceinline__ double readlaneD(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Cell-local Picard central convection block (reference kernel
+// h4_assemble_convection_only_kernel, gpu/serial_cuda/h4_kernels.cuh:10-56):
+//   ur[m][j] = sum_d u_d(m) invJ[j][d]
+//   C[a][b]  = det * sum_{m,j} ur[m][j] T[a][m][b][j]
+// One wavefront per cell, lane = 8a+b.  Each lane keeps its 24 tensor
+// entries in registers for the whole grid-stride loop; the 24 ur values are
+// broadcast with v_readlane, so the inner product runs on scalar operands.
+static __global__ void __launch_bounds__(256) k_conv_local(int nc, const int* __restrict__ cref,
+                                                    const double* __restrict__ cinvJ,
+                                                    const double* __restrict__ cdet, const double* __restrict__ u,
+                                                    const double* __restrict__ fixedVal,
+                                                    const double* __restrict__ centT, double* __restrict__ cbuf) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int nwaves = gridDim.x * 4;
+  const int a = lane >> 3, b = lane & 7;
+  double T[24];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) T[m * 3 + j] = centT[((a * 8 + m) * 8 + b) * 3 + j];
+  const int lm = lane < 24 ? lane / 3 : 0;
+  const int lj = lane < 24 ? lane % 3 : 0;
+  for (int c = wave; c < nc; c += nwaves) {
+    double cf = 0.0;
+    if (lane < 24) {
+      const int r = cref[(long long)c * 8 + lm];
+      cf = r >= 0 ? u[3ll * r + lj] : fixedVal[3ll * (-r - 1) + lj];
+    }
+    // lane 3m+j holds cf[d=j][m]; ur for lane (m,j)
+    const double c0 = __shfl(cf, 3 * lm + 0);
+    const double c1 = __shfl(cf, 3 * lm + 1);
+    const double c2 = __shfl(cf, 3 * lm + 2);
+    const double* I = cinvJ + (long long)c * 9 + 3 * lj;
+    const double ur = lane < 24 ? (c0 * I[0] + c1 * I[1] + c2 * I[2]) : 0.0;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 24; ++q) v += readlaneD(ur, q) * T[q];
+    cbuf[(long long)c * 64 + lane] = cdet[c] * v;
+  }
+}
+
+// Per momentum row: gather the assembled matrix (persistent diffusion +
+// convection terms), row-L1 under-relaxation delta = (1/alpha-1) sum|a_ik|
+// added to the diagonal, rAU = 1/a_ii (g4_finalize_relax_kernel,
+// g5e_fp32_kernels.cuh:78-85), the Dirichlet convection RHS, B^T p by the
+// velocity incidence, and the momentum RHS b = s + c + B^T p + delta u
+// (g4_momentum_rhs_kernel, g5e_fp32_kernels.cuh:97-104).
+template <int L>
+__global__ void __launch_bounds__(256) k_mom_rows(
+    int nv, const int* __restrict__ mrow, const int* __restrict__ mdiagPos, const double* __restrict__ mdiff,
+    const int* __restrict__ termOff, const int* __restrict__ termIdx, const double* __restrict__ cbuf,
+    const int* __restrict__ fixOff, const int* __restrict__ fixTerm, const double* __restrict__ fixedVal,
+    const int* __restrict__ incOff, const unsigned* __restrict__ incPacked, const double* __restrict__ bcoef,
+    const double* __restrict__ p, const double* __restrict__ srhs, const double* __restrict__ u, double fac,
+    double* __restrict__ aval, double* __restrict__ delta, double* __restrict__ diag, double* __restrict__ rau,
+    double* __restrict__ brhs) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / L;
+  const int lane = gid % L;
+  if (i >= nv) return;
+  const int rb = mrow[i], re = mrow[i + 1], dp = mdiagPos[i];
+  double sabs = 0.0, vdiag = 0.0;
+  for (int k = rb + lane; k < re; k += L) {
+    double v = mdiff[k];
+    for (int t = termOff[k]; t < termOff[k + 1]; ++t) v += cbuf[termIdx[t]];
+    aval[k] = v;
+    sabs += fabs(v);
+    if (k == dp) vdiag = v;
+  }
+  sabs = groupSum(sabs, L);
+  vdiag = groupSum(vdiag, L);
+  const double de = fac * sabs;
+  const double dg = vdiag + de;
+  double c0 = 0, c1 = 0, c2 = 0;
+  for (int t = fixOff[i] + lane; t < fixOff[i + 1]; t += L) {
+    const int q = fixTerm[2 * t], slot = fixTerm[2 * t + 1];
+    const double C = cbuf[q];
+    c0 -= C * fixedVal[3ll * slot + 0];
+    c1 -= C * fixedVal[3ll * slot + 1];
+    c2 -= C * fixedVal[3ll * slot + 2];
+  }
+  double t0 = 0, t1 = 0, t2 = 0;
+  for (int k = incOff[i] + lane; k < incOff[i + 1]; k += L) {
+    const unsigned pk = incPacked[k];
+    const int c = (int)(pk >> 3), a = (int)(pk & 7u);
+    const double pv = p[c];
+    const double* B = bcoef + (long long)c * 24 + 3 * a;
+    t0 += B[0] * pv;
+    t1 += B[1] * pv;
+    t2 += B[2] * pv;
+  }
+  c0 = groupSum(c0, L);
+  c1 = groupSum(c1, L);
+  c2 = groupSum(c2, L);
+  t0 = groupSum(t0, L);
+  t1 = groupSum(t1, L);
+  t2 = groupSum(t2, L);
+  if (dp >= rb + lane && ((dp - rb - lane) % L) == 0) aval[dp] = dg;
+  if (lane == 0) {
+    delta[i] = de;
+    diag[i] = dg;
+    rau[i] = 1.0 / dg;
+    brhs[3ll * i + 0] = srhs[3ll * i + 0] + c0 + t0 + de * u[3ll * i + 0];
+    brhs[3ll * i + 1] = srhs[3ll * i + 1] + c1 + t1 + de * u[3ll * i + 1];
+    brhs[3ll * i + 2] = srhs[3ll * i + 2] + c2 + t2 + de * u[3ll * i + 2];
+  }
+}
+
+// One colour of the three-component forward Gauss-Seidel sweep sharing A
+// (g4_mcgs_color_kernel, g5e_fp32_kernels.cuh:106-116); rows [r0, r1).
+template <int L>
+__global__ void __launch_bounds__(256) k_mom_gs(int r0, int r1, const int* __restrict__ mrow,
+                                                const int* __restrict__ mcol, const double* __restrict__ aval,
+                                                const double* __restrict__ diag, const double* __restrict__ brhs,
+                                                double* __restrict__ u, double omega) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = r0 + gid / L;
+  const int lane = gid % L;
+  if (i >= r1) return;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int k = mrow[i] + lane; k < mrow[i + 1]; k += L) {
+    const int j = mcol[k];
+    if (j == i) continue;
+    const double a = aval[k];
+    s0 += a * u[3ll * j + 0];
+    s1 += a * u[3ll * j + 1];
+    s2 += a * u[3ll * j + 2];
+  }
+  s0 = groupSum(s0, L);
+  s1 = groupSum(s1, L);
+  s2 = groupSum(s2, L);
+  if (lane == 0) {
+    const double iv = 1.0 / diag[i];
+    const double w1 = 1.0 - omega;
+    u[3ll * i + 0] = w1 * u[3ll * i + 0] + omega * ((brhs[3ll * i + 0] - s0) * iv);
+    u[3ll * i + 1] = w1 * u[3ll * i + 1] + omega * ((brhs[3ll * i + 1] - s1) * iv);
+    u[3ll * i + 2] = w1 * u[3ll * i + 2] + omega * ((brhs[3ll * i + 2] - s2) * iv);
+  }
+}
+
+// Momentum residual audit partial sums: |b_d|^2 and |b_d - (A u)_d|^2
+// (g4_mom_norms_kernel, g5e_fp32_kernels.cuh:117-127).  part[6*grid].
+template <int L>
+__global__ void __launch_bounds__(256) k_mom_norms(int nv, const int* __restrict__ mrow, const int* __restrict__ mcol,
+                                                   const double* __restrict__ aval, const double* __restrict__ brhs,
+                                                   const double* __restrict__ u, double* __restrict__ part) {
+  __shared__ double sh[4];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int groups = gridDim.x * (256 / L);
+  const int lane = threadIdx.x % L;
+  for (int i = (blockIdx.x * 256 + threadIdx.x) / L; i < nv; i += groups) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = mrow[i] + lane; k < mrow[i + 1]; k += L) {
+      const int j = mcol[k];
+      const double a = aval[k];
+      s0 += a * u[3ll * j + 0];
+      s1 += a * u[3ll * j + 1];
+      s2 += a * u[3ll * j + 2];
+    }
+    s0 = groupSum(s0, L);
+    s1 = groupSum(s1, L);
+    s2 = groupSum(s2, L);
+    if (lane == 0) {
+      const double b0 = brhs[3ll * i], b1 = brhs[3ll * i + 1], b2 = brhs[3ll * i + 2];
+      acc[0] += b0 * b0;
+      acc[1] += b1 * b1;
+      acc[2] += b2 * b2;
+      acc[3] += (b0 - s0) * (b0 - s0);
+      acc[4] += (b1 - s1) * (b1 - s1);
+      acc[5] += (b2 - s2) * (b2 - s2);
+    }
+  }
+  for (int q = 0; q < 6; ++q) {
+    const double t = blockSum256(acc[q], sh);
+    if (threadIdx.x == 0) part[(long long)q * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Continuity residual r_c = fixedDiv_c + sum_a B[c][a] . u(ref a)
+// (h8_continuity_kernel, h8_precomputed_b.cuh:140-163), written negated as
+// the pressure right-hand side, with block partials of r^2.
+static __global__ void __launch_bounds__(256) k_continuity(int nc, const int* __restrict__ cref,
+                                                    const double* __restrict__ bcoef,
+                                                    const double* __restrict__ fixedDiv, const double* __restrict__ u,
+                                                    double* __restrict__ negCont, double* __restrict__ part) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < nc; c += gridDim.x * 256) {
+    double s = fixedDiv[c];
+    const double* B = bcoef + (long long)c * 24;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int g = cref[(long long)c * 8 + a];
+      if (g < 0) continue;
+      s += B[3 * a] * u[3ll * g] + B[3 * a + 1] * u[3ll * g + 1] + B[3 * a + 2] * u[3ll * g + 2];
+    }
+    negCont[c] = -s;
+    acc += s * s;
+  }
+  acc = blockSum256(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Fine P0 Schur numeric refresh, one nonzero per lane:
+//   val[k] = sum_{shared free DOF g} rAU_g * (B[c][a] . B[c2][b])
+// (h8_fine_csr_refresh_warp_kernel, h8_precomputed_b.cuh:37-91, restated as
+// a gather over the terms recorded at setup; no atomics).  Also writes the
+// diagonal, optional lower-precision copies and the row-L1 sums.
+template <int L, bool F32, bool B16, bool L1>
+__global__ void __launch_bounds__(256) k_fine_refresh(
+    int nc, const int* __restrict__ frow, const int* __restrict__ fcol, const int* __restrict__ fdiagPos,
+    const int* __restrict__ nzOff, const unsigned* __restrict__ nzTerm, const double* __restrict__ bcoef,
+    const double* __restrict__ rau, double* __restrict__ val, float* __restrict__ valf, bf16* __restrict__ valb,
+    double* __restrict__ diag, float* __restrict__ diagf, double* __restrict__ l1d, float* __restrict__ l1f) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int c = gid / L;
+  const int lane = gid % L;
+  if (c >= nc) return;
+  const int rb = frow[c], re = frow[c + 1], dp = fdiagPos[c];
+  const double* Bc = bcoef + (long long)c * 24;
+  double sabs = 0.0;
+  for (int k = rb + lane; k < re; k += L) {
+    const int c2 = fcol[k];
+    const double* Bn = bcoef + (long long)c2 * 24;
+    double v = 0.0;
+    for (int t = nzOff[k]; t < nzOff[k + 1]; ++t) {
+      const unsigned q = nzTerm[t];
+      const int g = (int)(q >> 6), a = (int)((q >> 3) & 7u), b = (int)(q & 7u);
+      const double dot = Bc[3 * a] * Bn[3 * b] + Bc[3 * a + 1] * Bn[3 * b + 1] + Bc[3 * a + 2] * Bn[3 * b + 2];
+      v += rau[g] * dot;
+    }
+    val[k] = v;
+    if (F32) valf[k] = (float)v;
+    if (B16) valb[k] = floatToBf16((float)v);
+    if (L1) sabs += fabs(v);
+    if (k == dp) {
+      diag[c] = v;
+      if (F32 || B16) diagf[c] = (float)v;
+    }
+  }
+  if (L1) {
+    sabs = groupSum(sabs, L);
+    if (lane == 0) {
+      l1d[c] = sabs;
+      if (F32 || B16) l1f[c] = (float)sabs;
+    }
+  }
+}
+
+}  // namespace dev
+}  // namespace nodals

@@ -1,0 +1,923 @@
+// H8 solver context: setup upload, SIMPLE stages, pressure Krylov + AMG.
+#include "h8_solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+
+#include "h8_kernels.hpp"
+
+namespace nodals {
+
+using namespace dev;
+
+// ------------------------------------------------------------------ options
+H8Options parseH8Args(int argc, const char* const* argv) {
+  H8Options o;
+  auto num = [](const char* s) { return std::atof(s); };
+  auto inum = [](const char* s) { return std::atoi(s); };
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    const bool hasv = i + 1 < argc;
+    auto next = [&]() { return argv[++i]; };
+    if (!hasv) fail("H8 usage error: missing value for " + a);
+    if (a == "--mesh") o.mesh = next();
+    else if (a == "--tag") o.tag = next();
+    else if (a == "--vtu-out") o.vtuOut = next();
+    else if (a == "--wall") o.wall = next();
+    else if (a == "--inlet") o.inlet = next();
+    else if (a == "--outlet") o.outlet = next();
+    else if (a == "--re") o.re = num(next());
+    else if (a == "--bulk") o.bulk = num(next());
+    else if (a == "--mixing-length") o.mixingLength = inum(next()) != 0;
+    else if (a == "--mixlen-scale") o.mixlenScale = num(next());
+    else if (a == "--dg-inlet") o.dgInlet = inum(next()) != 0;
+    else if (a == "--weak-wall") o.weakWall = inum(next()) != 0;
+    else if (a == "--wall-traction-mode") o.wallTractionMode = next();
+    else if (a == "--wall-reference-f") o.wallReferenceF = num(next());
+    else if (a == "--wall-diag-csv") o.wallDiagCsv = next();
+    else if (a == "--supg") o.supg = inum(next()) != 0;
+    else if (a == "--supg-tau-scale") o.supgTauScale = num(next());
+    else if (a == "--supg-magic") o.supgMagic = num(next());
+    else if (a == "--supg-form") o.supgForm = next();
+    else if (a == "--supg-quad-points") o.supgQuadPoints = inum(next());
+    else if (a == "--simple-tol") o.simpleTol = num(next());
+    else if (a == "--max-outer") o.maxOuter = inum(next());
+    else if (a == "--alpha-u") o.alphaU = num(next());
+    else if (a == "--alpha-p") o.alphaP = num(next());
+    else if (a == "--rau-scale") o.rauScale = num(next());
+    else if (a == "--mom-rtol") o.momRtol = num(next());
+    else if (a == "--mom-atol") o.momAtol = num(next());
+    else if (a == "--mom-drop") o.momDrop = num(next());
+    else if (a == "--mom-omega") o.momOmega = num(next());
+    else if (a == "--mom-max") o.momMax = inum(next());
+    else if (a == "--p-rtol") o.pRtol = num(next());
+    else if (a == "--p-atol") o.pAtol = num(next());
+    else if (a == "--p-max") o.pMax = inum(next());
+    else if (a == "--snapshot-tol") o.snapshotTol = num(next());
+    else if (a == "--fine-csr-refresh-every") o.fineCsrRefreshEvery = inum(next());
+    else if (a == "--amg-hierarchy") o.amgHierarchy = next();
+    else if (a == "--cf-theta") o.cfTheta = num(next());
+    else if (a == "--cf-pmax") o.cfPmax = inum(next());
+    else if (a == "--cf-interp") o.cfInterp = next();
+    else if (a == "--cf-coarsening") o.cfCoarsening = next();
+    else if (a == "--cf-strength") o.cfStrength = next();
+    else if (a == "--cf-aggressive-first") o.cfAggressiveFirst = inum(next());
+    else if (a == "--amg-terminal") o.amgTerminal = inum(next());
+    else if (a == "--amg-spectrum-policy") o.amgSpectrumPolicy = next();
+    else if (a == "--amg-power-its") o.amgPowerIts = inum(next());
+    else if (a == "--amg-cheb-degree") o.amgChebDegree = inum(next());
+    else if (a == "--amg-lambda-safety") o.amgLambdaSafety = num(next());
+    else if (a == "--amg-lambda-low-fraction") o.amgLambdaLowFraction = num(next());
+    else if (a == "--pressure-solver") o.pressureSolver = next();
+    else if (a == "--pressure-richardson-omega") o.pressureRichardsonOmega = num(next());
+    else if (a == "--pressure-cheb-degree") o.pressureChebDegree = inum(next());
+    else if (a == "--pressure-power-its") o.pressurePowerIts = inum(next());
+    else if (a == "--pressure-cheb-low-fraction") o.pressureChebLowFraction = num(next());
+    else if (a == "--pressure-power-safety") o.pressurePowerSafety = num(next());
+    else if (a == "--amg-smoother") o.amgSmoother = next();
+    else if (a == "--amg-jacobi-omega") o.amgJacobiOmega = num(next());
+    else if (a == "--amg-jacobi-fine-sweeps") o.amgJacobiFineSweeps = inum(next());
+    else if (a == "--amg-jacobi-coarse-sweeps") o.amgJacobiCoarseSweeps = inum(next());
+    else if (a == "--amg-mcgs-omega") o.amgMcgsOmega = num(next());
+    else if (a == "--amg-mcgs-fine-sweeps") o.amgMcgsFineSweeps = inum(next());
+    else if (a == "--amg-mcgs-coarse-sweeps") o.amgMcgsCoarseSweeps = inum(next());
+    else if (a == "--amg-mcgs-order") o.amgMcgsOrder = next();
+    else if (a == "--momentum-work") o.momentumWork = next();
+    else if (a == "--run-mode") o.runMode = next();
+    else if (a == "--amg-precision") o.amgPrecision = next();
+    else if (a == "--pressure-restart") o.pressureRestart = inum(next());
+    else if (a == "--verbose") o.verbose = inum(next());
+    else fail("H8 usage error: unknown option " + a);
+  }
+  require(!o.mesh.empty(), "--mesh required");
+  require(o.simpleTol > 0.0 && o.maxOuter >= 1, "H8 invalid SIMPLE controls");
+  require(o.alphaU > 0.0 && o.alphaU <= 1.0 && o.alphaP > 0.0 && o.alphaP <= 1.0, "H8 alpha-u/alpha-p must be in (0,1]");
+  require(o.rauScale > 0.0 && std::isfinite(o.rauScale), "H8 rau-scale must be finite and > 0");
+  require(o.momRtol >= 0.0 && o.momAtol >= 0.0 && o.momDrop >= 0.0 && o.momMax >= 1 && o.momOmega > 0.0,
+          "H8 invalid momentum controls");
+  require(o.pRtol >= 0.0 && o.pAtol >= 0.0 && o.pMax >= 1, "H8 invalid pressure controls");
+  require(o.amgHierarchy == "sa" || o.amgHierarchy == "cf", "H8 amg-hierarchy must be sa or cf");
+  require(o.cfTheta > 0.0 && o.cfTheta <= 1.0, "H8 cf-theta must be in (0,1]");
+  require(o.cfPmax >= 1 && o.cfPmax <= 16, "H8 cf-pmax must be in [1,16]");
+  require(o.cfInterp == "direct" || o.cfInterp == "exti", "H8 cf-interp must be direct or exti");
+  require(o.cfCoarsening == "pmis", "H8 cf-coarsening currently supports pmis");
+  require(o.cfStrength == "classical-negative", "H8 cf-strength currently supports classical-negative");
+  require(o.cfAggressiveFirst == 0 || o.cfAggressiveFirst == 1, "H8 cf-aggressive-first must be 0 or 1");
+  require(o.amgTerminal >= 16, "H8 amg-terminal must be >=16");
+  require(o.amgSpectrumPolicy == "auto" || o.amgSpectrumPolicy == "always" || o.amgSpectrumPolicy == "off",
+          "H8 amg-spectrum-policy must be auto, always or off");
+  require(o.amgPowerIts >= 1 && o.amgPowerIts <= 64, "H8 amg-power-its must be in [1,64]");
+  require(o.amgChebDegree >= 1 && o.amgChebDegree <= 16, "H8 amg-cheb-degree must be in [1,16]");
+  require(o.amgLambdaSafety > 1.0 && o.amgLambdaSafety <= 3.0, "H8 amg-lambda-safety must be in (1,3]");
+  require(o.amgLambdaLowFraction > 0.0 && o.amgLambdaLowFraction < 1.0, "H8 amg-lambda-low-fraction must be in (0,1)");
+  require(o.pressureSolver == "pcg" || o.pressureSolver == "richardson" || o.pressureSolver == "cheb" ||
+              o.pressureSolver == "fgmres",
+          "pressure-solver must be pcg, richardson, cheb or fgmres");
+  require(o.pressureRichardsonOmega > 0.0 && o.pressureRichardsonOmega < 2.0, "Richardson omega must be in (0,2)");
+  require(o.pressureChebDegree >= 1 && o.pressureChebDegree <= 16, "Cheb degree must be in [1,16]");
+  require(o.pressurePowerIts >= 1 && o.pressurePowerIts <= 64, "pressure power its must be in [1,64]");
+  require(o.pressureChebLowFraction > 0.0 && o.pressureChebLowFraction < 1.0, "Cheb low fraction must be in (0,1)");
+  require(o.pressurePowerSafety > 1.0 && o.pressurePowerSafety <= 2.0, "pressure power safety must be in (1,2]");
+  require(o.amgSmoother == "cheb2" || o.amgSmoother == "jacobi" || o.amgSmoother == "l1jacobi" ||
+              o.amgSmoother == "mcgs",
+          "H8 amg-smoother must be cheb2, jacobi, l1jacobi or mcgs");
+  require(o.amgJacobiOmega > 0.0 && o.amgJacobiOmega <= 2.0, "H8 amg-jacobi-omega must be in (0,2]");
+  require(o.amgJacobiFineSweeps >= 1 && o.amgJacobiFineSweeps <= 16 && o.amgJacobiCoarseSweeps >= 1 &&
+              o.amgJacobiCoarseSweeps <= 16,
+          "H8 Jacobi sweeps must be in [1,16]");
+  require(o.amgMcgsOmega > 0.0 && o.amgMcgsOmega < 2.0, "H8 amg-mcgs-omega must be in (0,2)");
+  require(o.amgMcgsFineSweeps >= 1 && o.amgMcgsFineSweeps <= 16 && o.amgMcgsCoarseSweeps >= 1 &&
+              o.amgMcgsCoarseSweeps <= 16,
+          "H8 MCGS sweeps must be in [1,16]");
+  require(o.amgMcgsOrder == "symmetric" || o.amgMcgsOrder == "forward" || o.amgMcgsOrder == "backward",
+          "H8 amg-mcgs-order must be symmetric, forward or backward");
+  require(!(o.pressureSolver == "pcg" && o.amgSmoother == "mcgs" && o.amgMcgsOrder != "symmetric"),
+          "H8 PCG requires symmetric MCGS");
+  require(o.snapshotTol > 0.0, "H8 snapshot tolerance must be positive");
+  require(o.fineCsrRefreshEvery == 1, "H8 requires fine CSR refreshEvery=1");
+  require(o.momentumWork == "fgs1" || o.momentumWork == "altgs1" || o.momentumWork == "sgs1",
+          "H8 momentum-work must be fgs1, altgs1 or sgs1");
+  require(o.runMode == "fixed10" || o.runMode == "converge", "H8 run-mode must be fixed10 or converge");
+  require(!(o.runMode == "fixed10" && o.maxOuter != 10), "H8 fixed10 requires maxOuter=10");
+  require(o.amgPrecision == "fp64" || o.amgPrecision == "fp32" || o.amgPrecision == "bf16",
+          "amg-precision must be fp64, fp32 or bf16");
+  require(o.pressureRestart >= 1 && o.pressureRestart <= 31, "pressure-restart must be in [1,31]");
+  // Features of the reference H8 binary that this build does not implement.
+  require(!o.mixingLength, "--mixing-length 1 is not supported by this build");
+  require(!o.dgInlet, "--dg-inlet 1 is not supported by this build");
+  require(!o.weakWall, "--weak-wall 1 is not supported by this build");
+  require(!o.supg, "--supg 1 is not supported by this build");
+  require(o.amgHierarchy == "sa", "--amg-hierarchy cf is not supported by this build; use sa");
+  return o;
+}
+
+// ------------------------------------------------------------------ setup
+H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
+  const auto t0 = std::chrono::steady_clock::now();
+  M_ = readPolyMesh(opt_.mesh);
+  H8Params prm;
+  prm.wall = opt_.wall;
+  prm.inlet = opt_.inlet;
+  prm.outlet = opt_.outlet;
+  prm.re = opt_.re;
+  prm.bulk = opt_.bulk;
+  prm.alphaU = opt_.alphaU;
+  prm.rauScale = opt_.rauScale;
+  S_ = buildH8Setup(M_, prm);
+  F_ = buildFineTopo(S_);
+  int powerIts = 0;
+  if (opt_.amgSpectrumPolicy == "always") powerIts = opt_.amgPowerIts;
+  else if (opt_.amgSpectrumPolicy == "auto" && opt_.amgSmoother == "cheb2") powerIts = opt_.amgPowerIts;
+  require(!(opt_.amgSmoother == "cheb2" && powerIts <= 0), "Chebyshev AMG smoother requires spectrum estimation");
+  AmgOptions ao;
+  ao.terminal = opt_.amgTerminal;
+  ao.powerIts = powerIts;
+  ao.safety = opt_.amgLambdaSafety;
+  ao.lowFraction = opt_.amgLambdaLowFraction;
+  H_ = buildSaHierarchy(M_, S_, F_, S_.initRau, ao);
+  nc_ = S_.nc;
+  nv_ = S_.nv;
+  if (!withDevice) {
+    setupMs_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return;
+  }
+  NODALS_HIP(hipSetDevice(0));
+  NODALS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  red_.init(32);
+  upload();
+  setupAmg();
+  if (opt_.pressureSolver == "cheb") {
+    refresh();
+    chebLambda_ = pressurePrecondPower(opt_.pressurePowerIts, opt_.pressurePowerSafety);
+  }
+  sync();
+  setupMs_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+H8Solver::~H8Solver() {
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void H8Solver::sync() {
+  require(stream_ != nullptr, "H8 context was created without a device");
+  NODALS_HIP(hipStreamSynchronize(stream_));
+}
+
+void H8Solver::upload() {
+  nc_ = S_.nc;
+  nv_ = S_.nv;
+  nFix_ = S_.nFixed;
+  const CsrD& A = S_.mom;
+  momNnz_ = (long long)A.col.size();
+  fineNnz_ = (long long)F_.col.size();
+  require(momNnz_ < (1ll << 31) && fineNnz_ < (1ll << 31), "matrix too large for int32 device indices");
+  require((long long)nc_ * 64 < (1ll << 31), "too many cells for the convection buffer index");
+  // colour-ordered numbering
+  perm_ = S_.coloring.rows;
+  pinv_.assign((std::size_t)nv_, -1);
+  for (int i = 0; i < nv_; ++i) pinv_[(std::size_t)perm_[(std::size_t)i]] = i;
+  colorOff_ = S_.coloring.offset;
+  auto mapRef = [&](int r) { return r >= 0 ? pinv_[(std::size_t)r] : r; };
+
+  // cells
+  std::vector<int> cref((std::size_t)nc_ * 8);
+  std::vector<double> invJ((std::size_t)nc_ * 9), det((std::size_t)nc_);
+  for (int c = 0; c < nc_; ++c) {
+    const CellPlan& cp = S_.cells[(std::size_t)c];
+    for (int a = 0; a < 8; ++a) cref[(std::size_t)c * 8 + a] = mapRef(cp.ref[a]);
+    for (int k = 0; k < 9; ++k) invJ[(std::size_t)c * 9 + k] = cp.invJ[k];
+    det[(std::size_t)c] = cp.det;
+  }
+  cref_.upload(cref);
+  cinvJ_.upload(invJ);
+  cdet_.upload(det);
+  bcoef_.upload(S_.bcoef);
+  std::vector<double> fv((std::size_t)nFix_ * 3);
+  for (int s = 0; s < nFix_; ++s)
+    for (int d = 0; d < 3; ++d) fv[(std::size_t)s * 3 + d] = S_.fixedValue[(std::size_t)s][(std::size_t)d];
+  if (fv.empty()) fv.push_back(0.0);
+  fixedVal_.upload(fv);
+  const Tensors& T = referenceTensors();
+  std::vector<double> ct(&T.cent[0][0][0][0], &T.cent[0][0][0][0] + 8 * 8 * 8 * 3);
+  centT_.upload(ct);
+
+  // momentum CSR in colour order; row entries keep the reference order
+  std::vector<int> mrow((std::size_t)nv_ + 1, 0), mcol((std::size_t)momNnz_), mdp((std::size_t)nv_);
+  std::vector<double> mdiff((std::size_t)momNnz_);
+  std::vector<long long> oldStart((std::size_t)nv_);
+  for (int i = 0; i < nv_; ++i) {
+    const int r = perm_[(std::size_t)i];
+    mrow[(std::size_t)i + 1] = mrow[(std::size_t)i] + (int)(A.row[(std::size_t)r + 1] - A.row[(std::size_t)r]);
+  }
+  std::vector<long long> newOfOld((std::size_t)momNnz_);
+  for (int i = 0; i < nv_; ++i) {
+    const int r = perm_[(std::size_t)i];
+    const long long b = A.row[(std::size_t)r];
+    const long long len = A.row[(std::size_t)r + 1] - b;
+    for (long long k = 0; k < len; ++k) {
+      const long long o = b + k, nk = mrow[(std::size_t)i] + k;
+      newOfOld[(std::size_t)o] = nk;
+      mcol[(std::size_t)nk] = pinv_[(std::size_t)A.col[(std::size_t)o]];
+      mdiff[(std::size_t)nk] = A.val[(std::size_t)o];
+    }
+    mdp[(std::size_t)i] = (int)newOfOld[(std::size_t)S_.momDiagPos[(std::size_t)r]];
+  }
+  mrow_.upload(mrow);
+  mcol_.upload(mcol);
+  mdiagPos_.upload(mdp);
+  mdiff_.upload(mdiff);
+  aval_.alloc((std::size_t)momNnz_);
+  momLanes_ = lanesFor((double)momNnz_ / std::max(nv_, 1));
+
+  // assembly gather terms (cell order) and Dirichlet convection terms
+  std::vector<int> tcount((std::size_t)momNnz_ + 1, 0), fcount((std::size_t)nv_ + 1, 0);
+  for (int c = 0; c < nc_; ++c) {
+    const CellPlan& cp = S_.cells[(std::size_t)c];
+    for (int a = 0; a < 8; ++a) {
+      const int r = cp.ref[a];
+      if (r < 0) continue;
+      for (int b = 0; b < 8; ++b) {
+        if (cp.ref[b] >= 0) ++tcount[(std::size_t)newOfOld[(std::size_t)(A.row[(std::size_t)r] + cp.rowSlot[8 * a + b])] + 1];
+        else ++fcount[(std::size_t)pinv_[(std::size_t)r] + 1];
+      }
+    }
+  }
+  for (long long k = 0; k < momNnz_; ++k) tcount[(std::size_t)k + 1] += tcount[(std::size_t)k];
+  for (int i = 0; i < nv_; ++i) fcount[(std::size_t)i + 1] += fcount[(std::size_t)i];
+  std::vector<int> tidx((std::size_t)tcount[(std::size_t)momNnz_]), fterm(2 * (std::size_t)fcount[(std::size_t)nv_] + 2);
+  {
+    std::vector<int> tc(tcount.begin(), tcount.end() - 1), fc(fcount.begin(), fcount.end() - 1);
+    for (int c = 0; c < nc_; ++c) {
+      const CellPlan& cp = S_.cells[(std::size_t)c];
+      for (int a = 0; a < 8; ++a) {
+        const int r = cp.ref[a];
+        if (r < 0) continue;
+        for (int b = 0; b < 8; ++b) {
+          const int q = c * 64 + a * 8 + b;
+          if (cp.ref[b] >= 0) {
+            const long long nk = newOfOld[(std::size_t)(A.row[(std::size_t)r] + cp.rowSlot[8 * a + b])];
+            tidx[(std::size_t)tc[(std::size_t)nk]++] = q;
+          } else {
+            const int i = pinv_[(std::size_t)r];
+            const int z = fc[(std::size_t)i]++;
+            fterm[2 * (std::size_t)z] = q;
+            fterm[2 * (std::size_t)z + 1] = -cp.ref[b] - 1;
+          }
+        }
+      }
+    }
+  }
+  termOff_.upload(tcount);
+  if (tidx.empty()) tidx.push_back(0);
+  termIdx_.upload(tidx);
+  fixOff_.upload(fcount);
+  fixTerm_.upload(fterm);
+  cbuf_.alloc((std::size_t)nc_ * 64);
+
+  // velocity incidence (B^T p gather) in colour numbering
+  std::vector<int> ioff((std::size_t)nv_ + 1, 0);
+  std::vector<unsigned> ipk;
+  ipk.reserve(F_.packed.size());
+  for (int i = 0; i < nv_; ++i) {
+    const int g = perm_[(std::size_t)i];
+    for (int k = F_.incOff[(std::size_t)g]; k < F_.incOff[(std::size_t)g + 1]; ++k) ipk.push_back(F_.packed[(std::size_t)k]);
+    ioff[(std::size_t)i + 1] = (int)ipk.size();
+  }
+  incOff_.upload(ioff);
+  incPacked_.upload(ipk);
+
+  // state vectors
+  auto aos = [&](const std::array<std::vector<double>, 3>& v) {
+    std::vector<double> o((std::size_t)nv_ * 3);
+    for (int i = 0; i < nv_; ++i)
+      for (int d = 0; d < 3; ++d) o[(std::size_t)i * 3 + d] = v[(std::size_t)d][(std::size_t)perm_[(std::size_t)i]];
+    return o;
+  };
+  srhs_.upload(aos(S_.staticRhs));
+  u_.alloc((std::size_t)nv_ * 3);
+  fill<double>(stream_, (long long)nv_ * 3, u_.data(), 0.0);
+  brhs_.alloc((std::size_t)nv_ * 3);
+  for (DBuf<double>* b : {&delta_, &diag_, &rau_}) b->alloc((std::size_t)nv_);
+  std::vector<double> rs((std::size_t)nv_);
+  for (int i = 0; i < nv_; ++i) rs[(std::size_t)i] = S_.initRau[(std::size_t)perm_[(std::size_t)i]];
+  rauSnap_.upload(rs);
+  rau_.upload(rs);
+  p_.alloc((std::size_t)nc_);
+  fill<double>(stream_, nc_, p_.data(), 0.0);
+  fixedDiv_.upload(S_.fixedDiv);
+  cont_.alloc((std::size_t)nc_);
+  pcx_.alloc((std::size_t)nc_);
+  tmpA_.alloc((std::size_t)std::max(nc_, nv_ * 3));
+  tmpB_.alloc((std::size_t)std::max(nc_, nv_ * 3));
+
+  // fine Schur CSR (cell rows unchanged; velocity ids renumbered)
+  frow_.upload(F_.row);
+  fcol_.upload(F_.col);
+  fdiagPos_.upload(F_.diagPos);
+  nzOff_.upload(F_.nzOff);
+  std::vector<unsigned> nt(F_.nzTerm.size());
+  for (std::size_t t = 0; t < nt.size(); ++t) {
+    const unsigned q = F_.nzTerm[t];
+    nt[t] = ((unsigned)pinv_[(std::size_t)(q >> 6)] << 6) | (q & 63u);
+  }
+  nzTerm_.upload(nt);
+  fval_.alloc((std::size_t)fineNnz_);
+  fdiag_.alloc((std::size_t)nc_);
+  fl1_.alloc((std::size_t)nc_);
+  fineLanes_ = lanesFor((double)fineNnz_ / std::max(nc_, 1));
+  if (opt_.amgPrecision != "fp64") {
+    fdiagf_.alloc((std::size_t)nc_);
+    fl1f_.alloc((std::size_t)nc_);
+    rf_.alloc((std::size_t)nc_);
+    zf_.alloc((std::size_t)nc_);
+    if (opt_.amgPrecision == "fp32") fvalf_.alloc((std::size_t)fineNnz_);
+    else fvalb_.alloc((std::size_t)fineNnz_);
+  }
+  sync();
+}
+
+void H8Solver::setupAmg() {
+  SmootherOptions so;
+  so.kind = opt_.amgSmoother;
+  so.jacobiOmega = opt_.amgJacobiOmega;
+  so.jacobiFineSweeps = opt_.amgJacobiFineSweeps;
+  so.jacobiCoarseSweeps = opt_.amgJacobiCoarseSweeps;
+  so.chebDegree = opt_.amgChebDegree;
+  so.lambdaLowFraction = opt_.amgLambdaLowFraction;
+  so.mcgsOmega = opt_.amgMcgsOmega;
+  so.mcgsFineSweeps = opt_.amgMcgsFineSweeps;
+  so.mcgsCoarseSweeps = opt_.amgMcgsCoarseSweeps;
+  so.mcgsOrder = opt_.amgMcgsOrder;
+  Coloring fineColor;
+  if (so.kind == "mcgs") fineColor = greedyColoring32(nc_, F_.row.data(), F_.col.data());
+  const Coloring* fc = so.kind == "mcgs" ? &fineColor : nullptr;
+  if (opt_.amgPrecision == "fp64") {
+    amg64_ = std::make_unique<DevAmg<double, double>>();
+    amg64_->setup(H_, nc_, so, fc);
+    FineView<double, double> v;
+    v.n = nc_;
+    v.lanes = fineLanes_;
+    v.row = frow_.data();
+    v.col = fcol_.data();
+    v.val = fval_.data();
+    v.diag = fdiag_.data();
+    v.l1 = fl1_.data();
+    amg64_->setFine(v);
+  } else if (opt_.amgPrecision == "fp32") {
+    amg32_ = std::make_unique<DevAmg<float, float>>();
+    amg32_->setup(H_, nc_, so, fc);
+    FineView<float, float> v;
+    v.n = nc_;
+    v.lanes = fineLanes_;
+    v.row = frow_.data();
+    v.col = fcol_.data();
+    v.val = fvalf_.data();
+    v.diag = fdiagf_.data();
+    v.l1 = fl1f_.data();
+    amg32_->setFine(v);
+  } else {
+    amgb_ = std::make_unique<DevAmg<bf16, float>>();
+    amgb_->setup(H_, nc_, so, fc);
+    FineView<bf16, float> v;
+    v.n = nc_;
+    v.lanes = fineLanes_;
+    v.row = frow_.data();
+    v.col = fcol_.data();
+    v.val = fvalb_.data();
+    v.diag = fdiagf_.data();
+    v.l1 = fl1f_.data();
+    amgb_->setFine(v);
+  }
+}
+
+// ------------------------------------------------------------------ stages
+void H8Solver::assemble(double alphaU, double rauScale) {
+  requireDevice();
+  require(alphaU > 0.0 && alphaU <= 1.0, "alpha-u must be in (0,1]");
+  const int waves = std::min(nc_, 8192);
+  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, stream_, nc_, cref_.data(), cinvJ_.data(),
+                     cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), cbuf_.data());
+  NODALS_LAUNCHED();
+  const double fac = 1.0 / alphaU - 1.0;
+  const int grid = blocksFor((long long)nv_ * momLanes_);
+#define NODALS_MOM_ROWS(L)                                                                                           \
+  hipLaunchKernelGGL(k_mom_rows<L>, grid, 256, 0, stream_, nv_, mrow_.data(), mdiagPos_.data(), mdiff_.data(),     \
+                     termOff_.data(), termIdx_.data(), cbuf_.data(), fixOff_.data(), fixTerm_.data(),               \
+                     fixedVal_.data(), incOff_.data(), incPacked_.data(), bcoef_.data(), p_.data(), srhs_.data(),   \
+                     u_.data(), fac, aval_.data(), delta_.data(), diag_.data(), rau_.data(), brhs_.data())
+  switch (momLanes_) {
+    case 4: NODALS_MOM_ROWS(4); break;
+    case 8: NODALS_MOM_ROWS(8); break;
+    case 16: NODALS_MOM_ROWS(16); break;
+    case 32: NODALS_MOM_ROWS(32); break;
+    default: NODALS_MOM_ROWS(64); break;
+  }
+#undef NODALS_MOM_ROWS
+  NODALS_LAUNCHED();
+  if (rauScale != 1.0) {
+    hipLaunchKernelGGL(k_scale_into<double>, blocksFor(nv_), 256, 0, stream_, (long long)nv_, rauScale, rau_.data(),
+                       rau_.data());
+    NODALS_LAUNCHED();
+  }
+}
+
+const char* H8Solver::momentumWork(const std::string& mode, double omega, int outer) {
+  requireDevice();
+  auto sweep = [&](bool reverse) {
+    const int nco = (int)colorOff_.size() - 1;
+    for (int q = 0; q < nco; ++q) {
+      const int c = reverse ? nco - 1 - q : q;
+      const int r0 = colorOff_[(std::size_t)c], r1 = colorOff_[(std::size_t)c + 1];
+      if (r1 <= r0) continue;
+      const int grid = blocksFor((long long)(r1 - r0) * momLanes_);
+#define NODALS_GS(L)                                                                                                   \
+  hipLaunchKernelGGL(k_mom_gs<L>, grid, 256, 0, stream_, r0, r1, mrow_.data(), mcol_.data(), aval_.data(), diag_.data(), \
+                     brhs_.data(), u_.data(), omega)
+      switch (momLanes_) {
+        case 4: NODALS_GS(4); break;
+        case 8: NODALS_GS(8); break;
+        case 16: NODALS_GS(16); break;
+        case 32: NODALS_GS(32); break;
+        default: NODALS_GS(64); break;
+      }
+#undef NODALS_GS
+      NODALS_LAUNCHED();
+    }
+  };
+  if (mode == "fgs1") {
+    sweep(false);
+    return "FWD";
+  }
+  if (mode == "altgs1") {
+    if (outer & 1) {
+      sweep(false);
+      return "FWD";
+    }
+    sweep(true);
+    return "BWD";
+  }
+  if (mode == "sgs1") {
+    sweep(false);
+    sweep(true);
+    return "FWD+BWD";
+  }
+  fail("unknown fixed momentum work mode " + mode);
+}
+
+std::array<double, 3> H8Solver::momentumAudit() {
+  requireDevice();
+  double* part = red_.partials();
+  const int grid = kReduceParts;
+#define NODALS_NORMS(L)                                                                                              \
+  hipLaunchKernelGGL(k_mom_norms<L>, grid, 256, 0, stream_, nv_, mrow_.data(), mcol_.data(), aval_.data(),          \
+                     brhs_.data(), u_.data(), part)
+  switch (momLanes_) {
+    case 4: NODALS_NORMS(4); break;
+    case 8: NODALS_NORMS(8); break;
+    case 16: NODALS_NORMS(16); break;
+    case 32: NODALS_NORMS(32); break;
+    default: NODALS_NORMS(64); break;
+  }
+#undef NODALS_NORMS
+  NODALS_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce_final, 6, 256, 0, stream_, grid, part, red_.hostOut());
+  NODALS_LAUNCHED();
+  sync();
+  const double* h = red_.hostOut();
+  std::array<double, 3> q{};
+  for (int d = 0; d < 3; ++d) {
+    double bn = std::sqrt(std::max(0.0, h[d]));
+    if (bn == 0.0) bn = 1.0;
+    q[(std::size_t)d] = std::sqrt(std::max(0.0, h[3 + d])) / bn;
+  }
+  return q;
+}
+
+double H8Solver::continuity() {
+  requireDevice();
+  double* part = red_.partials();
+  hipLaunchKernelGGL(k_continuity, kReduceParts, 256, 0, stream_, nc_, cref_.data(), bcoef_.data(), fixedDiv_.data(),
+                     u_.data(), cont_.data(), part);
+  NODALS_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce_final, 1, 256, 0, stream_, kReduceParts, part, red_.hostOut());
+  NODALS_LAUNCHED();
+  sync();
+  return std::sqrt(red_.hostOut()[0]);
+}
+
+void H8Solver::refresh() {
+  requireDevice();
+  const int grid = blocksFor((long long)nc_ * fineLanes_);
+  const bool f32 = opt_.amgPrecision == "fp32", b16 = opt_.amgPrecision == "bf16";
+  const bool l1 = opt_.amgSmoother == "l1jacobi";
+#define NODALS_REFRESH(L, F, B, LL)                                                                                  \
+  hipLaunchKernelGGL((k_fine_refresh<L, F, B, LL>), grid, 256, 0, stream_, nc_, frow_.data(), fcol_.data(),         \
+                     fdiagPos_.data(), nzOff_.data(), nzTerm_.data(), bcoef_.data(), rau_.data(), fval_.data(),     \
+                     fvalf_.data(), fvalb_.data(), fdiag_.data(), fdiagf_.data(), fl1_.data(), fl1f_.data())
+#define NODALS_REFRESH_P(L)                         \
+  if (f32) {                                        \
+    if (l1) NODALS_REFRESH(L, true, false, true);   \
+    else NODALS_REFRESH(L, true, false, false);     \
+  } else if (b16) {                                 \
+    if (l1) NODALS_REFRESH(L, false, true, true);   \
+    else NODALS_REFRESH(L, false, true, false);     \
+  } else {                                          \
+    if (l1) NODALS_REFRESH(L, false, false, true);  \
+    else NODALS_REFRESH(L, false, false, false);    \
+  }
+  switch (fineLanes_) {
+    case 4: NODALS_REFRESH_P(4); break;
+    case 8: NODALS_REFRESH_P(8); break;
+    case 16: NODALS_REFRESH_P(16); break;
+    case 32: NODALS_REFRESH_P(32); break;
+    default: NODALS_REFRESH_P(64); break;
+  }
+#undef NODALS_REFRESH_P
+#undef NODALS_REFRESH
+  NODALS_LAUNCHED();
+}
+
+void H8Solver::fineApply(hipStream_t s, const double* x, double* y) {
+  spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y);
+}
+
+void H8Solver::fineSpmv(const double* x, double* y) {
+  requireDevice();
+  fineApply(stream_, x, y);
+}
+
+void H8Solver::precondApply(hipStream_t s, const double* r, double* z) {
+  if (amg64_) {
+    amg64_->apply(s, r, z);
+  } else if (amg32_) {
+    cast<float, double>(s, nc_, r, rf_.data());
+    amg32_->apply(s, rf_.data(), zf_.data());
+    cast<double, float>(s, nc_, zf_.data(), z);
+  } else {
+    cast<float, double>(s, nc_, r, rf_.data());
+    amgb_->apply(s, rf_.data(), zf_.data());
+    cast<double, float>(s, nc_, zf_.data(), z);
+  }
+}
+
+double H8Solver::pressurePrecondPower(int its, double safety) {
+  // pressure_precond_power (h8_main_base.cu:613-668): power iteration on
+  // M^-1 A from the sin/cos start vector.
+  std::vector<double> v0((std::size_t)nc_);
+  for (int i = 0; i < nc_; ++i) {
+    const double g = (double)(i + 1);
+    v0[(std::size_t)i] = std::sin(0.731 * g) + 0.27 * std::cos(1.117 * g);
+  }
+  double* v = tmpA_.data();
+  double* q = tmpB_.data();
+  double* z = pcx_.data();
+  NODALS_HIP(hipMemcpyAsync(v, v0.data(), sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+  const double vn = std::sqrt(red_.dot(stream_, nc_, v, v));
+  hipLaunchKernelGGL(k_scale_into<double>, blocksFor(nc_), 256, 0, stream_, (long long)nc_, 1.0 / vn, v, v);
+  NODALS_LAUNCHED();
+  double est = 0.0;
+  for (int k = 0; k < its; ++k) {
+    fineApply(stream_, v, q);
+    precondApply(stream_, q, z);
+    const double zn = std::sqrt(red_.dot(stream_, nc_, z, z));
+    const double ray = red_.dot(stream_, nc_, v, z);
+    require(zn > 0.0 && std::isfinite(zn) && std::isfinite(ray), "preconditioned power iteration failed");
+    est = std::max(est, std::max(std::abs(ray), zn));
+    hipLaunchKernelGGL(k_scale_into<double>, blocksFor(nc_), 256, 0, stream_, (long long)nc_, 1.0 / zn, z, v);
+    NODALS_LAUNCHED();
+  }
+  return safety * est;
+}
+
+KrylovResult H8Solver::pressureSolve(const std::string& solver, double rtol, double atol, int maxit) {
+  requireDevice();
+  DevOp A = [this](hipStream_t s, const double* x, double* y) { fineApply(s, x, y); };
+  DevOp M = [this](hipStream_t s, const double* r, double* z) { precondApply(s, r, z); };
+  KrylovResult r;
+  if (solver == "pcg") r = pcg(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_);
+  else if (solver == "richardson")
+    r = richardsonCheb(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_, false,
+                       opt_.pressureRichardsonOmega, 1.0, 0.5, 1);
+  else if (solver == "cheb")
+    r = richardsonCheb(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_, true, 1.0, chebLambda_,
+                       opt_.pressureChebLowFraction, opt_.pressureChebDegree);
+  else if (solver == "fgmres")
+    r = fgmres(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, opt_.pressureRestart, kw_);
+  else fail("unknown pressure solver " + solver);
+  return r;
+}
+
+void H8Solver::pressureUpdate(double alphaP) {
+  requireDevice();
+  require(kw_.n == nc_, "pressure update before a pressure solve", kErrInvalid);
+  axpy<double>(stream_, nc_, alphaP, kw_.x.data(), p_.data());
+}
+
+double H8Solver::rauSnapshotRel() {
+  requireDevice();
+  hipLaunchKernelGGL(k_scale_into<double>, blocksFor(nv_), 256, 0, stream_, (long long)nv_, 1.0, rau_.data(),
+                     tmpA_.data());
+  NODALS_LAUNCHED();
+  axpy<double>(stream_, nv_, -1.0, rauSnap_.data(), tmpA_.data());
+  const double dn = std::sqrt(red_.dot(stream_, nv_, tmpA_.data(), tmpA_.data()));
+  const double rn = std::sqrt(red_.dot(stream_, nv_, rauSnap_.data(), rauSnap_.data()));
+  return dn / std::max(rn, 1e-300);
+}
+
+// ------------------------------------------------------------------ SIMPLE loop
+H8RunResult H8Solver::run() {
+  requireDevice();
+  const H8Options& o = opt_;
+  H8RunResult R;
+  R.setupMs = setupMs_;
+  R.exactPressureDrop = S_.pipe.hpDrop;
+  const bool verbose = o.verbose > 0;
+  if (verbose) {
+    std::printf(
+        "NODALS_GPU_H8_SETUP tag=%s cells=%d freeVel=%d momentumNnz=%lld colors=%d hierarchyLevels=%zu terminal=%d "
+        "fineNnz=%lld amgPrecision=%s pressureSolver=%s setupMs=%.3f status=PASS\n",
+        o.tag.c_str(), nc_, nv_, momNnz_, S_.coloring.ncolors, H_.A.size(), H_.terminalN, fineNnz_,
+        o.amgPrecision.c_str(), o.pressureSolver.c_str(), setupMs_);
+    std::fflush(stdout);
+  }
+  hipEvent_t ev[6];
+  for (auto& e : ev) NODALS_HIP(hipEventCreate(&e));
+  bool converged = false, pressureAll = true, finiteAll = true;
+  double cont0 = -1.0, contRel = 1.0;
+  long long sumP = 0;
+  std::array<double, 3> lastAudit{{NAN, NAN, NAN}};
+  int finalIt = 0;
+  const auto wall0 = std::chrono::steady_clock::now();
+  try {
+    for (int it = 1; it <= o.maxOuter; ++it) {
+      NODALS_HIP(hipEventRecord(ev[0], stream_));
+      assemble(o.alphaU, o.rauScale);
+      NODALS_HIP(hipEventRecord(ev[1], stream_));
+      if (it == 1) {
+        const double rr = rauSnapshotRel();
+        if (verbose)
+          std::printf("NODALS_GPU_H8_SNAPSHOT_PARITY it=1 rAURel=%.3e tol=%.3e status=%s\n", rr, o.snapshotTol,
+                      rr < o.snapshotTol ? "PASS" : "FAIL");
+        require(rr < o.snapshotTol, "H8 setup rAU snapshot mismatch", kErrInternal);
+      }
+      bool audited = false;
+      std::array<double, 3> audit{{NAN, NAN, NAN}};
+      if (o.runMode == "converge" && it > 1 && contRel <= o.simpleTol) {
+        audit = momentumAudit();
+        lastAudit = audit;
+        audited = true;
+      }
+      const char* dir = momentumWork(o.momentumWork, o.momOmega, it);
+      NODALS_HIP(hipEventRecord(ev[2], stream_));
+      const double cn = continuity();
+      if (it == 1) cont0 = cn;
+      contRel = cn / std::max(cont0, 1e-300);
+      NODALS_HIP(hipEventRecord(ev[3], stream_));
+      refresh();
+      const KrylovResult pr = pressureSolve(o.pressureSolver, o.pRtol, o.pAtol, o.pMax);
+      pressureAll = pressureAll && pr.ok;
+      require(pr.ok, "H8 pressure solve failed to reach its target", kErrNotConverged);
+      sumP += pr.its;
+      NODALS_HIP(hipEventRecord(ev[4], stream_));
+      pressureUpdate(o.alphaP);
+      NODALS_HIP(hipEventRecord(ev[5], stream_));
+      NODALS_HIP(hipEventSynchronize(ev[5]));
+      float ms[5];
+      for (int k = 0; k < 5; ++k) NODALS_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+      R.assemblyMs += ms[0];
+      R.momentumMs += ms[1];
+      R.continuityMs += ms[2];
+      R.pressureMs += ms[3];
+      R.pupdateMs += ms[4];
+      finiteAll = finiteAll && std::isfinite(contRel) && std::isfinite(pr.rel);
+      bool allMet = false;
+      if (audited) {
+        allMet = std::isfinite(audit[0]) && std::isfinite(audit[1]) && std::isfinite(audit[2]) &&
+                 audit[0] <= o.simpleTol && audit[1] <= o.simpleTol && audit[2] <= o.simpleTol;
+        finiteAll = finiteAll && std::isfinite(audit[0]) && std::isfinite(audit[1]) && std::isfinite(audit[2]);
+      }
+      converged = o.runMode == "converge" && finiteAll && pressureAll && contRel <= o.simpleTol && audited && allMet;
+      finalIt = it;
+      if (verbose && (it <= 10 || it % 25 == 0 || audited || converged)) {
+        std::printf(
+            "NODALS_GPU_H8_SIMPLE tag=%s it=%d relCont=%.12e momentumWork=%s momentumDirection=%s residualAudit=%s "
+            "auditInitRel=[%.3e,%.3e,%.3e] pCG=%d pTrueRel=%.3e allMomentumMet=%d converged=%d status=%s\n",
+            o.tag.c_str(), it, contRel, o.momentumWork.c_str(), dir, audited ? "DONE" : "SKIPPED", audit[0], audit[1],
+            audit[2], pr.its, pr.rel, (int)allMet, (int)converged, (pressureAll && finiteAll) ? "PASS" : "FAIL");
+        std::fflush(stdout);
+      }
+      if (!finiteAll) fail("non-finite SIMPLE residual", kErrNonFinite);
+      if (converged) break;
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  sync();
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  R.loopMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  R.outer = finalIt;
+  R.converged = converged ? 1 : 0;
+  R.finalRelCont = contRel;
+  R.avgPressureIts = finalIt ? (double)sumP / finalIt : 0.0;
+  const std::vector<double> p = getCellField("p");
+  R.pressureDrop = pressureDropFit(S_, p);
+  R.pass = o.runMode == "fixed10" ? (finalIt == 10 && pressureAll && finiteAll) : converged;
+  (void)lastAudit;
+  return R;
+}
+
+// ------------------------------------------------------------------ host access
+std::vector<double> H8Solver::getVelocity(int component) {
+  require(component >= 0 && component < 3, "velocity component out of range");
+  sync();
+  const std::vector<double> a = u_.host();
+  std::vector<double> o((std::size_t)nv_);
+  for (int i = 0; i < nv_; ++i) o[(std::size_t)perm_[(std::size_t)i]] = a[(std::size_t)i * 3 + component];
+  return o;
+}
+
+void H8Solver::setVelocity(int component, const std::vector<double>& v) {
+  require(component >= 0 && component < 3 && (int)v.size() == nv_, "velocity set: bad component or size");
+  sync();
+  std::vector<double> a = u_.host();
+  for (int i = 0; i < nv_; ++i) a[(std::size_t)i * 3 + component] = v[(std::size_t)perm_[(std::size_t)i]];
+  u_.upload(a);
+}
+
+std::vector<double> H8Solver::getBrhs(int component) {
+  require(component >= 0 && component < 3, "rhs component out of range");
+  sync();
+  const std::vector<double> a = brhs_.host();
+  std::vector<double> o((std::size_t)nv_);
+  for (int i = 0; i < nv_; ++i) o[(std::size_t)perm_[(std::size_t)i]] = a[(std::size_t)i * 3 + component];
+  return o;
+}
+
+std::vector<double> H8Solver::getCellField(const std::string& name) {
+  sync();
+  if (name == "p") return p_.host();
+  if (name == "cont") return cont_.host();
+  if (name == "fine_diag") return fdiag_.host();
+  if (name == "fine_l1") return fl1_.host();
+  if (name == "dp") {
+    require(kw_.n == nc_, "no pressure solution yet");
+    return kw_.x.host();
+  }
+  fail("unknown cell field " + name);
+}
+
+void H8Solver::setCellField(const std::string& name, const std::vector<double>& v) {
+  require((int)v.size() == nc_, "cell field size mismatch");
+  sync();
+  if (name == "p") p_.upload(v);
+  else if (name == "cont") cont_.upload(v);
+  else fail("cell field not settable: " + name);
+}
+
+std::vector<double> H8Solver::getRowField(const std::string& name) {
+  sync();
+  const DBuf<double>* b = nullptr;
+  if (name == "rau") b = &rau_;
+  else if (name == "delta") b = &delta_;
+  else if (name == "diag") b = &diag_;
+  else fail("unknown row field " + name);
+  const std::vector<double> a = b->host();
+  std::vector<double> o((std::size_t)nv_);
+  for (int i = 0; i < nv_; ++i) o[(std::size_t)perm_[(std::size_t)i]] = a[(std::size_t)i];
+  return o;
+}
+
+std::vector<double> H8Solver::getMomentumValues() {
+  sync();
+  const std::vector<double> a = aval_.host();
+  const CsrD& A = S_.mom;
+  std::vector<double> o(a.size());
+  std::vector<int> mrow = mrow_.host();
+  for (int i = 0; i < nv_; ++i) {
+    const int r = perm_[(std::size_t)i];
+    const long long b = A.row[(std::size_t)r];
+    for (long long k = 0; k < A.row[(std::size_t)r + 1] - b; ++k) o[(std::size_t)(b + k)] = a[(std::size_t)(mrow[(std::size_t)i] + k)];
+  }
+  return o;
+}
+
+std::vector<double> H8Solver::getFineValues() {
+  sync();
+  return fval_.host();
+}
+
+double H8Solver::timeKernel(const std::string& kernel, int reps, double* bytes) {
+  requireDevice();
+  require(reps >= 1, "reps must be >= 1");
+  const Bytes b = algorithmicBytes();
+  std::function<void()> launch;
+  double by = 0;
+  if (kernel == "fine_spmv") {
+    fill<double>(stream_, nc_, tmpA_.data(), 1.0);
+    launch = [this]() { fineApply(stream_, tmpA_.data(), tmpB_.data()); };
+    by = b.fineSpmv;
+  } else if (kernel == "refresh") {
+    launch = [this]() { refresh(); };
+    by = b.refresh;
+  } else if (kernel == "mom_gs") {
+    launch = [this]() { momentumWork("fgs1", opt_.momOmega, 1); };
+    by = b.momGsSweep;
+  } else if (kernel == "conv_local") {
+    const int waves = std::min(nc_, 8192);
+    launch = [this, waves]() {
+      hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, stream_, nc_, cref_.data(),
+                         cinvJ_.data(), cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), cbuf_.data());
+      NODALS_LAUNCHED();
+    };
+    by = b.convLocal;
+  } else if (kernel == "assemble") {
+    launch = [this]() { assemble(opt_.alphaU, 1.0); };
+    by = b.convLocal + b.momRows;
+  } else if (kernel == "continuity") {
+    launch = [this]() {
+      hipLaunchKernelGGL(k_continuity, kReduceParts, 256, 0, stream_, nc_, cref_.data(), bcoef_.data(),
+                         fixedDiv_.data(), u_.data(), cont_.data(), red_.partials());
+      NODALS_LAUNCHED();
+    };
+    by = b.continuity;
+  } else {
+    fail("unknown kernel " + kernel);
+  }
+  launch();
+  hipEvent_t e0, e1;
+  NODALS_HIP(hipEventCreate(&e0));
+  NODALS_HIP(hipEventCreate(&e1));
+  NODALS_HIP(hipEventRecord(e0, stream_));
+  for (int r = 0; r < reps; ++r) launch();
+  NODALS_HIP(hipEventRecord(e1, stream_));
+  NODALS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  NODALS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (bytes) *bytes = by;
+  return (double)ms / reps;
+}
+
+H8Solver::Bytes H8Solver::algorithmicBytes() const {
+  Bytes b;
+  const double n = nc_, nnz = (double)fineNnz_;
+  // y = A x, FP64 values, int32 columns and row pointers, x and y once
+  b.fineSpmv = nnz * (8 + 4) + (n + 1) * 4 + n * 8 + n * 8;
+  // refresh: row/diag pointers, per-nnz column + term offset + value write,
+  // per-term packed word, B table once, rAU once, diagonal write
+  b.refresh = (n + 1) * 4 + n * 4 + nnz * (4 + 4 + 8) + (double)F_.nzTerm.size() * 4 + n * 24 * 8 + nv_ * 8.0 + n * 8;
+  // one forward sweep: A (values+cols), row pointers, b (3), diag, u read+write (3 each)
+  b.momGsSweep = momNnz_ * 12.0 + (nv_ + 1) * 4.0 + nv_ * (24.0 + 8.0 + 48.0);
+  b.convLocal = n * (8 * 4 + 9 * 8 + 8 + 64 * 8) + nv_ * 24.0;
+  b.momRows = momNnz_ * (8.0 + 8.0 + 4.0) + (double)(momNnz_) * 1.75 * (4 + 8) + nv_ * (4 * 8.0 + 24 * 3);
+  b.continuity = n * (8 * 4 + 24 * 8 + 8 + 8) + nv_ * 24.0;
+  return b;
+}
+
+}  // namespace nodals

@@ -1,0 +1,173 @@
+// H8 laminar SIMPLE solver context on one MI355X.
+//
+// Mirrors the device state and solver calls of the reference H8 driver
+// (gpu/serial_cuda/h8_main_base.cu): assemble_live (:1137-1155),
+// bt_state + momentum RHS (:2061-2062), fixed_momentum_work (:1700-1715),
+// the continuity residual (:2070-2081), the fine CSR refresh (:2083-2086),
+// the pressure Krylov solvers (:567-726, plus FGMRES) and the p update
+// (:2102).  All device memory is owned by the context; one HIP stream per
+// context; errors are thrown as nodals::Error and turned into status codes at
+// the C ABI.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../host/amg_setup.hpp"
+#include "../host/h8_setup.hpp"
+#include "amg.hpp"
+#include "krylov.hpp"
+
+namespace nodals {
+
+struct H8Options {
+  std::string mesh, tag = "h8", wall = "patch_0_0", inlet = "patch_2_0", outlet = "patch_1_0", vtuOut;
+  double re = 20, bulk = 1, simpleTol = 1e-6;
+  double alphaU = .5, alphaP = .5, rauScale = 1.0;
+  double momRtol = 1e-6, momAtol = 1e-12, momDrop = .1, momOmega = 1.0;
+  double pRtol = .5, pAtol = 1e-12;
+  double snapshotTol = 5e-6;
+  bool mixingLength = false, dgInlet = false, weakWall = false, supg = false;
+  std::string wallTractionMode = "spalding", wallDiagCsv;
+  double wallReferenceF = 0.0284003, mixlenScale = 1.0, supgTauScale = 0.05, supgMagic = 9.0;
+  int supgQuadPoints = 64;
+  std::string supgForm = "implicit";
+  int maxOuter = 2500, momMax = 20000, pMax = 20;
+  int fineCsrRefreshEvery = 1;
+  std::string momentumWork = "fgs1", runMode = "fixed10", amgSmoother = "cheb2", amgHierarchy = "sa",
+              cfInterp = "direct", pressureSolver = "pcg", cfCoarsening = "pmis", cfStrength = "classical-negative",
+              amgSpectrumPolicy = "auto";
+  double amgJacobiOmega = .7;
+  int amgJacobiFineSweeps = 1, amgJacobiCoarseSweeps = 1;
+  double amgMcgsOmega = 1.0;
+  int amgMcgsFineSweeps = 1, amgMcgsCoarseSweeps = 1;
+  std::string amgMcgsOrder = "symmetric";
+  double pressureRichardsonOmega = 1.0, pressureChebLowFraction = .05, pressurePowerSafety = 1.15;
+  int pressureChebDegree = 3, pressurePowerIts = 6;
+  int cfAggressiveFirst = 0, amgTerminal = 1000, amgPowerIts = 16, amgChebDegree = 2;
+  double amgLambdaSafety = 1.5, amgLambdaLowFraction = .05, cfTheta = .25;
+  int cfPmax = 4;
+  // Extensions of this implementation.
+  std::string amgPrecision = "fp64";  // fp64 | fp32 | bf16 (finest level bf16, coarse fp32)
+  int pressureRestart = 30;           // FGMRES restart length
+  int verbose = 1;
+};
+
+// Parses the reference H8 argv (h8_main_base.cu:1787-1850) and validates it
+// the same way (:1851-1911).  argv[0] is not examined.
+H8Options parseH8Args(int argc, const char* const* argv);
+
+struct H8RunResult {
+  int outer = 0;
+  int converged = 0;
+  int pass = 0;
+  double finalRelCont = 0;
+  double avgPressureIts = 0;
+  double pressureDrop = 0;
+  double exactPressureDrop = 0;
+  double loopMs = 0;
+  double assemblyMs = 0, momentumMs = 0, continuityMs = 0, pressureMs = 0, pupdateMs = 0;
+  double setupMs = 0;
+};
+
+class H8Solver {
+ public:
+  // withDevice=false builds the host setup only (mesh, DOFs, CSR patterns,
+  // colouring, AMG hierarchy); every device call then fails with kErrInvalid.
+  explicit H8Solver(const H8Options& o, bool withDevice = true);
+  ~H8Solver();
+  bool hasDevice() const { return stream_ != nullptr; }
+  H8Solver(const H8Solver&) = delete;
+  H8Solver& operator=(const H8Solver&) = delete;
+
+  const H8Options& options() const { return opt_; }
+  hipStream_t stream() const { return stream_; }
+
+  void assemble(double alphaU, double rauScale);
+  const char* momentumWork(const std::string& mode, double omega, int outer);
+  std::array<double, 3> momentumAudit();
+  double continuity();
+  void refresh();
+  dev::KrylovResult pressureSolve(const std::string& solver, double rtol, double atol, int maxit);
+  void pressureUpdate(double alphaP);
+  double rauSnapshotRel();
+  H8RunResult run();
+
+  // Host access in reference numbering (free DOFs as in the reference).
+  std::vector<double> getVelocity(int component);
+  void setVelocity(int component, const std::vector<double>& v);
+  std::vector<double> getCellField(const std::string& name);
+  void setCellField(const std::string& name, const std::vector<double>& v);
+  std::vector<double> getRowField(const std::string& name);
+  std::vector<double> getMomentumValues();  // reference CSR order
+  std::vector<double> getFineValues();      // fine CSR order (cell rows unchanged)
+  std::vector<double> getBrhs(int component);
+
+  const H8Setup& setup() const { return S_; }
+  const FineTopo& fineTopo() const { return F_; }
+  const AmgHierarchy& hierarchy() const { return H_; }
+  const TetMesh& mesh() const { return M_; }
+  int ncolors() const { return S_.coloring.ncolors; }
+
+  // Kernel-level handles for measurement: the fine CSR SpMV on device data.
+  void fineSpmv(const double* x, double* y);
+  // Mean duration (ms) of `reps` launches of a hot kernel measured with HIP
+  // events on the context stream; *bytes = algorithmic bytes per launch.
+  double timeKernel(const std::string& kernel, int reps, double* bytes);
+  double* devScratchA() { return tmpA_.data(); }
+  double* devScratchB() { return tmpB_.data(); }
+  void sync();
+
+  // algorithmic bytes of one application of each hot kernel
+  struct Bytes {
+    double fineSpmv = 0, refresh = 0, momGsSweep = 0, convLocal = 0, momRows = 0, continuity = 0;
+  };
+  Bytes algorithmicBytes() const;
+
+ private:
+  void requireDevice() const {
+    require(stream_ != nullptr, "H8 context was created without a device (host setup only)");
+  }
+  void upload();
+  void setupAmg();
+  void precondApply(hipStream_t s, const double* r, double* z);
+  void fineApply(hipStream_t s, const double* x, double* y);
+  double pressurePrecondPower(int its, double safety);
+
+  H8Options opt_;
+  TetMesh M_;
+  H8Setup S_;
+  FineTopo F_;
+  AmgHierarchy H_;
+  hipStream_t stream_ = nullptr;
+  dev::Reducer red_;
+  dev::KrylovWork kw_;
+
+  std::vector<int> perm_, pinv_;  // device numbering: perm_[new] = reference row
+  int nc_ = 0, nv_ = 0, nFix_ = 0, fineLanes_ = 16, momLanes_ = 16;
+  long long momNnz_ = 0, fineNnz_ = 0;
+  std::vector<int> colorOff_;
+
+  DBuf<int> cref_;
+  DBuf<double> cinvJ_, cdet_, bcoef_, fixedVal_, centT_;
+  DBuf<int> mrow_, mcol_, mdiagPos_, termOff_, termIdx_, fixOff_, fixTerm_, incOff_;
+  DBuf<unsigned> incPacked_;
+  DBuf<double> mdiff_, aval_, cbuf_;
+  DBuf<double> u_, brhs_, srhs_, delta_, diag_, rau_, rauSnap_;
+  DBuf<double> p_, fixedDiv_, cont_, pcx_;
+  DBuf<int> frow_, fcol_, fdiagPos_, nzOff_;
+  DBuf<unsigned> nzTerm_;
+  DBuf<double> fval_, fdiag_, fl1_;
+  DBuf<float> fvalf_, fdiagf_, fl1f_, rf_, zf_;
+  DBuf<bf16> fvalb_;
+  DBuf<double> tmpA_, tmpB_;
+  std::unique_ptr<dev::DevAmg<double, double>> amg64_;
+  std::unique_ptr<dev::DevAmg<float, float>> amg32_;
+  std::unique_ptr<dev::DevAmg<bf16, float>> amgb_;
+  double setupMs_ = 0;
+  double chebLambda_ = 1.0;
+};
+
+}  // namespace nodals

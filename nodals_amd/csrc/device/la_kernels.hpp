@@ -1,0 +1,350 @@
+// Sparse and dense linear-algebra kernels for gfx950 (64-lane wavefronts).
+//
+// Row-parallel kernels map one row to a group of LANES consecutive lanes of
+// a wavefront (LANES in {4,8,16,32,64}), chosen from the mean row length so a
+// 64-lane wavefront keeps its loads coalesced on short rows without idling on
+// long ones.  Group reductions use __shfl_xor inside the group, so every row's
+// sum has a fixed order: results are deterministic run to run.  Global FP64
+// reductions are two-stage (fixed grid of partial sums, then one block per
+// output), also deterministic.  Nothing here uses atomics.
+#pragma once
+
+#include "common.hpp"
+
+namespace nodals {
+namespace dev {
+
+constexpr int kReduceParts = 256;  // partial sums per reduction (grid size)
+
+template <class T>
+__device__ __forceinline__ T groupSum(T v, int lanes) {
+  for (int o = lanes >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, lanes);
+  return v;
+}
+
+// ---------------------------------------------------------------- BLAS-1
+template <class T>
+__global__ void k_fill(long long n, T* x, T a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = a;
+}
+template <class TD, class TS>
+__global__ void k_cast(long long n, const TS* __restrict__ a, TD* __restrict__ b) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (TD)a[i];
+}
+template <class T>
+__global__ void k_axpy(long long n, double a, const T* __restrict__ x, T* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += (T)a * x[i];
+}
+template <class T>
+__global__ void k_scale_into(long long n, double a, const T* __restrict__ x, T* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (T)a * x[i];
+}
+template <class T>
+__global__ void k_xpby(long long n, const T* __restrict__ x, double b, T* __restrict__ y) {
+  // y = x + b*y
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + (T)b * y[i];
+}
+template <class T>
+__global__ void k_axpy2(long long n, double a, const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ x,
+                        T* __restrict__ r) {
+  // x += a p ; r -= a q
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const T aa = (T)a;
+    x[i] += aa * p[i];
+    r[i] -= aa * q[i];
+  }
+}
+template <class T>
+__global__ void k_jacobi_zero(int n, double omega, const T* __restrict__ b, const T* __restrict__ d, T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (T)omega * b[i] / d[i];
+}
+template <class T>
+__global__ void k_jacobi_add(int n, double omega, const T* __restrict__ r, const T* __restrict__ d, T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += (T)omega * r[i] / d[i];
+}
+
+// Multi-vector update y -= sum_k c_k V_k (V column-major, stride n).
+struct Coef32 {
+  double c[32];
+};
+static __global__ void k_multi_axpy(long long n, int k, const double* __restrict__ V, Coef32 c, double* __restrict__ y,
+                             double sign) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; ++j) s += c.c[j] * V[(long long)j * n + i];
+  y[i] += sign * s;
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double blockSum256(double v, double* sh) {
+  v = groupSum(v, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) t = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return t;
+}
+
+template <class TA, class TB>
+__global__ void __launch_bounds__(256) k_dot_part(long long n, const TA* __restrict__ x, const TB* __restrict__ y,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    s += (double)x[i] * (double)y[i];
+  s = blockSum256(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// K dot products against one vector: part[k*gridDim + block] = sum V_k . w
+template <int KMAX>
+__global__ void __launch_bounds__(256) k_multidot_part(long long n, int k, const double* __restrict__ V,
+                                                       const double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double sh[4];
+  double s[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) s[j] = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double wi = w[i];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+      if (j < k) s[j] += V[(long long)j * n + i] * wi;
+  }
+  for (int j = 0; j < k; ++j) {
+    const double t = blockSum256(s[j], sh);
+    if (threadIdx.x == 0) part[(long long)j * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// out[b] = sum of part[b*nparts .. b*nparts+nparts)
+static __global__ void __launch_bounds__(256) k_reduce_final(int nparts, const double* __restrict__ part, double* out) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) s += part[(long long)blockIdx.x * nparts + i];
+  s = blockSum256(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------- CSR kernels
+// y = A x (RESID=false) or y = b - A x (RESID=true); ACC=true adds into y.
+template <int LANES, class TM, class TV, bool RESID, bool ACC>
+__global__ void __launch_bounds__(256) k_spmv(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                              const TM* __restrict__ val, const TV* __restrict__ x,
+                                              const TV* __restrict__ b, TV* __restrict__ y) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int r = gid / LANES;
+  const int lane = gid % LANES;
+  if (r >= n) return;
+  TV s = 0;
+  const int e = row[r + 1];
+  for (int k = row[r] + lane; k < e; k += LANES) s += ld<TV>(val, k) * x[col[k]];
+  s = groupSum(s, LANES);
+  if (lane == 0) {
+    if (RESID) y[r] = b[r] - s;
+    else if (ACC) y[r] += s;
+    else y[r] = s;
+  }
+}
+
+// l1[r] = sum_k |a_rk|
+template <int LANES, class TM, class TV>
+__global__ void __launch_bounds__(256) k_row_l1(int n, const int* __restrict__ row, const TM* __restrict__ val,
+                                                TV* __restrict__ l1) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int r = gid / LANES;
+  const int lane = gid % LANES;
+  if (r >= n) return;
+  TV s = 0;
+  for (int k = row[r] + lane; k < row[r + 1]; k += LANES) {
+    const TV v = ld<TV>(val, k);
+    s += v >= TV(0) ? v : -v;
+  }
+  s = groupSum(s, LANES);
+  if (lane == 0) l1[r] = s;
+}
+
+// One multicolour Gauss-Seidel colour: rows crow[begin..begin+count).
+// x_i <- x_i + omega ((b_i - sum_{j!=i} a_ij x_j)/d_i - x_i)
+template <int LANES, class TM, class TV>
+__global__ void __launch_bounds__(256) k_mcgs_color(int begin, int count, const int* __restrict__ crow,
+                                                    const int* __restrict__ row, const int* __restrict__ col,
+                                                    const TM* __restrict__ val, const TV* __restrict__ diag,
+                                                    const TV* __restrict__ b, TV* __restrict__ x, double omega) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int q = gid / LANES;
+  const int lane = gid % LANES;
+  if (q >= count) return;
+  const int i = crow[begin + q];
+  TV s = 0;
+  for (int k = row[i] + lane; k < row[i + 1]; k += LANES) {
+    const int j = col[k];
+    if (j != i) s += ld<TV>(val, k) * x[j];
+  }
+  s = groupSum(s, LANES);
+  if (lane == 0) {
+    const TV old = x[i];
+    const TV gs = (b[i] - s) / diag[i];
+    x[i] = old + (TV)omega * (gs - old);
+  }
+}
+
+// x = T b for a dense row-major n x n matrix (terminal AMG level).
+template <class TV>
+__global__ void __launch_bounds__(256) k_dense_mv(int n, const TV* __restrict__ T, const TV* __restrict__ b,
+                                                  TV* __restrict__ x) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int r = gid >> 6;
+  const int lane = gid & 63;
+  if (r >= n) return;
+  TV s = 0;
+  const TV* tr = T + (long long)r * n;
+  for (int j = lane; j < n; j += 64) s += tr[j] * b[j];
+  s = groupSum(s, 64);
+  if (lane == 0) x[r] = s;
+}
+
+template <class T>
+__global__ void k_diag_gather(int n, const int* __restrict__ diagPos, const T* __restrict__ val, T* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = val[diagPos[i]];
+}
+
+// ---------------------------------------------------------------- launch helpers
+inline int lanesFor(double meanRow) {
+  if (meanRow <= 6) return 4;
+  if (meanRow <= 20) return 8;
+  if (meanRow <= 48) return 16;
+  if (meanRow <= 128) return 32;
+  return 64;
+}
+
+template <class TM, class TV, bool RESID, bool ACC>
+void spmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TV* x, const TV* b,
+          TV* y) {
+  if (n <= 0) return;
+  const int grid = blocksFor((long long)n * lanes);
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((k_spmv<4, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
+    case 8: hipLaunchKernelGGL((k_spmv<8, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
+    case 16: hipLaunchKernelGGL((k_spmv<16, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
+    case 32: hipLaunchKernelGGL((k_spmv<32, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
+    default: hipLaunchKernelGGL((k_spmv<64, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
+  }
+  NODALS_LAUNCHED();
+}
+
+template <class TM, class TV>
+void rowL1(hipStream_t s, int lanes, int n, const int* row, const TM* val, TV* l1) {
+  if (n <= 0) return;
+  const int grid = blocksFor((long long)n * lanes);
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((k_row_l1<4, TM, TV>), grid, 256, 0, s, n, row, val, l1); break;
+    case 8: hipLaunchKernelGGL((k_row_l1<8, TM, TV>), grid, 256, 0, s, n, row, val, l1); break;
+    case 16: hipLaunchKernelGGL((k_row_l1<16, TM, TV>), grid, 256, 0, s, n, row, val, l1); break;
+    case 32: hipLaunchKernelGGL((k_row_l1<32, TM, TV>), grid, 256, 0, s, n, row, val, l1); break;
+    default: hipLaunchKernelGGL((k_row_l1<64, TM, TV>), grid, 256, 0, s, n, row, val, l1); break;
+  }
+  NODALS_LAUNCHED();
+}
+
+template <class TM, class TV>
+void mcgsColor(hipStream_t s, int lanes, int begin, int count, const int* crow, const int* row, const int* col,
+               const TM* val, const TV* diag, const TV* b, TV* x, double omega) {
+  if (count <= 0) return;
+  const int grid = blocksFor((long long)count * lanes);
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((k_mcgs_color<4, TM, TV>), grid, 256, 0, s, begin, count, crow, row, col, val, diag, b, x, omega); break;
+    case 8: hipLaunchKernelGGL((k_mcgs_color<8, TM, TV>), grid, 256, 0, s, begin, count, crow, row, col, val, diag, b, x, omega); break;
+    case 16: hipLaunchKernelGGL((k_mcgs_color<16, TM, TV>), grid, 256, 0, s, begin, count, crow, row, col, val, diag, b, x, omega); break;
+    case 32: hipLaunchKernelGGL((k_mcgs_color<32, TM, TV>), grid, 256, 0, s, begin, count, crow, row, col, val, diag, b, x, omega); break;
+    default: hipLaunchKernelGGL((k_mcgs_color<64, TM, TV>), grid, 256, 0, s, begin, count, crow, row, col, val, diag, b, x, omega); break;
+  }
+  NODALS_LAUNCHED();
+}
+
+template <class T>
+void fill(hipStream_t s, long long n, T* x, T a) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<T>, blocksFor(n), 256, 0, s, n, x, a);
+  NODALS_LAUNCHED();
+}
+template <class TD, class TS>
+void cast(hipStream_t s, long long n, const TS* a, TD* b) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_cast<TD, TS>), blocksFor(n), 256, 0, s, n, a, b);
+  NODALS_LAUNCHED();
+}
+template <class T>
+void copy(hipStream_t s, long long n, const T* a, T* b) {
+  if (n <= 0) return;
+  NODALS_HIP(hipMemcpyAsync(b, a, (std::size_t)n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+template <class T>
+void axpy(hipStream_t s, long long n, double a, const T* x, T* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy<T>, blocksFor(n), 256, 0, s, n, a, x, y);
+  NODALS_LAUNCHED();
+}
+
+// Deterministic FP64 dot products; results land in pinned host memory.
+class Reducer {
+ public:
+  Reducer() = default;
+  ~Reducer() {
+    if (host_) (void)hipHostFree(host_);
+  }
+  Reducer(const Reducer&) = delete;
+  Reducer& operator=(const Reducer&) = delete;
+
+  void init(int maxVectors) {
+    maxVec_ = maxVectors;
+    part_.alloc((std::size_t)kReduceParts * maxVectors);
+    if (host_) (void)hipHostFree(host_);
+    NODALS_HIP(hipHostMalloc((void**)&host_, sizeof(double) * (std::size_t)maxVectors, hipHostMallocDefault));
+  }
+
+  template <class TA, class TB>
+  void dotAsync(hipStream_t s, long long n, const TA* x, const TB* y) {
+    hipLaunchKernelGGL((k_dot_part<TA, TB>), kReduceParts, 256, 0, s, n, x, y, part_.data());
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_final, 1, 256, 0, s, kReduceParts, part_.data(), host_);
+    NODALS_LAUNCHED();
+  }
+  template <class TA, class TB>
+  double dot(hipStream_t s, long long n, const TA* x, const TB* y) {
+    dotAsync(s, n, x, y);
+    NODALS_HIP(hipStreamSynchronize(s));
+    return host_[0];
+  }
+  // out[j] = V_j . w for j < k (V column-major with stride n).
+  void multidot(hipStream_t s, long long n, int k, const double* V, const double* w, double* out) {
+    if (k > maxVec_ || k > 32) fail("multidot width exceeds reducer capacity", kErrInternal);
+    hipLaunchKernelGGL(k_multidot_part<32>, kReduceParts, 256, 0, s, n, k, V, w, part_.data());
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_final, k, 256, 0, s, kReduceParts, part_.data(), host_);
+    NODALS_LAUNCHED();
+    NODALS_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j) out[j] = host_[j];
+  }
+  double* partials() { return part_.data(); }
+  double* hostOut() { return host_; }
+
+ private:
+  DBuf<double> part_;
+  double* host_ = nullptr;
+  int maxVec_ = 0;
+};
+
+}  // namespace dev
+}  // namespace nodals
