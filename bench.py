@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds-to-converge of the H8 SIMPLE pipeline on one MI355X.
+
+Workload (one "step" = one SIMPLE solve from rest to the convergence test,
+setup excluded): laminar straight circular pipe, D=1, L=10D, Re=20, O-grid
+section of 224 quads x 30 slabs, 6 tets per hex (40,320 tets), Poiseuille
+Dirichlet inlet, zero-pressure outlet, no-slip wall; FGS momentum sweep,
+SA-AMG-preconditioned FGMRES pressure solve, to continuity and momentum
+residual 1e-6.  `value` is the mean wall-clock seconds of the FP32-AMG form
+(AMG hierarchy and smoother in FP32 inside the FP64 FGMRES); the plain FP64
+form is timed beside it.  Inputs (mesh, setup, operators) are resident in
+HBM before the timed region.
+
+Also reported: the fine pressure SpMV roofline (algorithmic bytes per launch
+over the mean launch time from HIP events on the solver stream) and the CPU
+oracle (FP64 numpy/scipy restatement) timed on a bounded sample of the same
+case on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "s-to-converge 207k SST (res 1e-5): learned+FP32 AMG vs FP64; SpMV GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def solver_args(mesh: str, precision: str) -> list:
+    return ["--mesh", mesh, "--run-mode", "converge", "--max-outer", "20000", "--simple-tol", "1e-6",
+            "--alpha-u", "0.5", "--alpha-p", "0.5", "--amg-smoother", "jacobi", "--amg-jacobi-omega", "0.7",
+            "--pressure-solver", "fgmres", "--p-rtol", "0.1", "--p-max", "200", "--pressure-restart", "30",
+            "--amg-precision", precision, "--verbose", "0", "--tag", f"bench_{precision}"]
+
+
+def cpu_baseline(mesh: str, gpu_outer: int, iters: int) -> dict:
+    """Oracle SIMPLE iterations on the host; projected to the GPU's count."""
+    import numpy as np
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    from nodals_amd._lib import H8
+    from oracle import h8_oracle as O
+
+    h = H8(solver_args(mesh, "fp64"), device=False)
+    case = O.case_from_product(h)
+    h.close()
+    O.fine_topology(case)  # setup part, not timed
+    ctx = threadpool_limits(1) if threadpool_limits else None
+    t0 = time.perf_counter()
+    O.simple(case, p_solver="fgmres", p_rtol=0.1, p_max=200, smoother=O.Smoother(kind="jacobi"),
+             run_mode="fixed10", max_outer=iters)
+    dt = time.perf_counter() - t0
+    if ctx is not None:
+        ctx.__exit__(None, None, None)
+    per_it = dt / iters
+    return {"value": per_it * gpu_outer, "unit": "s", "cores": 1, "kind": "port",
+            "sample": (f"{iters} SIMPLE iterations of the same 40,320-tet pipe by the FP64 numpy/scipy oracle "
+                       f"({dt:.1f} s, {per_it:.3f} s/iteration), projected to the GPU's {gpu_outer} iterations "
+                       f"to converge")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=500)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    from nodals_amd import meshgen
+    from nodals_amd._lib import H8
+
+    mesh_root = ROOT / "_meshes"
+    mesh = mesh_root / "pipe_224x30"
+    if not (mesh / "faces").exists():
+        meshgen.write_pipe_case(mesh, core=8, ring=5, nslab=30)
+    mesh = str(mesh)
+
+    results = {}
+    ctxs = {}
+    for prec in ("fp64", "fp32"):
+        h = H8(solver_args(mesh, prec))
+        ctxs[prec] = h
+        for _ in range(a.warmup):
+            h.set("p", __import__("numpy").zeros(h.info["cells"]))
+            for d in range(3):
+                h.set(f"u{d}", __import__("numpy").zeros(h.info["nv"]))
+            h.run()
+        times, runs = [], []
+        for _ in range(a.steps):
+            import numpy as np
+            h.set("p", np.zeros(h.info["cells"]))
+            for d in range(3):
+                h.set(f"u{d}", np.zeros(h.info["nv"]))
+            t0 = time.perf_counter()
+            r = h.run()
+            times.append(time.perf_counter() - t0)
+            runs.append(r)
+            print(f"[bench] {prec} step: {times[-1]:.3f} s, outer={r['outer']} converged={r['converged']} "
+                  f"avgP={r['avg_pressure_its']:.2f} dp={r['pressure_drop']:.6f}", file=sys.stderr, flush=True)
+        results[prec] = {"times": times, "runs": runs}
+
+    import numpy as np
+    h = ctxs["fp32"]
+    ms, by = h.time_kernel("fine_spmv", a.spmv_reps)
+    achieved = by / (ms * 1e-3) / 1e9
+    extra = {}
+    for k in ("refresh", "mom_gs", "assemble", "continuity"):
+        kms, kby = h.time_kernel(k, 100)
+        extra[k] = {"ms": kms, "GB_s": kby / (kms * 1e-3) / 1e9}
+
+    fp32 = results["fp32"]
+    fp64 = results["fp64"]
+    value = float(np.mean(fp32["times"]))
+    outer = int(fp32["runs"][-1]["outer"])
+    dp32 = fp32["runs"][-1]["pressure_drop"]
+    dp64 = fp64["runs"][-1]["pressure_drop"]
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(mesh, int(fp64["runs"][-1]["outer"]), a.cpu_iters)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": value * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state / f32 AMG",
+        "data": "synthetic (generated O-grid pipe mesh)",
+        "config": {
+            "workload": ("laminar straight pipe D=1 L=10D Re=20, O-grid 224 quads x 30 slabs, 40,320 tets; "
+                         "SIMPLE from rest to residual 1e-6 (FGS momentum, SA-AMG FGMRES pressure)"),
+            "cells": ctxs["fp32"].info["cells"], "free_velocity_dofs": ctxs["fp32"].info["nv"],
+            "fp32_amg_s": value, "fp64_s": float(np.mean(fp64["times"])),
+            "speedup_fp32_amg_vs_fp64": float(np.mean(fp64["times"])) / value,
+            "outer_iterations_fp32_amg": outer, "outer_iterations_fp64": int(fp64["runs"][-1]["outer"]),
+            "avg_pressure_its_fp32_amg": fp32["runs"][-1]["avg_pressure_its"],
+            "pressure_drop_fp64": dp64, "pressure_drop_fp32_amg": dp32,
+            "pressure_drop_rel_diff": abs(dp32 - dp64) / abs(dp64),
+            "hagen_poiseuille_drop": fp64["runs"][-1]["exact_pressure_drop"],
+            "setup_ms": fp32["runs"][-1]["setup_ms"],
+            "kernels": extra,
+            "parallelism": "single GPU",
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv<16,double,double,false,false> (fine P0 Schur SpMV)",
+                     "kernel_ms": ms, "bytes_per_launch": by},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for h in ctxs.values():
+        h.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -119,6 +119,9 @@ int nodals_h8_set(nodals_h8* h, const char* name, const void* host, int64_t coun
  * the mean launch time from HIP events on the context stream and
  * bytes the algorithmic bytes of one launch. */
 int nodals_h8_fine_spmv(nodals_h8* h, const double* x, double* y);
+/* One AMG preconditioner application z = M^-1 r (G4AMG::apply,
+ * h8_main_base.cu:557-564) on the current fine operator. */
+int nodals_h8_precond(nodals_h8* h, const double* r, double* z);
 int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double* ms_avg, double* bytes);
 
 /* ---------------------------------------------------------------- SYS ---

@@ -47,6 +47,7 @@ EXPORTS = [
     "nodals_h8_create", "nodals_h8_create_host", "nodals_h8_destroy", "nodals_h8_info", "nodals_h8_assemble", "nodals_h8_momentum_work",
     "nodals_h8_momentum_audit", "nodals_h8_continuity", "nodals_h8_pressure_refresh", "nodals_h8_pressure_solve",
     "nodals_h8_pressure_update", "nodals_h8_run", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
+    "nodals_h8_precond",
     "nodals_h8_time_kernel",
     "nodals_sys_create", "nodals_sys_destroy", "nodals_sys_info", "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
@@ -80,6 +81,7 @@ def lib() -> C.CDLL:
     L.nodals_h8_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     L.nodals_h8_set.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     L.nodals_h8_fine_spmv.argtypes = [vp, vp, vp]
+    L.nodals_h8_precond.argtypes = [vp, vp, vp]
     L.nodals_h8_time_kernel.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_destroy.argtypes = [vp]
@@ -179,6 +181,12 @@ class H8:
         y = np.empty_like(x)
         _check(lib().nodals_h8_fine_spmv(self._h, _ptr(x), _ptr(y)))
         return y
+
+    def precond(self, r: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        _check(lib().nodals_h8_precond(self._h, _ptr(r), _ptr(z)))
+        return z
 
     def time_kernel(self, name: str, reps: int):
         ms, by = C.c_double(), C.c_double()

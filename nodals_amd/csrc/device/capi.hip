@@ -287,6 +287,18 @@ int nodals_h8_fine_spmv(nodals_h8* h, const double* x, double* y) {
   });
 }
 
+int nodals_h8_precond(nodals_h8* h, const double* r, double* z) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(r != nullptr && z != nullptr, "null argument");
+    const int nc = s.setup().nc;
+    NODALS_HIP(hipMemcpyAsync(s.devScratchA(), r, sizeof(double) * nc, hipMemcpyHostToDevice, s.stream()));
+    s.precond(s.devScratchA(), s.devScratchB());
+    NODALS_HIP(hipMemcpyAsync(z, s.devScratchB(), sizeof(double) * nc, hipMemcpyDeviceToHost, s.stream()));
+    s.sync();
+  });
+}
+
 int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double* ms_avg, double* bytes) {
   return guarded([&]() {
     nodals::require(kernel != nullptr, "null kernel name");

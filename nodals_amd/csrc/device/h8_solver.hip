@@ -590,6 +590,11 @@ void H8Solver::fineSpmv(const double* x, double* y) {
   fineApply(stream_, x, y);
 }
 
+void H8Solver::precond(const double* r, double* z) {
+  requireDevice();
+  precondApply(stream_, r, z);
+}
+
 void H8Solver::precondApply(hipStream_t s, const double* r, double* z) {
   if (amg64_) {
     amg64_->apply(s, r, z);

@@ -113,6 +113,9 @@ class H8Solver {
 
   // Kernel-level handles for measurement: the fine CSR SpMV on device data.
   void fineSpmv(const double* x, double* y);
+  // z = M^-1 r with the AMG V-cycle in the configured precision (device
+  // pointers, length = cells).
+  void precond(const double* r, double* z);
   // Mean duration (ms) of `reps` launches of a hot kernel measured with HIP
   // events on the context stream; *bytes = algorithmic bytes per launch.
   double timeKernel(const std::string& kernel, int reps, double* bytes);

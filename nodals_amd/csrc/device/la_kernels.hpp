@@ -311,7 +311,9 @@ class Reducer {
     maxVec_ = maxVectors;
     part_.alloc((std::size_t)kReduceParts * maxVectors);
     if (host_) (void)hipHostFree(host_);
-    NODALS_HIP(hipHostMalloc((void**)&host_, sizeof(double) * (std::size_t)maxVectors, hipHostMallocDefault));
+    // Coherent pinned memory: the final reduction kernel writes it directly
+    // and the host reads it after the stream synchronisation.
+    NODALS_HIP(hipHostMalloc((void**)&host_, sizeof(double) * (std::size_t)maxVectors, hipHostMallocCoherent));
   }
 
   template <class TA, class TB>

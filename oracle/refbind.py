@@ -1,0 +1,85 @@
+"""ctypes binding of oracle/_ref/libnodals_ref.so (the reference's host setup
+code compiled from the reference tree by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: the checker side of setup parity tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+REF_LIB = Path(__file__).resolve().parent / "_ref" / "libnodals_ref.so"
+
+
+def available() -> bool:
+    return REF_LIB.exists()
+
+
+class RefCase:
+    def __init__(self, mesh: str, re: float = 20.0, bulk: float = 1.0, wall: str = "patch_0_0",
+                 inlet: str = "patch_2_0", outlet: str = "patch_1_0", alpha_u: float = 0.5):
+        L = C.CDLL(str(REF_LIB))
+        L.ref_open.restype = C.c_void_p
+        L.ref_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double]
+        L.ref_close.argtypes = [C.c_void_p]
+        L.ref_size.restype = C.c_longlong
+        L.ref_size.argtypes = [C.c_void_p, C.c_char_p]
+        L.ref_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        L.ref_build_sa.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.ref_schur_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_assemble.argtypes = [C.c_void_p] + [C.c_void_p] * 3 + [C.c_double] + [C.c_void_p] * 4
+        L.ref_pressure_drop.restype = C.c_double
+        L.ref_pressure_drop.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_last_error.restype = C.c_char_p
+        self.L = L
+        self.h = L.ref_open(str(mesh).encode(), re, bulk, wall.encode(), inlet.encode(), outlet.encode(), alpha_u)
+        if not self.h:
+            raise RuntimeError("reference setup failed: " + L.ref_last_error().decode())
+        self.nv = self.get("mom_row", np.int64).size - 1
+        self.nc = self.get("cell_det", np.float64).size
+
+    def close(self):
+        if self.h:
+            self.L.ref_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get(self, name: str, dtype) -> np.ndarray:
+        n = self.L.ref_size(self.h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n // np.dtype(dtype).itemsize, dtype=dtype)
+        self.L.ref_get(self.h, name.encode(), out.ctypes.data)
+        return out
+
+    def build_sa(self, terminal=1000, power_its=16, safety=1.5, low_fraction=0.05, damping=4.0 / 3.0):
+        if self.L.ref_build_sa(self.h, terminal, power_its, safety, low_fraction, damping) != 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+
+    def schur_apply(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.L.ref_schur_apply(self.h, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def assemble(self, U: np.ndarray, alpha_u: float):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        nnz = int(self.get("mom_row", np.int64)[-1])
+        aval = np.empty(nnz)
+        conv = np.empty(3 * self.nv)
+        rau = np.empty(self.nv)
+        delta = np.empty(self.nv)
+        rc = self.L.ref_assemble(self.h, U[0].ctypes.data, U[1].ctypes.data, U[2].ctypes.data, alpha_u,
+                                 aval.ctypes.data, conv.ctypes.data, rau.ctypes.data, delta.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+        return aval, conv.reshape(3, self.nv), rau, delta
+
+    def pressure_drop(self, p: np.ndarray) -> float:
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        return float(self.L.ref_pressure_drop(self.h, p.ctypes.data))
