@@ -1,0 +1,241 @@
+"""Pins the CPU oracle before it is trusted as the GPU checker.
+
+* reference tensors: exact polynomial integrals over the reference tet
+  (int lambda^a = a! b! c! d! 3! / (|a|+3)! * 1/6), which a dropped term,
+  wrong index or transposed operand in the quadrature restatement fails;
+* fine Schur pattern: brute force over cell pairs sharing a free DOF;
+* fine Schur values: action parity with the reference's matrix-free
+  cpu_schur_apply (g2_host.hpp:77) compiled in oracle/_ref;
+* convection assembly + relaxation: the reference host functions
+  host_assemble_central_g4 / host_finalize_relax_g4 compiled in oracle/_ref;
+* Gauss-Seidel: a row-by-row Python loop in colour order;
+* B / B^T: adjointness;
+* Krylov solvers: dense direct solves.
+"""
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import h8_oracle as O
+from oracle import refbind
+
+# ---------------------------------------------------------------- polynomials in barycentrics
+
+
+def _poly_basis():
+    """P1+BF3 functions as {exponent tuple: coeff} and their reference
+    gradients as lists of three such polynomials (d/dx_j lambda_i = GL[i][j])."""
+    GL = [[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+
+    def mono(e):
+        return {tuple(e): Fraction(1)}
+
+    def mul(p, q):
+        out = {}
+        for ea, ca in p.items():
+            for eb, cb in q.items():
+                e = tuple(x + y for x, y in zip(ea, eb))
+                out[e] = out.get(e, 0) + ca * cb
+        return out
+
+    def scale(p, s):
+        return {e: c * s for e, c in p.items()}
+
+    def add(p, q):
+        out = dict(p)
+        for e, c in q.items():
+            out[e] = out.get(e, 0) + c
+        return out
+
+    def deriv(p, j):
+        out = {}
+        for e, c in p.items():
+            for i in range(4):
+                if e[i] and GL[i][j]:
+                    e2 = list(e)
+                    e2[i] -= 1
+                    out[tuple(e2)] = out.get(tuple(e2), 0) + c * e[i] * GL[i][j]
+        return out
+
+    lam = [mono([1 if k == i else 0 for k in range(4)]) for i in range(4)]
+    val = list(lam)
+    for i in range(4):
+        js = [j for j in range(4) if j != i]
+        val.append(scale(mul(mul(lam[js[0]], lam[js[1]]), lam[js[2]]), 27))
+    grad = [[deriv(v, j) for j in range(3)] for v in val]
+    return val, grad, mul
+
+
+def _integrate(p):
+    s = Fraction(0)
+    for e, c in p.items():
+        s += c * Fraction(math.prod(math.factorial(x) for x in e) * 6, math.factorial(sum(e) + 3)) / 6
+    return s
+
+
+def test_reference_tensors_exact():
+    val, grad, mul = _poly_basis()
+    D, C = O.tensors()
+    for a, b in itertools.product(range(8), range(8)):
+        for j, k in itertools.product(range(3), range(3)):
+            assert abs(D[a, b, j, k] - float(_integrate(mul(grad[a][j], grad[b][k])))) < 1e-14
+    for a, mm, b in itertools.product(range(8), range(8), range(8)):
+        vm = mul(val[a], val[mm])
+        for j in range(3):
+            assert abs(C[a, mm, b, j] - float(_integrate(mul(vm, grad[b][j])))) < 1e-14
+
+
+def test_tensors_match_product_upload(mesh_dir):
+    """Central tensor uploaded by the product equals the oracle's (closed-form
+    pinned) tensor: checked through the diffusion values it produces."""
+    from nodals_amd._lib import H8
+
+    h = H8(["--mesh", mesh_dir["small"], "--max-outer", "10"], device=False)
+    case = O.case_from_product(h, with_hierarchy=False)
+    D, _ = O.tensors()
+    # diffusion CSR from the oracle tensor
+    aval = np.zeros(case.nnz)
+    nu = h.get("pipe", 5)[0]
+    for c in range(case.nc):
+        I = case.cell_invJ[c]
+        K = nu * case.cell_det[c] * np.einsum("abjk,jd,kd->ab", D, I, I)
+        for a in range(8):
+            r = case.cell_ref[c, a]
+            if r < 0:
+                continue
+            for b in range(8):
+                if case.cell_ref[c, b] >= 0:
+                    aval[case.mom_row[r] + case.cell_slot[c, 8 * a + b]] += K[a, b]
+    assert np.abs(aval - case.mom_diff).max() <= 1e-13 * np.abs(case.mom_diff).max()
+
+
+@pytest.fixture(scope="module")
+def small_case(mesh_dir):
+    from nodals_amd._lib import H8
+
+    h = H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--amg-smoother", "jacobi"], device=False)
+    return O.case_from_product(h)
+
+
+def test_fine_pattern_brute_force(small_case):
+    case = small_case
+    T = O.fine_topology(case)
+    owners = {}
+    for c in range(case.nc):
+        for g in case.cell_ref[c]:
+            if g >= 0:
+                owners.setdefault(int(g), set()).add(c)
+    want = [set() for _ in range(case.nc)]
+    for cells in owners.values():
+        for c in cells:
+            want[c] |= cells
+    for c in range(case.nc):
+        got = T["col"][T["row"][c]:T["row"][c + 1]]
+        assert list(got) == sorted(want[c])
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built")
+def test_fine_values_vs_reference_matrix_free(mesh_dir, small_case):
+    R = refbind.RefCase(mesh_dir["small"])
+    A, diag = O.fine_refresh(small_case, small_case.init_rau)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        x = rng.standard_normal(small_case.nc)
+        y = R.schur_apply(x)
+        assert np.linalg.norm(A @ x - y) <= 1e-13 * np.linalg.norm(y)
+    assert np.allclose(diag, A.diagonal(), rtol=0, atol=0)
+    assert abs(A - A.T).max() <= 1e-14 * abs(A).max()
+
+
+@pytest.mark.skipif(not refbind.available(), reason="oracle/_ref not built")
+def test_assembly_vs_reference_host(mesh_dir, small_case):
+    R = refbind.RefCase(mesh_dir["small"])
+    rng = np.random.default_rng(4)
+    U = rng.standard_normal((3, small_case.nv))
+    a_ref, c_ref, rau_ref, d_ref = R.assemble(U, 0.6)
+    _, T = O.tensors()
+    aval, conv, _ = O.assemble_central(small_case, U, T)
+    delta, diag, rau = O.finalize_relax(small_case, aval, 0.6)
+    assert np.abs(aval - a_ref).max() <= 1e-13 * np.abs(a_ref).max()
+    assert np.abs(conv - c_ref).max() <= 1e-13 * max(np.abs(c_ref).max(), 1e-300)
+    assert np.abs(rau - rau_ref).max() <= 1e-13 * np.abs(rau_ref).max()
+    assert np.abs(delta - d_ref).max() <= 1e-13 * np.abs(d_ref).max()
+
+
+def test_gauss_seidel_against_row_loop(small_case):
+    case = small_case
+    rng = np.random.default_rng(8)
+    _, T = O.tensors()
+    U = rng.standard_normal((3, case.nv))
+    aval, conv, _ = O.assemble_central(case, U, T)
+    delta, diag, _ = O.finalize_relax(case, aval, 0.5)
+    b = rng.standard_normal((3, case.nv))
+    A = O.mom_matrix(case, aval)
+    got = O.gs_sweep(case, A, diag, b, U, 0.9)
+    want = U.copy()
+    order = np.argsort(case.color, kind="stable")
+    # Rows of one colour do not couple, so a row-by-row sweep in colour
+    # order is the same multicolour sweep.
+    ip, ix, av = A.indptr, A.indices, A.data
+    for i in order:
+        for d in range(3):
+            s = sum(av[k] * want[d, ix[k]] for k in range(ip[i], ip[i + 1]) if ix[k] != i)
+            want[d, i] = (1 - 0.9) * want[d, i] + 0.9 * (b[d, i] - s) / diag[i]
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_b_bt_adjoint(small_case):
+    case = small_case
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(case.nc)
+    U = rng.standard_normal((3, case.nv))
+    lhs = float(np.sum(O.bt_apply(case, p) * U))
+    rhs = float(p @ (O.continuity(case, U) - case.fixed_div))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_krylov_vs_direct():
+    rng = np.random.default_rng(2)
+    n = 60
+    Q = rng.standard_normal((n, n))
+    A = sp.csr_matrix(Q @ Q.T + n * np.eye(n))
+    b = rng.standard_normal(n)
+    x_ref = np.linalg.solve(A.toarray(), b)
+    M = lambda r: r / A.diagonal()
+    for fn in (O.pcg, O.fgmres):
+        x, info = fn(A, M, b, 1e-12, 0.0, 500)
+        assert info["ok"] and np.linalg.norm(x - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+    x, info = O.richardson(A, lambda r: r / (2 * n + np.abs(Q @ Q.T).sum(1)), b, 1e-6, 0.0, 5000)
+    assert info["ok"]
+
+
+def test_vcycle_is_symmetric_positive(small_case):
+    """Jacobi V-cycle with an exact terminal solve is an SPD operator (PCG
+    needs it): z^T M w = w^T M z and w^T M w > 0."""
+    A, diag = O.fine_refresh(small_case, small_case.init_rau)
+    vc = O.VCycle(small_case.hier, O.Smoother(kind="jacobi"))
+    vc.set_fine(A, diag)
+    rng = np.random.default_rng(5)
+    z, w = rng.standard_normal((2, small_case.nc))
+    assert abs(z @ vc.apply(w) - w @ vc.apply(z)) <= 1e-12 * abs(z @ vc.apply(w))
+    assert w @ vc.apply(w) > 0
+    H = small_case.hier
+    Ac = H["A"][-1].toarray()
+    assert np.abs(H["Tinv"] @ Ac - np.eye(Ac.shape[0])).max() < 1e-9
+
+
+def test_simple_reaches_poiseuille(small_case):
+    """The oracle's SIMPLE loop converges on the pipe and its pressure drop is
+    the Hagen-Poiseuille 32 nu U L / D^2 within the discretisation error."""
+    ref = O.simple(small_case, p_solver="fgmres", p_rtol=1e-2, p_max=200, smoother=O.Smoother(kind="jacobi"),
+                   run_mode="converge", max_outer=3000, simple_tol=1e-7)
+    assert ref["converged"]
+    dp = O.pressure_drop_fit(small_case, ref["p"])
+    assert abs(dp - 16.0) / 16.0 < 0.02
+    # converged state satisfies the discrete continuity equation
+    r = O.continuity(small_case, ref["U"])
+    assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(small_case.fixed_div)
