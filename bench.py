@@ -67,6 +67,32 @@ def cpu_baseline(mesh: str, gpu_outer: int, iters: int) -> dict:
                        f"to converge")}
 
 
+def poisson_leg(n: int, reps: int) -> dict:
+    """Pressure-Poisson kernel bench: 64^3 Kuhn P1 Laplacian, SA V-cycle with 2
+    forward MCGS sweeps in FGMRES(30) to rtol 1e-8, hierarchy in FP64 / FP32 /
+    BF16-finest storage, FP64 outer residual; SpMV GB/s per level."""
+    from nodals_amd import p1
+    from nodals_amd._lib import System
+
+    row, col, val = p1.kuhn_p1_laplacian(n)
+    b = p1.poisson_rhs(row.size - 1)
+    out = {"rows": int(row.size - 1), "nnz": int(row[-1])}
+    for prec in ("fp64", "fp32", "bf16"):
+        S = System(row, col, val, f"precision={prec} smoother=mcgs sweeps=2 order=forward")
+        S.set_rhs(b)
+        ms, info = S.solve_timed("fgmres", 1e-8, 30, 500, reps)
+        levels = []
+        for lvl in range(-1, S.levels - 1):
+            t, by = S.time_spmv(lvl, 100)
+            levels.append({"level": lvl, "rows": S.n if lvl < 0 else S.level_rows[lvl],
+                           "ms": t, "GB_s": by / (t * 1e-3) / 1e9})
+        out[prec] = {"solve_ms": ms, "fgmres_its": info["its"], "true_rel_residual": info["rel"], "spmv": levels}
+        print(f"[bench] poisson {prec}: {ms:.2f} ms, its={info['its']} rel={info['rel']:.2e} "
+              f"fine SpMV {levels[0]['GB_s']:.0f} GB/s", file=sys.stderr, flush=True)
+        S.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +101,9 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=500)
+    ap.add_argument("--no-poisson", action="store_true")
+    ap.add_argument("--kernel-profile", action="store_true",
+                    help="only the fine SpMV and the copy calibration kernel (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -88,6 +117,16 @@ def main() -> int:
     if not (mesh / "faces").exists():
         meshgen.write_pipe_case(mesh, core=8, ring=5, nslab=30)
     mesh = str(mesh)
+
+    if a.kernel_profile:
+        h = H8(solver_args(mesh, "fp32"))
+        h.assemble(0.5)
+        h.pressure_refresh()
+        ms, by = h.time_kernel("fine_spmv", 20)
+        cms, cby = h.time_kernel("copy", 5)
+        print(json.dumps({"fine_spmv_ms": ms, "fine_spmv_bytes": by, "copy_ms": cms, "copy_bytes": cby}))
+        h.close()
+        return 0
 
     results = {}
     ctxs = {}
@@ -128,6 +167,7 @@ def main() -> int:
     outer = int(fp32["runs"][-1]["outer"])
     dp32 = fp32["runs"][-1]["pressure_drop"]
     dp64 = fp64["runs"][-1]["pressure_drop"]
+    poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(mesh, int(fp64["runs"][-1]["outer"]), a.cpu_iters)
@@ -157,6 +197,7 @@ def main() -> int:
             "hagen_poiseuille_drop": fp64["runs"][-1]["exact_pressure_drop"],
             "setup_ms": fp32["runs"][-1]["setup_ms"],
             "kernels": extra,
+            "poisson_64": poisson,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

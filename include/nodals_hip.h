@@ -132,10 +132,16 @@ int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double
  * sweeps=N omega=W order=symmetric|forward|backward terminal=N power_its=N. */
 int nodals_sys_create(int32_t n, const int64_t* row, const int32_t* col, const double* val, const char* options,
                       nodals_sys** out);
+/* Host hierarchy only (no HIP call); only nodals_sys_info/nodals_sys_get. */
+int nodals_sys_create_host(int32_t n, const int64_t* row, const int32_t* col, const double* val,
+                           const char* options, nodals_sys** out);
 void nodals_sys_destroy(nodals_sys* s);
 /* info[0..3] = rows, nnz, AMG explicit levels, terminal rows;
  * info[4+l] = rows of explicit level l (l < 12). */
 int nodals_sys_info(nodals_sys* s, int64_t info[16]);
+/* Host hierarchy arrays: agg<l>, P<l>_row|col|val, A<l>_row|col|val,
+ * terminal_inv, level_lambda, fine_lambda. */
+int nodals_sys_get(nodals_sys* s, const char* name, void* host, int64_t count);
 int nodals_sys_spmv(nodals_sys* s, const double* x, double* y);
 /* One preconditioner application z = M^-1 r. */
 int nodals_sys_precond(nodals_sys* s, const double* r, double* z);

@@ -49,7 +49,8 @@ EXPORTS = [
     "nodals_h8_pressure_update", "nodals_h8_run", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
     "nodals_h8_precond",
     "nodals_h8_time_kernel",
-    "nodals_sys_create", "nodals_sys_destroy", "nodals_sys_info", "nodals_sys_spmv", "nodals_sys_precond",
+    "nodals_sys_create", "nodals_sys_create_host", "nodals_sys_get", "nodals_sys_destroy", "nodals_sys_info",
+    "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
 ]
 
@@ -84,6 +85,8 @@ def lib() -> C.CDLL:
     L.nodals_h8_precond.argtypes = [vp, vp, vp]
     L.nodals_h8_time_kernel.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
+    L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
+    L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     L.nodals_sys_destroy.argtypes = [vp]
     L.nodals_sys_destroy.restype = None
     L.nodals_sys_info.argtypes = [vp, C.POINTER(C.c_int64)]
@@ -208,14 +211,14 @@ class H8:
 class System:
     """Explicit FP64 SPD CSR system with a device SA hierarchy."""
 
-    def __init__(self, row: np.ndarray, col: np.ndarray, val: np.ndarray, options: str = ""):
+    def __init__(self, row: np.ndarray, col: np.ndarray, val: np.ndarray, options: str = "", device: bool = True):
         self._row = np.ascontiguousarray(row, dtype=np.int64)
         self._col = np.ascontiguousarray(col, dtype=np.int32)
         self._val = np.ascontiguousarray(val, dtype=np.float64)
         n = self._row.size - 1
         h = C.c_void_p()
-        _check(lib().nodals_sys_create(n, _ptr(self._row), _ptr(self._col), _ptr(self._val), options.encode(),
-                                       C.byref(h)))
+        create = lib().nodals_sys_create if device else lib().nodals_sys_create_host
+        _check(create(n, _ptr(self._row), _ptr(self._col), _ptr(self._val), options.encode(), C.byref(h)))
         self._h = h
         self.n = n
         info = (C.c_int64 * 16)()
@@ -237,6 +240,33 @@ class System:
         y = np.empty_like(x)
         _check(lib().nodals_sys_spmv(self._h, _ptr(x), _ptr(y)))
         return y
+
+    def get(self, name: str, count: int, dtype=np.float64) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        _check(lib().nodals_sys_get(self._h, name.encode(), _ptr(out), count))
+        return out
+
+    def hierarchy(self) -> dict:
+        """Host SA hierarchy as scipy CSR matrices (oracle V-cycle input)."""
+        import scipy.sparse as sp
+
+        H = {"P": [], "A": []}
+        nfine = self.n
+        for l in range(self.levels):
+            pr = self.get(f"P{l}_row", nfine + 1, np.int64)
+            pc = self.get(f"P{l}_col", int(pr[-1]), np.int32)
+            pv = self.get(f"P{l}_val", int(pr[-1]))
+            nc = self.level_rows[l]
+            ar = self.get(f"A{l}_row", nc + 1, np.int64)
+            ac = self.get(f"A{l}_col", int(ar[-1]), np.int32)
+            av = self.get(f"A{l}_val", int(ar[-1]))
+            H["P"].append(sp.csr_matrix((pv, pc, pr), shape=(nfine, nc)))
+            H["A"].append(sp.csr_matrix((av, ac, ar), shape=(nc, nc)))
+            nfine = nc
+        H["Tinv"] = self.get("terminal_inv", nfine * nfine).reshape(nfine, nfine)
+        H["fine_lambda"] = float(self.get("fine_lambda", 1)[0])
+        H["level_lambda"] = self.get("level_lambda", self.levels)
+        return H
 
     def precond(self, r: np.ndarray) -> np.ndarray:
         r = np.ascontiguousarray(r, dtype=np.float64)

@@ -124,6 +124,12 @@ class DevAmg {
   int levels() const { return (int)L_.size(); }
   const Level& level(int l) const { return L_[(std::size_t)l]; }
 
+  // apply() replayed from a captured HIP graph (b and z must stay fixed
+  // buffers; the finest values behind the fine view may change in place).
+  void applyGraphed(hipStream_t s, const TV* b, TV* z) {
+    graph_.run(s, graphKey({b, z, fine_.val, fine_.diag}), [&]() { apply(s, b, z); });
+  }
+
   // z = M^{-1} b on the finest level.
   void apply(hipStream_t s, const TV* b, TV* z) {
     smoothFine(s, b, z);
@@ -274,6 +280,7 @@ class DevAmg {
   DBuf<TV> tinv_;
   int termN_ = 0;
   DBuf<TV> fr_, ftmp_, fcorr_;
+  GraphCache graph_;
 };
 
 }  // namespace dev

@@ -318,6 +318,17 @@ int nodals_sys_create(int32_t n, const int64_t* row, const int32_t* col, const d
   });
 }
 
+int nodals_sys_create_host(int32_t n, const int64_t* row, const int32_t* col, const double* val,
+                           const char* options, nodals_sys** out) {
+  return guarded([&]() {
+    nodals::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    auto h = std::make_unique<nodals_sys>();
+    h->s = std::make_unique<nodals::SysSolver>(n, row, col, val, options ? options : "", false);
+    *out = h.release();
+  });
+}
+
 void nodals_sys_destroy(nodals_sys* s) {
   try {
     delete s;
@@ -334,6 +345,44 @@ int nodals_sys_info(nodals_sys* h, int64_t info[16]) {
     info[2] = s.levels();
     info[3] = s.hierarchy().terminalN;
     for (int l = 0; l < s.levels() && l < 12; ++l) info[4 + l] = s.hierarchy().A[(std::size_t)l].n;
+  });
+}
+
+int nodals_sys_get(nodals_sys* h, const char* name, void* host, int64_t count) {
+  return guarded([&]() {
+    auto& s = sys(h);
+    nodals::require(name != nullptr, "null name");
+    const std::string n = name;
+    const auto& H = s.hierarchy();
+    if (n.rfind("agg", 0) == 0) {
+      const std::size_t l = (std::size_t)std::stoi(n.substr(3));
+      nodals::require(l < H.agg.size(), "aggregate level out of range");
+      copyOut(H.agg[l].id, host, count);
+    } else if (n == "terminal_inv") {
+      copyOut(H.terminalInv, host, count);
+    } else if (n == "level_lambda") {
+      copyOut(H.levelLambda, host, count);
+    } else if (n == "fine_lambda") {
+      copyOut(std::vector<double>{H.fineLambda}, host, count);
+    } else if (n[0] == 'P' || n[0] == 'A') {
+      const auto u = n.find('_');
+      nodals::require(u != std::string::npos, "unknown field " + n);
+      const std::size_t l = (std::size_t)std::stoi(n.substr(1, u - 1));
+      const std::string part = n.substr(u + 1);
+      if (n[0] == 'P') {
+        nodals::require(l < H.P.size(), "transfer level out of range");
+        if (part == "row") copyOut(H.P[l].row, host, count);
+        else if (part == "col") copyOut(H.P[l].col, host, count);
+        else copyOut(H.P[l].val, host, count);
+      } else {
+        nodals::require(l < H.A.size(), "operator level out of range");
+        if (part == "row") copyOut(H.A[l].row, host, count);
+        else if (part == "col") copyOut(H.A[l].col, host, count);
+        else copyOut(H.A[l].val, host, count);
+      }
+    } else {
+      nodals::fail("unknown field " + n);
+    }
   });
 }
 

@@ -4,7 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <utility>
 #include <vector>
@@ -77,6 +80,75 @@ class DBuf {
   T* p_ = nullptr;
   std::size_t n_ = 0;
 };
+
+// A captured launch sequence replayed with one hipGraphLaunch.  The sequence
+// is captured on first use and recaptured when its key (the pointers and
+// scalars baked into the kernel arguments) changes.  Captured bodies use
+// kernels only (no memset/memcpy nodes).  NODALS_HIP_GRAPHS=0 runs the body
+// eagerly instead.
+class GraphCache {
+ public:
+  GraphCache() = default;
+  ~GraphCache() { reset(); }
+  GraphCache(const GraphCache&) = delete;
+  GraphCache& operator=(const GraphCache&) = delete;
+
+  static bool enabled() {
+    static const bool on = [] {
+      const char* e = std::getenv("NODALS_HIP_GRAPHS");
+      return !(e && (std::strcmp(e, "0") == 0 || std::strcmp(e, "off") == 0));
+    }();
+    return on;
+  }
+
+  void reset() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    key_.clear();
+  }
+
+  template <class F>
+  void run(hipStream_t s, const std::string& key, F&& body) {
+    if (!enabled()) {
+      body();
+      return;
+    }
+    if (!exec_ || key != key_) {
+      reset();
+      NODALS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        body();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      NODALS_HIP(hipStreamEndCapture(s, &graph_));
+      NODALS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+      key_ = key;
+    }
+    NODALS_HIP(hipGraphLaunch(exec_, s));
+  }
+
+ private:
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  std::string key_;
+};
+
+inline std::string graphKey(std::initializer_list<const void*> ptrs, double scalar = 0.0) {
+  std::string k;
+  char buf[40];
+  for (const void* p : ptrs) {
+    std::snprintf(buf, sizeof(buf), "%p;", p);
+    k += buf;
+  }
+  std::snprintf(buf, sizeof(buf), "%.17g", scalar);
+  return k + buf;
+}
 
 constexpr int kBlock = 256;
 inline int blocksFor(long long n, int block = kBlock) { return (int)((n + block - 1) / block); }

@@ -353,6 +353,8 @@ void H8Solver::upload() {
   pcx_.alloc((std::size_t)nc_);
   tmpA_.alloc((std::size_t)std::max(nc_, nv_ * 3));
   tmpB_.alloc((std::size_t)std::max(nc_, nv_ * 3));
+  rd_.alloc((std::size_t)nc_);
+  zd_.alloc((std::size_t)nc_);
 
   // fine Schur CSR (cell rows unchanged; velocity ids renumbered)
   frow_.upload(F_.row);
@@ -488,21 +490,25 @@ const char* H8Solver::momentumWork(const std::string& mode, double omega, int ou
       NODALS_LAUNCHED();
     }
   };
+  // One colour launch per colour; the whole sweep is replayed from a graph.
+  const std::string key = graphKey({u_.data(), aval_.data(), brhs_.data()}, omega);
   if (mode == "fgs1") {
-    sweep(false);
+    sweepGraph_[0].run(stream_, key + "F", [&]() { sweep(false); });
     return "FWD";
   }
   if (mode == "altgs1") {
     if (outer & 1) {
-      sweep(false);
+      sweepGraph_[0].run(stream_, key + "F", [&]() { sweep(false); });
       return "FWD";
     }
-    sweep(true);
+    sweepGraph_[1].run(stream_, key + "B", [&]() { sweep(true); });
     return "BWD";
   }
   if (mode == "sgs1") {
-    sweep(false);
-    sweep(true);
+    sweepGraph_[0].run(stream_, key + "S", [&]() {
+      sweep(false);
+      sweep(true);
+    });
     return "FWD+BWD";
   }
   fail("unknown fixed momentum work mode " + mode);
@@ -596,15 +602,19 @@ void H8Solver::precond(const double* r, double* z) {
 }
 
 void H8Solver::precondApply(hipStream_t s, const double* r, double* z) {
+  // The V-cycle runs on fixed buffers so its launch sequence is replayed from
+  // one captured graph; only the copy/cast in and out follow the caller.
   if (amg64_) {
-    amg64_->apply(s, r, z);
+    cast<double, double>(s, nc_, r, rd_.data());
+    amg64_->applyGraphed(s, rd_.data(), zd_.data());
+    cast<double, double>(s, nc_, zd_.data(), z);
   } else if (amg32_) {
     cast<float, double>(s, nc_, r, rf_.data());
-    amg32_->apply(s, rf_.data(), zf_.data());
+    amg32_->applyGraphed(s, rf_.data(), zf_.data());
     cast<double, float>(s, nc_, zf_.data(), z);
   } else {
     cast<float, double>(s, nc_, r, rf_.data());
-    amgb_->apply(s, rf_.data(), zf_.data());
+    amgb_->applyGraphed(s, rf_.data(), zf_.data());
     cast<double, float>(s, nc_, zf_.data(), z);
   }
 }
@@ -883,6 +893,21 @@ double H8Solver::timeKernel(const std::string& kernel, int reps, double* bytes) 
   } else if (kernel == "assemble") {
     launch = [this]() { assemble(opt_.alphaU, 1.0); };
     by = b.convLocal + b.momRows;
+  } else if (kernel == "copy") {
+    // Plain streaming copy of 64 Mi doubles each way (512 MiB, far beyond the
+    // caches): the calibration kernel for HBM byte counters.
+    const long long n = 64ll << 20;
+    if (calibA_.size() != (std::size_t)n) {
+      calibA_.alloc((std::size_t)n);
+      calibB_.alloc((std::size_t)n);
+      fill<double>(stream_, n, calibA_.data(), 1.0);
+    }
+    launch = [this, n]() {
+      hipLaunchKernelGGL(k_scale_into<double>, blocksFor(n), 256, 0, stream_, n, 1.0, calibA_.data(),
+                         calibB_.data());
+      NODALS_LAUNCHED();
+    };
+    by = 16.0 * (double)n;
   } else if (kernel == "continuity") {
     launch = [this]() {
       hipLaunchKernelGGL(k_continuity, kReduceParts, 256, 0, stream_, nc_, cref_.data(), bcoef_.data(),

@@ -165,7 +165,8 @@ class H8Solver {
   DBuf<double> fval_, fdiag_, fl1_;
   DBuf<float> fvalf_, fdiagf_, fl1f_, rf_, zf_;
   DBuf<bf16> fvalb_;
-  DBuf<double> tmpA_, tmpB_;
+  DBuf<double> tmpA_, tmpB_, calibA_, calibB_, rd_, zd_;
+  GraphCache sweepGraph_[2];  // [0] forward / symmetric sweep, [1] backward sweep
   std::unique_ptr<dev::DevAmg<double, double>> amg64_;
   std::unique_ptr<dev::DevAmg<float, float>> amg32_;
   std::unique_ptr<dev::DevAmg<bf16, float>> amgb_;

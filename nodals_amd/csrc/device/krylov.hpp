@@ -175,14 +175,18 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
       for (int i = 0; i <= j; ++i) c.c[i] = h[(std::size_t)i];
       hipLaunchKernelGGL(k_multi_axpy, blocksFor(N), 256, 0, s, N, j + 1, V, c, w, -1.0);
       NODALS_LAUNCHED();
-      R.multidot(s, N, j + 1, V, w, h2.data());
+      // second pass fused with ||w||^2; after it ||w'||^2 = ||w||^2 - ||h2||^2
+      // (V orthonormal, small correction), so no separate norm reduction.
+      R.multidot(s, N, j + 1, V, w, h2.data(), true);
+      double hh = 0.0;
       for (int i = 0; i <= j; ++i) {
         c.c[i] = h2[(std::size_t)i];
         h[(std::size_t)i] += h2[(std::size_t)i];
+        hh += h2[(std::size_t)i] * h2[(std::size_t)i];
       }
       hipLaunchKernelGGL(k_multi_axpy, blocksFor(N), 256, 0, s, N, j + 1, V, c, w, -1.0);
       NODALS_LAUNCHED();
-      const double hn = std::sqrt(R.dot(s, N, w, w));
+      const double hn = std::sqrt(std::max(h2[(std::size_t)j + 1] - hh, 0.0));
       require(std::isfinite(hn), "FGMRES Arnoldi norm non-finite", kErrNonFinite);
       // Hessenberg column j with the previous rotations applied.
       for (int i = 0; i <= j; ++i) H[(std::size_t)i * m + j] = h[(std::size_t)i];

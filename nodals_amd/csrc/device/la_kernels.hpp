@@ -107,23 +107,29 @@ __global__ void __launch_bounds__(256) k_dot_part(long long n, const TA* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// K dot products against one vector: part[k*gridDim + block] = sum V_k . w
-template <int KMAX>
+// K dot products against one vector: part[k*gridDim + block] = sum V_k . w;
+// with NORM, entry k holds w . w (one pass over w for Gram-Schmidt + norm).
+template <int KMAX, bool NORM>
 __global__ void __launch_bounds__(256) k_multidot_part(long long n, int k, const double* __restrict__ V,
                                                        const double* __restrict__ w, double* __restrict__ part) {
   __shared__ double sh[4];
-  double s[KMAX];
+  double s[KMAX + 1];
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) s[j] = 0.0;
+  for (int j = 0; j <= KMAX; ++j) s[j] = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double wi = w[i];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
       if (j < k) s[j] += V[(long long)j * n + i] * wi;
+    if (NORM) s[KMAX] += wi * wi;
   }
   for (int j = 0; j < k; ++j) {
     const double t = blockSum256(s[j], sh);
     if (threadIdx.x == 0) part[(long long)j * gridDim.x + blockIdx.x] = t;
+  }
+  if (NORM) {
+    const double t = blockSum256(s[KMAX], sh);
+    if (threadIdx.x == 0) part[(long long)k * gridDim.x + blockIdx.x] = t;
   }
 }
 
@@ -329,15 +335,21 @@ class Reducer {
     NODALS_HIP(hipStreamSynchronize(s));
     return host_[0];
   }
-  // out[j] = V_j . w for j < k (V column-major with stride n).
-  void multidot(hipStream_t s, long long n, int k, const double* V, const double* w, double* out) {
-    if (k > maxVec_ || k > 32) fail("multidot width exceeds reducer capacity", kErrInternal);
-    hipLaunchKernelGGL(k_multidot_part<32>, kReduceParts, 256, 0, s, n, k, V, w, part_.data());
+  // out[j] = V_j . w for j < k (V column-major with stride n); with
+  // withNorm, out[k] = w . w from the same pass.
+  void multidot(hipStream_t s, long long n, int k, const double* V, const double* w, double* out,
+                bool withNorm = false) {
+    const int outs = k + (withNorm ? 1 : 0);
+    if (outs > maxVec_ || k > 32) fail("multidot width exceeds reducer capacity", kErrInternal);
+    if (withNorm)
+      hipLaunchKernelGGL((k_multidot_part<32, true>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
+    else
+      hipLaunchKernelGGL((k_multidot_part<32, false>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
     NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_final, k, 256, 0, s, kReduceParts, part_.data(), host_);
+    hipLaunchKernelGGL(k_reduce_final, outs, 256, 0, s, kReduceParts, part_.data(), host_);
     NODALS_LAUNCHED();
     NODALS_HIP(hipStreamSynchronize(s));
-    for (int j = 0; j < k; ++j) out[j] = host_[j];
+    for (int j = 0; j < outs; ++j) out[j] = host_[j];
   }
   double* partials() { return part_.data(); }
   double* hostOut() { return host_; }

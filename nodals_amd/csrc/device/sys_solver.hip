@@ -18,7 +18,7 @@ namespace nodals {
 using namespace dev;
 
 SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, const double* val,
-                     const std::string& options) {
+                     const std::string& options, bool withDevice) {
   require(n > 0 && row && col && val, "sys_create: empty system");
   std::istringstream is(options);
   std::string kv;
@@ -62,6 +62,8 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
   H_ = buildSaHierarchyCsr(A_, ao_);
   const auto t1 = std::chrono::steady_clock::now();
   setupHostMs_ = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  n_ = n;
+  if (!withDevice) return;
 
   NODALS_HIP(hipSetDevice(0));
   NODALS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -83,6 +85,8 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
   b_.alloc((std::size_t)n);
   x_.alloc((std::size_t)n);
   y_.alloc((std::size_t)n);
+  rd_.alloc((std::size_t)n);
+  zd_.alloc((std::size_t)n);
   Coloring fc;
   if (so_.kind == "mcgs") fc = greedyColoring(n, A_.row.data(), A_.col.data());
   const Coloring* pfc = so_.kind == "mcgs" ? &fc : nullptr;
@@ -148,19 +152,22 @@ void SysSolver::apply(hipStream_t s, const double* x, double* y) {
 
 void SysSolver::precond(hipStream_t s, const double* r, double* z) {
   if (amg64_) {
-    amg64_->apply(s, r, z);
+    cast<double, double>(s, n_, r, rd_.data());
+    amg64_->applyGraphed(s, rd_.data(), zd_.data());
+    cast<double, double>(s, n_, zd_.data(), z);
   } else if (amg32_) {
     cast<float, double>(s, n_, r, rf_.data());
-    amg32_->apply(s, rf_.data(), zf_.data());
+    amg32_->applyGraphed(s, rf_.data(), zf_.data());
     cast<double, float>(s, n_, zf_.data(), z);
   } else {
     cast<float, double>(s, n_, r, rf_.data());
-    amgb_->apply(s, rf_.data(), zf_.data());
+    amgb_->applyGraphed(s, rf_.data(), zf_.data());
     cast<double, float>(s, n_, zf_.data(), z);
   }
 }
 
 std::vector<double> SysSolver::spmvHost(const std::vector<double>& x) {
+  requireDevice();
   require((int)x.size() == n_, "spmv: size mismatch");
   x_.upload(x);
   apply(stream_, x_.data(), y_.data());
@@ -169,6 +176,7 @@ std::vector<double> SysSolver::spmvHost(const std::vector<double>& x) {
 }
 
 std::vector<double> SysSolver::precondHost(const std::vector<double>& r) {
+  requireDevice();
   require((int)r.size() == n_, "precond: size mismatch");
   x_.upload(r);
   precond(stream_, x_.data(), y_.data());
@@ -177,11 +185,13 @@ std::vector<double> SysSolver::precondHost(const std::vector<double>& r) {
 }
 
 void SysSolver::setRhs(const std::vector<double>& b) {
+  requireDevice();
   require((int)b.size() == n_, "rhs: size mismatch");
   b_.upload(b);
 }
 
 KrylovResult SysSolver::solveResident(const std::string& solver, double rtol, int restart, int maxit) {
+  requireDevice();
   DevOp A = [this](hipStream_t s, const double* x, double* y) { apply(s, x, y); };
   DevOp M = [this](hipStream_t s, const double* r, double* z) { precond(s, r, z); };
   if (solver == "fgmres") return fgmres(stream_, red_, n_, A, M, b_.data(), rtol, 0.0, maxit, restart, kw_);
@@ -217,6 +227,7 @@ double SysSolver::timedSolve(const std::string& solver, double rtol, int restart
 }
 
 double SysSolver::timeSpmv(int level, int reps, double* bytes) {
+  requireDevice();
   require(reps >= 1, "reps must be >= 1");
   std::function<void()> launch;
   double by = 0;

@@ -14,7 +14,9 @@ namespace nodals {
 
 class SysSolver {
  public:
-  SysSolver(int n, const std::int64_t* row, const std::int32_t* col, const double* val, const std::string& options);
+  // withDevice=false stops after the host hierarchy (no HIP call).
+  SysSolver(int n, const std::int64_t* row, const std::int32_t* col, const double* val, const std::string& options,
+            bool withDevice = true);
   ~SysSolver();
   SysSolver(const SysSolver&) = delete;
   SysSolver& operator=(const SysSolver&) = delete;
@@ -35,6 +37,9 @@ class SysSolver {
   double timeSpmv(int level, int reps, double* bytes);
 
  private:
+  void requireDevice() const {
+    require(stream_ != nullptr, "system context was created without a device (host setup only)");
+  }
   dev::KrylovResult solveResident(const std::string& solver, double rtol, int restart, int maxit);
   void apply(hipStream_t s, const double* x, double* y);
   void precond(hipStream_t s, const double* r, double* z);
@@ -50,7 +55,7 @@ class SysSolver {
   dev::Reducer red_;
   dev::KrylovWork kw_;
   DBuf<int> row_, col_;
-  DBuf<double> val_, diag_, l1_, b_, x_, y_;
+  DBuf<double> val_, diag_, l1_, b_, x_, y_, rd_, zd_;
   DBuf<float> valf_, diagf_, l1f_, rf_, zf_;
   DBuf<bf16> valb_;
   std::unique_ptr<dev::DevAmg<double, double>> amg64_;

@@ -134,6 +134,9 @@ void* ref_open(const char* mesh_dir, double re, double bulk, const char* wall,
 
 void ref_close(void* h) { delete static_cast<RefCase*>(h); }
 
+// Empty handle for the mesh-free entry points (ref_sa_from_csr).
+void* ref_new() { return new RefCase(); }
+
 // Byte size of a published array, or -1.
 long long ref_size(void* h, const char* name) {
   auto* R = static_cast<RefCase*>(h);
@@ -199,6 +202,57 @@ int ref_assemble(void* h, const double* u0, const double* u1, const double* u2, 
     for (int k = 0; k < 3; ++k) std::memcpy(conv + k * n, C[k].data(), n * sizeof(double));
     std::memcpy(rau, r.data(), n * sizeof(double));
     std::memcpy(delta, d.data(), n * sizeof(double));
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// SA hierarchy of an explicit CSR operator built from the reference's
+// coarse-level functions, composed as the level loop of build_sa_hierarchy
+// (g5_sa_host.hpp:1570-1577) applied from level 0: sa_power_csr,
+// aggregate_graph(csr_graph), build_sa_explicit_transfer_reference,
+// build_sa_ptap_reference, dense_inverse_cholesky.  Results are published
+// as csr_agg<l>, csr_P<l>_*, csr_A<l>_*, csr_terminal_inv, csr_dims.
+int ref_sa_from_csr(void* h, int n, const long long* row, const int* col, const double* val, int terminal,
+                    int power_its, double safety, double damping) {
+  try {
+    auto* R = static_cast<RefCase*>(h);
+    CSRHost A;
+    A.n = n;
+    A.row.assign(row, row + n + 1);
+    A.col.assign(col, col + row[n]);
+    A.val.assign(val, val + row[n]);
+    A.diag.assign((std::size_t)n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (long long k = row[i]; k < row[i + 1]; ++k)
+        if (col[k] == i) A.diag[(std::size_t)i] = val[k];
+    std::vector<CSRHost> levels;
+    std::vector<SATransferHost> P;
+    std::vector<AggResult> agg;
+    const CSRHost* cur = &A;
+    int lev = 0;
+    do {
+      const double lam = sa_power_csr(*cur, power_its, safety);
+      agg.push_back(aggregate_graph(csr_graph(*cur), 16, 6, 18));
+      P.push_back(build_sa_explicit_transfer_reference(*cur, agg.back(), lam, damping, 8, lev));
+      levels.push_back(build_sa_ptap_reference(*cur, P.back(), lev + 1));
+      cur = &levels.back();
+      ++lev;
+    } while (levels.back().n > terminal);
+    for (std::size_t l = 0; l < levels.size(); ++l) {
+      put(*R, "csr_agg" + std::to_string(l), agg[l].id);
+      put(*R, "csr_P" + std::to_string(l) + "_row", P[l].row);
+      put(*R, "csr_P" + std::to_string(l) + "_col", P[l].col);
+      put(*R, "csr_P" + std::to_string(l) + "_val", P[l].val);
+      put(*R, "csr_A" + std::to_string(l) + "_row", levels[l].row);
+      put(*R, "csr_A" + std::to_string(l) + "_col", levels[l].col);
+      put(*R, "csr_A" + std::to_string(l) + "_val", levels[l].val);
+    }
+    put(*R, "csr_terminal_inv", dense_inverse_cholesky(levels.back()));
+    std::vector<std::int32_t> dims = {(int)levels.size(), levels.back().n};
+    put(*R, "csr_dims", dims);
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();

@@ -80,6 +80,32 @@ class RefCase:
             raise RuntimeError(self.L.ref_last_error().decode())
         return aval, conv.reshape(3, self.nv), rau, delta
 
+    @classmethod
+    def blank(cls) -> "RefCase":
+        """Handle without a mesh, for ref_sa_from_csr."""
+        self = cls.__new__(cls)
+        L = C.CDLL(str(REF_LIB))
+        L.ref_new.restype = C.c_void_p
+        L.ref_close.argtypes = [C.c_void_p]
+        L.ref_size.restype = C.c_longlong
+        L.ref_size.argtypes = [C.c_void_p, C.c_char_p]
+        L.ref_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        L.ref_sa_from_csr.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_double, C.c_double]
+        L.ref_last_error.restype = C.c_char_p
+        self.L = L
+        self.h = L.ref_new()
+        return self
+
+    def sa_from_csr(self, row, col, val, terminal=1000, power_its=16, safety=1.5, damping=4.0 / 3.0):
+        row = np.ascontiguousarray(row, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        rc = self.L.ref_sa_from_csr(self.h, row.size - 1, row.ctypes.data, col.ctypes.data, val.ctypes.data,
+                                    terminal, power_its, safety, damping)
+        if rc != 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+
     def pressure_drop(self, p: np.ndarray) -> float:
         p = np.ascontiguousarray(p, dtype=np.float64)
         return float(self.L.ref_pressure_drop(self.h, p.ctypes.data))

@@ -126,6 +126,61 @@ def test_vcycle_matches_oracle(pipe):
     assert rel(h.precond(r), vc.apply(r)) < 1e-12
 
 
+_EAGER_SCRIPT = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+from nodals_amd._lib import H8
+h = H8(["--mesh", sys.argv[2], "--max-outer", "10", "--amg-smoother", "jacobi", "--verbose", "0"])
+rng = np.random.default_rng(7)
+U = rng.standard_normal((3, h.info["nv"])) * 0.3
+for d in range(3):
+    h.set(f"u{d}", U[d])
+h.assemble(0.5)
+h.pressure_refresh()
+r = np.random.default_rng(2).standard_normal(h.info["cells"])
+z = h.precond(r)
+h.momentum_work("fgs1", 1.0, 1)
+np.save(sys.argv[3], np.concatenate([z, h.get("u0", h.info["nv"])]))
+"""
+
+
+def test_graph_replays_match_eager(mesh_dir, tmp_path):
+    """V-cycle and GS-sweep graphs replayed several times, with eager work and
+    synchronisations in between, equal the eager (graphs disabled) launch
+    sequence bit for bit on every replay."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    from nodals_amd._lib import H8
+
+    root = str(Path(__file__).resolve().parents[1])
+    script = tmp_path / "eager.py"
+    script.write_text(_EAGER_SCRIPT)
+    env = dict(os.environ, NODALS_HIP_GRAPHS="0")
+    print("\n[gpu] eager (graphs disabled) reference in a subprocess", flush=True)
+    subprocess.run([sys.executable, str(script), root, mesh_dir["small"], str(tmp_path / "eager.npy")], env=env,
+                   check=True, timeout=240)
+    eager = np.load(tmp_path / "eager.npy")
+    h = H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--amg-smoother", "jacobi", "--verbose", "0"])
+    nc, nv = h.info["cells"], h.info["nv"]
+    rng = np.random.default_rng(7)
+    U = rng.standard_normal((3, nv)) * 0.3
+    r = np.random.default_rng(2).standard_normal(nc)
+    for rep in range(4):
+        for d in range(3):
+            h.set(f"u{d}", U[d])
+        h.assemble(0.5)
+        h.pressure_refresh()
+        h.continuity()  # eager work + host synchronisation between replays
+        z = h.precond(r)
+        assert np.array_equal(z, eager[:nc]), rep
+        h.momentum_work("fgs1", 1.0, 1)
+        assert np.array_equal(h.get("u0", nv), eager[nc:]), rep
+    h.close()
+
+
 @pytest.mark.parametrize("solver", ["pcg", "richardson", "fgmres"])
 def test_pressure_solve(pipe, solver):
     h, case = pipe
