@@ -93,6 +93,31 @@ def poisson_leg(n: int, reps: int) -> dict:
     return out
 
 
+def reset(h) -> None:
+    """Start state of a step: fluid at rest, zero pressure (set before the timer)."""
+    import numpy as np
+    h.set("p", np.zeros(h.info["cells"]))
+    for d in range(3):
+        h.set(f"u{d}", np.zeros(h.info["nv"]))
+
+
+def traffic_per_launch() -> dict | None:
+    """HBM bytes per fine-SpMV launch from the committed rocprofv3 counter
+    summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what
+    the same counter reads for a streaming copy of known size."""
+    f = ROOT / "profiles" / "fine_spmv_pipe40k_pmc.txt"
+    if not f.exists():
+        return None
+    vals = {}
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = float(v)
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,16 +159,11 @@ def main() -> int:
         h = H8(solver_args(mesh, prec))
         ctxs[prec] = h
         for _ in range(a.warmup):
-            h.set("p", __import__("numpy").zeros(h.info["cells"]))
-            for d in range(3):
-                h.set(f"u{d}", __import__("numpy").zeros(h.info["nv"]))
+            reset(h)
             h.run()
         times, runs = [], []
         for _ in range(a.steps):
-            import numpy as np
-            h.set("p", np.zeros(h.info["cells"]))
-            for d in range(3):
-                h.set(f"u{d}", np.zeros(h.info["nv"]))
+            reset(h)
             t0 = time.perf_counter()
             r = h.run()
             times.append(time.perf_counter() - t0)
@@ -154,8 +174,18 @@ def main() -> int:
 
     import numpy as np
     h = ctxs["fp32"]
-    ms, by = h.time_kernel("fine_spmv", a.spmv_reps)
+    # Roofline kernel: the FP64 fine Schur SpMV (the Krylov operator).  Its
+    # launches inside one more full solve of the same workload are bracketed
+    # by HIP events on the solver stream (kept out of the timed steps, whose
+    # wall clock the extra events would perturb).
+    reset(h)
+    h.probe_start()
+    h.run()
+    launches, probe_ms = h.probe_stop()
+    ms = probe_ms / max(launches, 1)
+    _, by = h.time_kernel("fine_spmv", 1)
     achieved = by / (ms * 1e-3) / 1e9
+    warm_ms, _ = h.time_kernel("fine_spmv", a.spmv_reps)
     extra = {}
     for k in ("refresh", "mom_gs", "assemble", "continuity"):
         kms, kby = h.time_kernel(k, 100)
@@ -171,6 +201,7 @@ def main() -> int:
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(mesh, int(fp64["runs"][-1]["outer"]), a.cpu_iters)
+    traffic = traffic_per_launch()
     line = {
         "metric": METRIC,
         "value": value,
@@ -196,14 +227,21 @@ def main() -> int:
             "pressure_drop_rel_diff": abs(dp32 - dp64) / abs(dp64),
             "hagen_poiseuille_drop": fp64["runs"][-1]["exact_pressure_drop"],
             "setup_ms": fp32["runs"][-1]["setup_ms"],
+            "stage_ms_per_simple_fp32_amg": {
+                k: fp32["runs"][-1][f"{k}_ms"] / max(outer, 1)
+                for k in ("assembly", "momentum", "continuity", "pressure", "pupdate")},
             "kernels": extra,
             "poisson_64": poisson,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<16,double,double,false,false> (fine P0 Schur SpMV)",
-                     "kernel_ms": ms, "bytes_per_launch": by},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic["traffic_bytes_per_launch"],
+                     "kernel": "k_spmv<32,double,double,false,false> (fine P0 Schur SpMV, FP64)",
+                     "kernel_ms": ms, "launches": launches, "bytes_per_launch": by,
+                     "warm_loop_kernel_ms": warm_ms,
+                     "traffic_calibration": None if traffic is None else
+                     {k: v for k, v in traffic.items() if k != "traffic_bytes_per_launch"}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

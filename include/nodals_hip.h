@@ -123,6 +123,13 @@ int nodals_h8_fine_spmv(nodals_h8* h, const double* x, double* y);
  * h8_main_base.cu:557-564) on the current fine operator. */
 int nodals_h8_precond(nodals_h8* h, const double* r, double* z);
 int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double* ms_avg, double* bytes);
+/* Launch probe of the fine Schur SpMV inside the solver's own calls (the
+ * Krylov operator of h8_main_base.cu:567-726): enable=1 starts bracketing
+ * every such launch with HIP events on the context stream; enable=0 stops
+ * and returns the launch count and the summed launch durations (ms).  No
+ * reference counterpart (measurement hook, like the reference's
+ * G5StageEvents, h8_main_base.cu:2048-2105). */
+int nodals_h8_probe(nodals_h8* h, int32_t enable, int64_t* launches, double* total_ms);
 
 /* ---------------------------------------------------------------- SYS ---
  * Explicit SPD system (FP64 CSR, int64 row pointers) with an SA hierarchy

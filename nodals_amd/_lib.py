@@ -48,7 +48,7 @@ EXPORTS = [
     "nodals_h8_momentum_audit", "nodals_h8_continuity", "nodals_h8_pressure_refresh", "nodals_h8_pressure_solve",
     "nodals_h8_pressure_update", "nodals_h8_run", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
     "nodals_h8_precond",
-    "nodals_h8_time_kernel",
+    "nodals_h8_time_kernel", "nodals_h8_probe",
     "nodals_sys_create", "nodals_sys_create_host", "nodals_sys_get", "nodals_sys_destroy", "nodals_sys_info",
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
     L.nodals_h8_fine_spmv.argtypes = [vp, vp, vp]
     L.nodals_h8_precond.argtypes = [vp, vp, vp]
     L.nodals_h8_time_kernel.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.nodals_h8_probe.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -195,6 +196,15 @@ class H8:
         ms, by = C.c_double(), C.c_double()
         _check(lib().nodals_h8_time_kernel(self._h, name.encode(), reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def probe_start(self):
+        _check(lib().nodals_h8_probe(self._h, 1, None, None))
+
+    def probe_stop(self):
+        """(launches, summed ms) of the fine-SpMV launches since probe_start."""
+        n, ms = C.c_int64(), C.c_double()
+        _check(lib().nodals_h8_probe(self._h, 0, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     # host access --------------------------------------------------------
     def get(self, name: str, count: int, dtype=None) -> np.ndarray:

@@ -307,6 +307,20 @@ int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double
   });
 }
 
+int nodals_h8_probe(nodals_h8* h, int32_t enable, int64_t* launches, double* total_ms) {
+  return guarded([&]() {
+    if (enable) {
+      solver(h).probeStart();
+      return;
+    }
+    long long n = 0;
+    double ms = 0;
+    solver(h).probeStop(&n, &ms);
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+  });
+}
+
 int nodals_sys_create(int32_t n, const int64_t* row, const int32_t* col, const double* val, const char* options,
                       nodals_sys** out) {
   return guarded([&]() {

@@ -199,6 +199,7 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
 }
 
 H8Solver::~H8Solver() {
+  for (hipEvent_t e : probeEv_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -588,7 +589,46 @@ void H8Solver::refresh() {
 }
 
 void H8Solver::fineApply(hipStream_t s, const double* x, double* y) {
+  const bool probe = probeOn_ && s == stream_;
+  if (probe) NODALS_HIP(hipEventRecord(probeEv_[2 * (std::size_t)probeUsed_], s));
   spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y);
+  if (probe) {
+    NODALS_HIP(hipEventRecord(probeEv_[2 * (std::size_t)probeUsed_ + 1], s));
+    if (++probeUsed_ == kProbePairs) probeHarvest();
+  }
+}
+
+void H8Solver::probeStart() {
+  requireDevice();
+  if (probeEv_.empty()) {
+    probeEv_.resize(2 * (std::size_t)kProbePairs);
+    for (hipEvent_t& e : probeEv_) NODALS_HIP(hipEventCreate(&e));
+  }
+  probeUsed_ = 0;
+  probeLaunches_ = 0;
+  probeMs_ = 0;
+  probeOn_ = true;
+}
+
+void H8Solver::probeHarvest() {
+  if (probeUsed_ == 0) return;
+  NODALS_HIP(hipEventSynchronize(probeEv_[2 * (std::size_t)probeUsed_ - 1]));
+  for (int k = 0; k < probeUsed_; ++k) {
+    float ms = 0;
+    NODALS_HIP(hipEventElapsedTime(&ms, probeEv_[2 * (std::size_t)k], probeEv_[2 * (std::size_t)k + 1]));
+    probeMs_ += ms;
+  }
+  probeLaunches_ += probeUsed_;
+  probeUsed_ = 0;
+}
+
+void H8Solver::probeStop(long long* launches, double* totalMs) {
+  requireDevice();
+  require(probeOn_, "probe_stop without probe_start");
+  probeHarvest();
+  probeOn_ = false;
+  if (launches) *launches = probeLaunches_;
+  if (totalMs) *totalMs = probeMs_;
 }
 
 void H8Solver::fineSpmv(const double* x, double* y) {

@@ -119,6 +119,11 @@ class H8Solver {
   // Mean duration (ms) of `reps` launches of a hot kernel measured with HIP
   // events on the context stream; *bytes = algorithmic bytes per launch.
   double timeKernel(const std::string& kernel, int reps, double* bytes);
+  // Launch probe: while on, every fine-SpMV launch the solver makes (the
+  // Krylov operator) is bracketed by HIP events on the context stream;
+  // probeStop returns the launch count and the summed launch durations.
+  void probeStart();
+  void probeStop(long long* launches, double* totalMs);
   double* devScratchA() { return tmpA_.data(); }
   double* devScratchB() { return tmpB_.data(); }
   void sync();
@@ -138,6 +143,7 @@ class H8Solver {
   void precondApply(hipStream_t s, const double* r, double* z);
   void fineApply(hipStream_t s, const double* x, double* y);
   double pressurePrecondPower(int its, double safety);
+  void probeHarvest();
 
   H8Options opt_;
   TetMesh M_;
@@ -172,6 +178,12 @@ class H8Solver {
   std::unique_ptr<dev::DevAmg<bf16, float>> amgb_;
   double setupMs_ = 0;
   double chebLambda_ = 1.0;
+  static constexpr int kProbePairs = 512;
+  std::vector<hipEvent_t> probeEv_;
+  int probeUsed_ = 0;
+  bool probeOn_ = false;
+  long long probeLaunches_ = 0;
+  double probeMs_ = 0;
 };
 
 }  // namespace nodals
