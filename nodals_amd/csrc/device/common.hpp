@@ -1,6 +1,7 @@
 // HIP runtime helpers: error checks, owning device buffers, launch shapes.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

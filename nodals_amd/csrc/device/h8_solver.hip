@@ -589,13 +589,13 @@ void H8Solver::refresh() {
 }
 
 void H8Solver::fineApply(hipStream_t s, const double* x, double* y) {
-  const bool probe = probeOn_ && s == stream_;
-  if (probe) NODALS_HIP(hipEventRecord(probeEv_[2 * (std::size_t)probeUsed_], s));
-  spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y);
-  if (probe) {
-    NODALS_HIP(hipEventRecord(probeEv_[2 * (std::size_t)probeUsed_ + 1], s));
-    if (++probeUsed_ == kProbePairs) probeHarvest();
+  if (!(probeOn_ && s == stream_)) {
+    spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y);
+    return;
   }
+  spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y,
+                                     probeEv_[2 * (std::size_t)probeUsed_], probeEv_[2 * (std::size_t)probeUsed_ + 1]);
+  if (++probeUsed_ == kProbePairs) probeHarvest();
 }
 
 void H8Solver::probeStart() {

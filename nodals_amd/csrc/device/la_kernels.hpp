@@ -237,16 +237,26 @@ inline int lanesFor(double meanRow) {
 
 template <class TM, class TV, bool RESID, bool ACC>
 void spmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TV* x, const TV* b,
-          TV* y) {
+          TV* y, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
   if (n <= 0) return;
   const int grid = blocksFor((long long)n * lanes);
+  // With events, the launch goes through hipExtLaunchKernelGGL, which stamps
+  // them from the kernel's own dispatch packet (its start and end), not from
+  // separate marker packets.
+#define NODALS_SPMV_LAUNCH(LN)                                                                             \
+  if (start)                                                                                               \
+    hipExtLaunchKernelGGL((k_spmv<LN, TM, TV, RESID, ACC>), dim3(grid), dim3(256), 0, s, start, stop, 0, n, \
+                          row, col, val, x, b, y);                                                         \
+  else                                                                                                     \
+    hipLaunchKernelGGL((k_spmv<LN, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y);
   switch (lanes) {
-    case 4: hipLaunchKernelGGL((k_spmv<4, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
-    case 8: hipLaunchKernelGGL((k_spmv<8, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
-    case 16: hipLaunchKernelGGL((k_spmv<16, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
-    case 32: hipLaunchKernelGGL((k_spmv<32, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
-    default: hipLaunchKernelGGL((k_spmv<64, TM, TV, RESID, ACC>), grid, 256, 0, s, n, row, col, val, x, b, y); break;
+    case 4: NODALS_SPMV_LAUNCH(4) break;
+    case 8: NODALS_SPMV_LAUNCH(8) break;
+    case 16: NODALS_SPMV_LAUNCH(16) break;
+    case 32: NODALS_SPMV_LAUNCH(32) break;
+    default: NODALS_SPMV_LAUNCH(64) break;
   }
+#undef NODALS_SPMV_LAUNCH
   NODALS_LAUNCHED();
 }
 
