@@ -63,52 +63,71 @@ static __global__ void __launch_bounds__(256) k_conv_local(int nc, const int* __
   }
 }
 
+// Momentum rows come in two length classes: vertex rows (about 75-87
+// nonzeros) and face-bubble rows (about 12).  One launch covers both: blocks
+// [0, nbLong) give each long row LL lanes, the remaining blocks give each
+// short row LS lanes, so neither class runs a long serial loop nor leaves
+// most lanes idle.  The branch is uniform per block.
+#define NODALS_TWO_CLASS(LL, LS, nLong, nShort, nbLong, BODY)            \
+  if ((int)blockIdx.x < (nbLong)) {                                       \
+    const int gid_ = blockIdx.x * 256 + threadIdx.x;                      \
+    const int q_ = gid_ / (LL);                                           \
+    if (q_ < (nLong)) BODY(LL, q_, gid_ % (LL), true);                   \
+  } else {                                                                \
+    const int gid_ = ((int)blockIdx.x - (nbLong)) * 256 + threadIdx.x;    \
+    const int q_ = gid_ / (LS);                                           \
+    if (q_ < (nShort)) BODY(LS, q_, gid_ % (LS), false);                 \
+  }
+
 // Per momentum row: gather the assembled matrix (persistent diffusion +
 // convection terms), row-L1 under-relaxation delta = (1/alpha-1) sum|a_ik|
 // added to the diagonal, rAU = 1/a_ii (g4_finalize_relax_kernel,
 // g5e_fp32_kernels.cuh:78-85), the Dirichlet convection RHS, B^T p by the
 // velocity incidence, and the momentum RHS b = s + c + B^T p + delta u
 // (g4_momentum_rhs_kernel, g5e_fp32_kernels.cuh:97-104).
+struct MomRowsArgs {
+  const int *mrow, *mdiagPos;
+  const double* mdiff;
+  const int *termOff, *termIdx;
+  const double* cbuf;
+  const int *fixOff, *fixTerm;
+  const double* fixedVal;
+  const int* incOff;
+  const unsigned* incPacked;
+  const double *bcoef, *p, *srhs, *u;
+  double fac;
+  double *aval, *delta, *diag, *rau, *brhs;
+};
+
 template <int L>
-__global__ void __launch_bounds__(256) k_mom_rows(
-    int nv, const int* __restrict__ mrow, const int* __restrict__ mdiagPos, const double* __restrict__ mdiff,
-    const int* __restrict__ termOff, const int* __restrict__ termIdx, const double* __restrict__ cbuf,
-    const int* __restrict__ fixOff, const int* __restrict__ fixTerm, const double* __restrict__ fixedVal,
-    const int* __restrict__ incOff, const unsigned* __restrict__ incPacked, const double* __restrict__ bcoef,
-    const double* __restrict__ p, const double* __restrict__ srhs, const double* __restrict__ u, double fac,
-    double* __restrict__ aval, double* __restrict__ delta, double* __restrict__ diag, double* __restrict__ rau,
-    double* __restrict__ brhs) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int i = gid / L;
-  const int lane = gid % L;
-  if (i >= nv) return;
-  const int rb = mrow[i], re = mrow[i + 1], dp = mdiagPos[i];
+__device__ __forceinline__ void mom_row(const MomRowsArgs& A, int i, int lane) {
+  const int rb = A.mrow[i], re = A.mrow[i + 1], dp = A.mdiagPos[i];
   double sabs = 0.0, vdiag = 0.0;
   for (int k = rb + lane; k < re; k += L) {
-    double v = mdiff[k];
-    for (int t = termOff[k]; t < termOff[k + 1]; ++t) v += cbuf[termIdx[t]];
-    aval[k] = v;
+    double v = A.mdiff[k];
+    for (int t = A.termOff[k]; t < A.termOff[k + 1]; ++t) v += A.cbuf[A.termIdx[t]];
+    A.aval[k] = v;
     sabs += fabs(v);
     if (k == dp) vdiag = v;
   }
   sabs = groupSum(sabs, L);
   vdiag = groupSum(vdiag, L);
-  const double de = fac * sabs;
+  const double de = A.fac * sabs;
   const double dg = vdiag + de;
   double c0 = 0, c1 = 0, c2 = 0;
-  for (int t = fixOff[i] + lane; t < fixOff[i + 1]; t += L) {
-    const int q = fixTerm[2 * t], slot = fixTerm[2 * t + 1];
-    const double C = cbuf[q];
-    c0 -= C * fixedVal[3ll * slot + 0];
-    c1 -= C * fixedVal[3ll * slot + 1];
-    c2 -= C * fixedVal[3ll * slot + 2];
+  for (int t = A.fixOff[i] + lane; t < A.fixOff[i + 1]; t += L) {
+    const int q = A.fixTerm[2 * t], slot = A.fixTerm[2 * t + 1];
+    const double C = A.cbuf[q];
+    c0 -= C * A.fixedVal[3ll * slot + 0];
+    c1 -= C * A.fixedVal[3ll * slot + 1];
+    c2 -= C * A.fixedVal[3ll * slot + 2];
   }
   double t0 = 0, t1 = 0, t2 = 0;
-  for (int k = incOff[i] + lane; k < incOff[i + 1]; k += L) {
-    const unsigned pk = incPacked[k];
+  for (int k = A.incOff[i] + lane; k < A.incOff[i + 1]; k += L) {
+    const unsigned pk = A.incPacked[k];
     const int c = (int)(pk >> 3), a = (int)(pk & 7u);
-    const double pv = p[c];
-    const double* B = bcoef + (long long)c * 24 + 3 * a;
+    const double pv = A.p[c];
+    const double* B = A.bcoef + (long long)c * 24 + 3 * a;
     t0 += B[0] * pv;
     t1 += B[1] * pv;
     t2 += B[2] * pv;
@@ -119,28 +138,33 @@ __global__ void __launch_bounds__(256) k_mom_rows(
   t0 = groupSum(t0, L);
   t1 = groupSum(t1, L);
   t2 = groupSum(t2, L);
-  if (dp >= rb + lane && ((dp - rb - lane) % L) == 0) aval[dp] = dg;
+  if (dp >= rb + lane && ((dp - rb - lane) % L) == 0) A.aval[dp] = dg;
   if (lane == 0) {
-    delta[i] = de;
-    diag[i] = dg;
-    rau[i] = 1.0 / dg;
-    brhs[3ll * i + 0] = srhs[3ll * i + 0] + c0 + t0 + de * u[3ll * i + 0];
-    brhs[3ll * i + 1] = srhs[3ll * i + 1] + c1 + t1 + de * u[3ll * i + 1];
-    brhs[3ll * i + 2] = srhs[3ll * i + 2] + c2 + t2 + de * u[3ll * i + 2];
+    A.delta[i] = de;
+    A.diag[i] = dg;
+    A.rau[i] = 1.0 / dg;
+    A.brhs[3ll * i + 0] = A.srhs[3ll * i + 0] + c0 + t0 + de * A.u[3ll * i + 0];
+    A.brhs[3ll * i + 1] = A.srhs[3ll * i + 1] + c1 + t1 + de * A.u[3ll * i + 1];
+    A.brhs[3ll * i + 2] = A.srhs[3ll * i + 2] + c2 + t2 + de * A.u[3ll * i + 2];
   }
 }
 
+// Rows by class lists (longRows[nLong], shortRows[nShort]).
+template <int LL, int LS>
+__global__ void __launch_bounds__(256) k_mom_rows(MomRowsArgs A, int nLong, const int* __restrict__ longRows,
+                                                  int nShort, const int* __restrict__ shortRows, int nbLong) {
+#define NODALS_MOM_ROW_BODY(LN, q, lane, isLong) mom_row<LN>(A, (isLong) ? longRows[q] : shortRows[q], lane)
+  NODALS_TWO_CLASS(LL, LS, nLong, nShort, nbLong, NODALS_MOM_ROW_BODY)
+#undef NODALS_MOM_ROW_BODY
+}
+
 // One colour of the three-component forward Gauss-Seidel sweep sharing A
-// (g4_mcgs_color_kernel, g5e_fp32_kernels.cuh:106-116); rows [r0, r1).
+// (g4_mcgs_color_kernel, g5e_fp32_kernels.cuh:106-116).  The colour's rows
+// are [r0, r1), its long rows first: [r0, rs) long, [rs, r1) short.
 template <int L>
-__global__ void __launch_bounds__(256) k_mom_gs(int r0, int r1, const int* __restrict__ mrow,
-                                                const int* __restrict__ mcol, const double* __restrict__ aval,
-                                                const double* __restrict__ diag, const double* __restrict__ brhs,
-                                                double* __restrict__ u, double omega) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int i = r0 + gid / L;
-  const int lane = gid % L;
-  if (i >= r1) return;
+__device__ __forceinline__ void mom_gs_row(int i, int lane, const int* __restrict__ mrow, const int* __restrict__ mcol,
+                                           const double* __restrict__ aval, const double* __restrict__ diag,
+                                           const double* __restrict__ brhs, double* __restrict__ u, double omega) {
   double s0 = 0, s1 = 0, s2 = 0;
   for (int k = mrow[i] + lane; k < mrow[i + 1]; k += L) {
     const int j = mcol[k];
@@ -162,38 +186,59 @@ __global__ void __launch_bounds__(256) k_mom_gs(int r0, int r1, const int* __res
   }
 }
 
+template <int LL, int LS>
+__global__ void __launch_bounds__(256) k_mom_gs(int r0, int rs, int r1, int nbLong, const int* __restrict__ mrow,
+                                                const int* __restrict__ mcol, const double* __restrict__ aval,
+                                                const double* __restrict__ diag, const double* __restrict__ brhs,
+                                                double* __restrict__ u, double omega) {
+#define NODALS_GS_BODY(LN, q, lane, isLong) \
+  mom_gs_row<LN>((isLong) ? r0 + (q) : rs + (q), lane, mrow, mcol, aval, diag, brhs, u, omega)
+  NODALS_TWO_CLASS(LL, LS, rs - r0, r1 - rs, nbLong, NODALS_GS_BODY)
+#undef NODALS_GS_BODY
+}
+
 // Momentum residual audit partial sums: |b_d|^2 and |b_d - (A u)_d|^2
-// (g4_mom_norms_kernel, g5e_fp32_kernels.cuh:117-127).  part[6*grid].
+// (g4_mom_norms_kernel, g5e_fp32_kernels.cuh:117-127).  One row per lane
+// group, rows by class lists; part[6][gridDim.x].
 template <int L>
-__global__ void __launch_bounds__(256) k_mom_norms(int nv, const int* __restrict__ mrow, const int* __restrict__ mcol,
+__device__ __forceinline__ void mom_norm_row(int i, int lane, const int* __restrict__ mrow,
+                                             const int* __restrict__ mcol, const double* __restrict__ aval,
+                                             const double* __restrict__ brhs, const double* __restrict__ u,
+                                             double* acc) {
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int k = mrow[i] + lane; k < mrow[i + 1]; k += L) {
+    const int j = mcol[k];
+    const double a = aval[k];
+    s0 += a * u[3ll * j + 0];
+    s1 += a * u[3ll * j + 1];
+    s2 += a * u[3ll * j + 2];
+  }
+  s0 = groupSum(s0, L);
+  s1 = groupSum(s1, L);
+  s2 = groupSum(s2, L);
+  if (lane == 0) {
+    const double b0 = brhs[3ll * i], b1 = brhs[3ll * i + 1], b2 = brhs[3ll * i + 2];
+    acc[0] += b0 * b0;
+    acc[1] += b1 * b1;
+    acc[2] += b2 * b2;
+    acc[3] += (b0 - s0) * (b0 - s0);
+    acc[4] += (b1 - s1) * (b1 - s1);
+    acc[5] += (b2 - s2) * (b2 - s2);
+  }
+}
+
+template <int LL, int LS>
+__global__ void __launch_bounds__(256) k_mom_norms(int nLong, const int* __restrict__ longRows, int nShort,
+                                                   const int* __restrict__ shortRows, int nbLong,
+                                                   const int* __restrict__ mrow, const int* __restrict__ mcol,
                                                    const double* __restrict__ aval, const double* __restrict__ brhs,
                                                    const double* __restrict__ u, double* __restrict__ part) {
   __shared__ double sh[4];
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  const int groups = gridDim.x * (256 / L);
-  const int lane = threadIdx.x % L;
-  for (int i = (blockIdx.x * 256 + threadIdx.x) / L; i < nv; i += groups) {
-    double s0 = 0, s1 = 0, s2 = 0;
-    for (int k = mrow[i] + lane; k < mrow[i + 1]; k += L) {
-      const int j = mcol[k];
-      const double a = aval[k];
-      s0 += a * u[3ll * j + 0];
-      s1 += a * u[3ll * j + 1];
-      s2 += a * u[3ll * j + 2];
-    }
-    s0 = groupSum(s0, L);
-    s1 = groupSum(s1, L);
-    s2 = groupSum(s2, L);
-    if (lane == 0) {
-      const double b0 = brhs[3ll * i], b1 = brhs[3ll * i + 1], b2 = brhs[3ll * i + 2];
-      acc[0] += b0 * b0;
-      acc[1] += b1 * b1;
-      acc[2] += b2 * b2;
-      acc[3] += (b0 - s0) * (b0 - s0);
-      acc[4] += (b1 - s1) * (b1 - s1);
-      acc[5] += (b2 - s2) * (b2 - s2);
-    }
-  }
+#define NODALS_NORM_BODY(LN, q, lane, isLong) \
+  mom_norm_row<LN>((isLong) ? longRows[q] : shortRows[q], lane, mrow, mcol, aval, brhs, u, acc)
+  NODALS_TWO_CLASS(LL, LS, nLong, nShort, nbLong, NODALS_NORM_BODY)
+#undef NODALS_NORM_BODY
   for (int q = 0; q < 6; ++q) {
     const double t = blockSum256(acc[q], sh);
     if (threadIdx.x == 0) part[(long long)q * gridDim.x + blockIdx.x] = t;

@@ -217,8 +217,37 @@ void H8Solver::upload() {
   fineNnz_ = (long long)F_.col.size();
   require(momNnz_ < (1ll << 31) && fineNnz_ < (1ll << 31), "matrix too large for int32 device indices");
   require((long long)nc_ * 64 < (1ll << 31), "too many cells for the convection buffer index");
-  // colour-ordered numbering
-  perm_ = S_.coloring.rows;
+  // Colour-ordered numbering.  Inside each colour the long (vertex) rows come
+  // first and the short (face-bubble) rows after them, each class in
+  // reference order; rows of one colour never couple, so the order inside a
+  // colour does not change any Gauss-Seidel result.
+  perm_.assign((std::size_t)nv_, 0);
+  colorSplit_.assign(S_.coloring.offset.size() > 0 ? S_.coloring.offset.size() - 1 : 0, 0);
+  {
+    const std::vector<int>& rows = S_.coloring.rows;
+    const std::vector<int>& off = S_.coloring.offset;
+    std::vector<int> longList, shortList;
+    for (std::size_t c = 0; c + 1 < off.size(); ++c) {
+      int w = off[c];
+      for (int k = off[c]; k < off[c + 1]; ++k) {
+        const int r = rows[(std::size_t)k];
+        if (A.row[(std::size_t)r + 1] - A.row[(std::size_t)r] > kLongRow) perm_[(std::size_t)w++] = r;
+      }
+      colorSplit_[c] = w;
+      for (int k = off[c]; k < off[c + 1]; ++k) {
+        const int r = rows[(std::size_t)k];
+        if (A.row[(std::size_t)r + 1] - A.row[(std::size_t)r] <= kLongRow) perm_[(std::size_t)w++] = r;
+      }
+      for (int k = off[c]; k < colorSplit_[c]; ++k) longList.push_back(k);
+      for (int k = colorSplit_[c]; k < off[c + 1]; ++k) shortList.push_back(k);
+    }
+    nLong_ = (int)longList.size();
+    nShort_ = (int)shortList.size();
+    if (longList.empty()) longList.push_back(0);
+    if (shortList.empty()) shortList.push_back(0);
+    longRows_.upload(longList);
+    shortRows_.upload(shortList);
+  }
   pinv_.assign((std::size_t)nv_, -1);
   for (int i = 0; i < nv_; ++i) pinv_[(std::size_t)perm_[(std::size_t)i]] = i;
   colorOff_ = S_.coloring.offset;
@@ -272,7 +301,6 @@ void H8Solver::upload() {
   mdiagPos_.upload(mdp);
   mdiff_.upload(mdiff);
   aval_.alloc((std::size_t)momNnz_);
-  momLanes_ = lanesFor((double)momNnz_ / std::max(nv_, 1));
 
   // assembly gather terms (cell order) and Dirichlet convection terms
   std::vector<int> tcount((std::size_t)momNnz_ + 1, 0), fcount((std::size_t)nv_ + 1, 0);
@@ -445,21 +473,31 @@ void H8Solver::assemble(double alphaU, double rauScale) {
   hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, stream_, nc_, cref_.data(), cinvJ_.data(),
                      cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), cbuf_.data());
   NODALS_LAUNCHED();
-  const double fac = 1.0 / alphaU - 1.0;
-  const int grid = blocksFor((long long)nv_ * momLanes_);
-#define NODALS_MOM_ROWS(L)                                                                                           \
-  hipLaunchKernelGGL(k_mom_rows<L>, grid, 256, 0, stream_, nv_, mrow_.data(), mdiagPos_.data(), mdiff_.data(),     \
-                     termOff_.data(), termIdx_.data(), cbuf_.data(), fixOff_.data(), fixTerm_.data(),               \
-                     fixedVal_.data(), incOff_.data(), incPacked_.data(), bcoef_.data(), p_.data(), srhs_.data(),   \
-                     u_.data(), fac, aval_.data(), delta_.data(), diag_.data(), rau_.data(), brhs_.data())
-  switch (momLanes_) {
-    case 4: NODALS_MOM_ROWS(4); break;
-    case 8: NODALS_MOM_ROWS(8); break;
-    case 16: NODALS_MOM_ROWS(16); break;
-    case 32: NODALS_MOM_ROWS(32); break;
-    default: NODALS_MOM_ROWS(64); break;
-  }
-#undef NODALS_MOM_ROWS
+  MomRowsArgs a;
+  a.mrow = mrow_.data();
+  a.mdiagPos = mdiagPos_.data();
+  a.mdiff = mdiff_.data();
+  a.termOff = termOff_.data();
+  a.termIdx = termIdx_.data();
+  a.cbuf = cbuf_.data();
+  a.fixOff = fixOff_.data();
+  a.fixTerm = fixTerm_.data();
+  a.fixedVal = fixedVal_.data();
+  a.incOff = incOff_.data();
+  a.incPacked = incPacked_.data();
+  a.bcoef = bcoef_.data();
+  a.p = p_.data();
+  a.srhs = srhs_.data();
+  a.u = u_.data();
+  a.fac = 1.0 / alphaU - 1.0;
+  a.aval = aval_.data();
+  a.delta = delta_.data();
+  a.diag = diag_.data();
+  a.rau = rau_.data();
+  a.brhs = brhs_.data();
+  const int nbL = blocksFor((long long)nLong_ * kLongLanes), nbS = blocksFor((long long)nShort_ * kShortLanes);
+  hipLaunchKernelGGL((k_mom_rows<kLongLanes, kShortLanes>), nbL + nbS, 256, 0, stream_, a, nLong_, longRows_.data(),
+                     nShort_, shortRows_.data(), nbL);
   NODALS_LAUNCHED();
   if (rauScale != 1.0) {
     hipLaunchKernelGGL(k_scale_into<double>, blocksFor(nv_), 256, 0, stream_, (long long)nv_, rauScale, rau_.data(),
@@ -476,18 +514,10 @@ const char* H8Solver::momentumWork(const std::string& mode, double omega, int ou
       const int c = reverse ? nco - 1 - q : q;
       const int r0 = colorOff_[(std::size_t)c], r1 = colorOff_[(std::size_t)c + 1];
       if (r1 <= r0) continue;
-      const int grid = blocksFor((long long)(r1 - r0) * momLanes_);
-#define NODALS_GS(L)                                                                                                   \
-  hipLaunchKernelGGL(k_mom_gs<L>, grid, 256, 0, stream_, r0, r1, mrow_.data(), mcol_.data(), aval_.data(), diag_.data(), \
-                     brhs_.data(), u_.data(), omega)
-      switch (momLanes_) {
-        case 4: NODALS_GS(4); break;
-        case 8: NODALS_GS(8); break;
-        case 16: NODALS_GS(16); break;
-        case 32: NODALS_GS(32); break;
-        default: NODALS_GS(64); break;
-      }
-#undef NODALS_GS
+      const int rs = colorSplit_[(std::size_t)c];
+      const int nbL = blocksFor((long long)(rs - r0) * kLongLanes), nbS = blocksFor((long long)(r1 - rs) * kShortLanes);
+      hipLaunchKernelGGL((k_mom_gs<kLongLanes, kShortLanes>), nbL + nbS, 256, 0, stream_, r0, rs, r1, nbL,
+                         mrow_.data(), mcol_.data(), aval_.data(), diag_.data(), brhs_.data(), u_.data(), omega);
       NODALS_LAUNCHED();
     }
   };
@@ -517,21 +547,14 @@ const char* H8Solver::momentumWork(const std::string& mode, double omega, int ou
 
 std::array<double, 3> H8Solver::momentumAudit() {
   requireDevice();
-  double* part = red_.partials();
-  const int grid = kReduceParts;
-#define NODALS_NORMS(L)                                                                                              \
-  hipLaunchKernelGGL(k_mom_norms<L>, grid, 256, 0, stream_, nv_, mrow_.data(), mcol_.data(), aval_.data(),          \
-                     brhs_.data(), u_.data(), part)
-  switch (momLanes_) {
-    case 4: NODALS_NORMS(4); break;
-    case 8: NODALS_NORMS(8); break;
-    case 16: NODALS_NORMS(16); break;
-    case 32: NODALS_NORMS(32); break;
-    default: NODALS_NORMS(64); break;
-  }
-#undef NODALS_NORMS
+  const int nbL = blocksFor((long long)nLong_ * kLongLanes), nbS = blocksFor((long long)nShort_ * kShortLanes);
+  const int grid = nbL + nbS;
+  if (normPart_.size() < (std::size_t)grid * 6) normPart_.alloc((std::size_t)grid * 6);
+  hipLaunchKernelGGL((k_mom_norms<kLongLanes, kShortLanes>), grid, 256, 0, stream_, nLong_, longRows_.data(), nShort_,
+                     shortRows_.data(), nbL, mrow_.data(), mcol_.data(), aval_.data(), brhs_.data(), u_.data(),
+                     normPart_.data());
   NODALS_LAUNCHED();
-  hipLaunchKernelGGL(k_reduce_final, 6, 256, 0, stream_, grid, part, red_.hostOut());
+  hipLaunchKernelGGL(k_reduce_final, 6, 256, 0, stream_, grid, normPart_.data(), red_.hostOut());
   NODALS_LAUNCHED();
   sync();
   const double* h = red_.hostOut();

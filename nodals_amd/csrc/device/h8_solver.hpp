@@ -155,9 +155,15 @@ class H8Solver {
   dev::KrylovWork kw_;
 
   std::vector<int> perm_, pinv_;  // device numbering: perm_[new] = reference row
-  int nc_ = 0, nv_ = 0, nFix_ = 0, fineLanes_ = 16, momLanes_ = 16;
+  int nc_ = 0, nv_ = 0, nFix_ = 0, fineLanes_ = 16;
   long long momNnz_ = 0, fineNnz_ = 0;
-  std::vector<int> colorOff_;
+  std::vector<int> colorOff_, colorSplit_;  // colour c: [off, split) long rows, [split, off+1) short rows
+  // Momentum rows with more than kLongRow nonzeros (vertex rows) get
+  // kLongLanes lanes each, the others (face-bubble rows) kShortLanes.
+  static constexpr int kLongRow = 24, kLongLanes = 32, kShortLanes = 16;
+  int nLong_ = 0, nShort_ = 0;
+  DBuf<int> longRows_, shortRows_;
+  DBuf<double> normPart_;
 
   DBuf<int> cref_;
   DBuf<double> cinvJ_, cdet_, bcoef_, fixedVal_, centT_;

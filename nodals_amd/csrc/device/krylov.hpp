@@ -29,9 +29,29 @@ struct KrylovResult {
 // y = A x and z = M r on length-n FP64 device vectors.
 using DevOp = std::function<void(hipStream_t, const double*, double*)>;
 
+// Device-side FGMRES state (doubles): Hessenberg H[i*32+j] (33 x 32, rotated
+// in place), Givens cos/sin, the rotated right-hand side g, the two
+// Gram-Schmidt coefficient vectors, the next basis scaling and beta^2.
+namespace fg {
+constexpr int kH = 0, kCs = 1056, kSn = 1088, kG = 1120, kH1 = 1160, kH2 = 1200, kScal = 1240, kBeta2 = 1241,
+              kSize = 1248;
+// Host-visible (coherent pinned) mirror: residual estimate |g_{j+1}| after
+// step j, lucky-breakdown and failure flags, r0, the rotated H and g.
+constexpr int hEst = 0, hZero = 32, hBad = 64, hR0 = 96, hH = 100, hG = 100 + 1056, hSize = 100 + 1056 + 40;
+}  // namespace fg
+
 struct KrylovWork {
   int n = 0, restart = 0;
-  DBuf<double> x, r, z, p, q, V, Z;
+  DBuf<double> x, r, z, p, q, V, Z, fgDev;
+  double* fgHost = nullptr;
+  std::vector<hipEvent_t> stepEv;
+  KrylovWork() = default;
+  KrylovWork(const KrylovWork&) = delete;
+  KrylovWork& operator=(const KrylovWork&) = delete;
+  ~KrylovWork() {
+    for (hipEvent_t e : stepEv) (void)hipEventDestroy(e);
+    if (fgHost) (void)hipHostFree(fgHost);
+  }
   void ensure(int n_, int restart_) {
     if (n_ != n) {
       n = n_;
@@ -43,8 +63,142 @@ struct KrylovWork {
       V.alloc((std::size_t)n * (restart + 1));
       Z.alloc((std::size_t)n * restart);
     }
+    if (restart_ > 0 && !fgHost) {
+      fgDev.alloc(fg::kSize);
+      NODALS_HIP(hipHostMalloc((void**)&fgHost, sizeof(double) * fg::hSize, hipHostMallocCoherent));
+      stepEv.resize(33);
+      for (hipEvent_t& e : stepEv) NODALS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
   }
 };
+
+// FGMRES cycle start: r0 = sqrt(beta^2), g = (r0, 0, ...), the scaling of the
+// first basis vector (0 when r0 = 0 so nothing non-finite is produced).
+static __global__ void k_fgmres_start(double* __restrict__ st, double* __restrict__ host, int first) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double b2 = st[fg::kBeta2];
+  const double beta = sqrt(b2 > 0.0 ? b2 : 0.0);
+  for (int i = 0; i < 33; ++i) st[fg::kG + i] = 0.0;
+  st[fg::kG] = beta;
+  st[fg::kScal] = beta > 0.0 ? 1.0 / beta : 0.0;
+  host[fg::hG] = beta;
+  if (first) host[fg::hR0] = beta;
+}
+
+// First Gram-Schmidt update fused with the second pass's dot products:
+//   w -= V h1;  part[q][block] = V_q . w (q <= j), part[j+1][block] = w . w.
+// Reads the basis once for both.
+static __global__ void __launch_bounds__(256) k_gs_update_dots(long long n, int k, const double* __restrict__ V,
+                                                                const double* __restrict__ h1, double* __restrict__ w,
+                                                                double* __restrict__ part) {
+  __shared__ double c[32];
+  __shared__ double sh[4][33];
+  if (threadIdx.x < 32) c[threadIdx.x] = (int)threadIdx.x < k ? h1[threadIdx.x] : 0.0;
+  __syncthreads();
+  double sv[33];
+#pragma unroll
+  for (int q = 0; q < 33; ++q) sv[q] = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    double vq[32];
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      vq[q] = q < k ? V[(long long)q * n + i] : 0.0;
+      acc += c[q] * vq[q];
+    }
+    const double wi = w[i] - acc;
+    w[i] = wi;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) sv[q] += vq[q] * wi;
+    sv[32] += wi * wi;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 33; ++q) {
+    if (q < k || q == 32) {
+      const double t = groupSum(sv[q], 64);
+      if (lane == 0) sh[wv][q] = t;
+    }
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < k) part[(long long)t * gridDim.x + blockIdx.x] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
+  if (t == k) part[(long long)k * gridDim.x + blockIdx.x] = sh[0][32] + sh[1][32] + sh[2][32] + sh[3][32];
+}
+
+// Second Gram-Schmidt update fused with the normalisation:
+//   w = (w - V h2) * scal.
+static __global__ void __launch_bounds__(256) k_gs_update_scale(long long n, int k, const double* __restrict__ V,
+                                                                 const double* __restrict__ h2,
+                                                                 const double* __restrict__ scal,
+                                                                 double* __restrict__ w) {
+  __shared__ double c[33];
+  if (threadIdx.x < 32) c[threadIdx.x] = (int)threadIdx.x < k ? h2[threadIdx.x] : 0.0;
+  if (threadIdx.x == 32) c[32] = scal[0];
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int q = 0; q < k; ++q) acc += c[q] * V[(long long)q * n + i];
+  w[i] = (w[i] - acc) * c[32];
+}
+
+// Arnoldi step j after the two Gram-Schmidt passes: h = h1 + h2,
+// h_{j+1,j} = sqrt(max(|w|^2 - |h2|^2, 0)), previous rotations applied to
+// column j, the new rotation, g updated; the residual estimate and flags go
+// to host memory, the scaling 1/h_{j+1,j} to device memory.
+// One 256-thread block: it first finishes the second pass's reduction
+// (h2[q] = sum of nparts block partials, q <= j+1, one wavefront per output
+// in turn, fixed order), then thread 0 does the Givens step.
+static __global__ void __launch_bounds__(256) k_arnoldi_givens(int j, int nparts, const double* __restrict__ part,
+                                                                double* __restrict__ st, double* __restrict__ host) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = wv; q <= j + 1; q += 4) {
+    double v = 0.0;
+    for (int i = lane; i < nparts; i += 64) v += part[(long long)q * nparts + i];
+    v = groupSum(v, 64);
+    if (lane == 0) st[fg::kH2 + q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double* H = st + fg::kH;
+  double* cs = st + fg::kCs;
+  double* sn = st + fg::kSn;
+  double* g = st + fg::kG;
+  const double* h1 = st + fg::kH1;
+  const double* h2 = st + fg::kH2;
+  double hh = 0.0;
+  for (int i = 0; i <= j; ++i) {
+    H[i * 32 + j] = h1[i] + h2[i];
+    hh += h2[i] * h2[i];
+  }
+  const double d2 = h2[j + 1] - hh;
+  const double hn = sqrt(d2 > 0.0 ? d2 : 0.0);
+  H[(j + 1) * 32 + j] = hn;
+  for (int i = 0; i < j; ++i) {
+    const double a = H[i * 32 + j], b = H[(i + 1) * 32 + j];
+    H[i * 32 + j] = cs[i] * a + sn[i] * b;
+    H[(i + 1) * 32 + j] = -sn[i] * a + cs[i] * b;
+  }
+  const double a = H[j * 32 + j], b = H[(j + 1) * 32 + j];
+  const double d = hypot(a, b);
+  const bool bad = !(d > 0.0) || !isfinite(d) || !isfinite(hn);
+  const double c = bad ? 0.0 : a / d, sgn = bad ? 0.0 : b / d;
+  cs[j] = c;
+  sn[j] = sgn;
+  H[j * 32 + j] = d;
+  H[(j + 1) * 32 + j] = 0.0;
+  const double gj = g[j];
+  g[j + 1] = -sgn * gj;
+  g[j] = c * gj;
+  st[fg::kScal] = hn > 0.0 ? 1.0 / hn : 0.0;
+  for (int i = 0; i <= j; ++i) host[fg::hH + i * 32 + j] = H[i * 32 + j];
+  host[fg::hG + j] = g[j];
+  host[fg::hG + j + 1] = g[j + 1];
+  host[fg::hZero + j] = hn == 0.0 ? 1.0 : 0.0;
+  host[fg::hBad + j] = bad ? 1.0 : 0.0;
+  host[fg::hEst + j] = fabs(g[j + 1]);
+}
 
 inline KrylovResult pcg(hipStream_t s, Reducer& R, int n, const DevOp& A, const DevOp& M, const double* rhs, double rtol,
                         double atol, int maxit, KrylovWork& W) {
@@ -132,103 +286,132 @@ inline KrylovResult richardsonCheb(hipStream_t s, Reducer& R, int n, const DevOp
   return out;
 }
 
+// FGMRES(m), device-resident Arnoldi.  One step enqueues M, A and four
+// kernels for classical Gram-Schmidt done twice (coefficients stay in device
+// memory), the Givens step and the basis scaling; the host never waits
+// inside a step.  It normally reads step j's residual estimate only after
+// step j+1 is queued, so the GPU has the next step in flight; when the last
+// contraction predicts that step j converges, it waits for step j first, so
+// that usually no step runs past convergence.  A step queued past
+// convergence is discarded: it writes only basis vector j+2, Z_{j+1}, column
+// j+1 of H and g[j+1..j+2], none of which the k = j+1 solution uses.
+// At the end of each cycle the host solves the k x k triangular system,
+// applies x += Z y and recomputes the true FP64 residual.
 inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, const DevOp& M, const double* rhs,
                            double rtol, double atol, int maxit, int restart, KrylovWork& W) {
   require(restart >= 1 && restart <= 31, "FGMRES restart must be in [1,31]");
   W.ensure(n, restart);
   const int m = restart;
   const long long N = n;
-  fill<double>(s, n, W.x.data(), 0.0);
-  copy<double>(s, n, rhs, W.r.data());
-  double beta = std::sqrt(R.dot(s, n, W.r.data(), W.r.data()));
-  require(std::isfinite(beta), "FGMRES initial norm non-finite", kErrNonFinite);
-  KrylovResult out;
-  out.r0 = beta;
-  if (beta <= atol) {
-    out.ok = true;
-    out.rel = 0.0;
-    return out;
-  }
-  const double r0 = beta;
-  const double target = std::max(atol, rtol * r0);
-  std::vector<double> H((std::size_t)(m + 1) * m), cs((std::size_t)m), sn((std::size_t)m), g((std::size_t)m + 1),
-      h((std::size_t)m + 1), h2((std::size_t)m + 1), y((std::size_t)m);
+  const int nb = blocksFor(N);
+  double* st = W.fgDev.data();
+  double* hs = W.fgHost;
   double* V = W.V.data();
   double* Z = W.Z.data();
-  int its = 0;
-  while (true) {
-    hipLaunchKernelGGL(k_scale_into<double>, blocksFor(N), 256, 0, s, N, 1.0 / beta, W.r.data(), V);
+  fill<double>(s, n, W.x.data(), 0.0);
+  // cycle start from r (first cycle r = rhs, x = 0)
+  auto start = [&](const double* r, bool first) {
+    R.dotDev(s, N, r, r, st + fg::kBeta2);
+    hipLaunchKernelGGL(k_fgmres_start, 1, 64, 0, s, st, hs, first ? 1 : 0);
     NODALS_LAUNCHED();
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
-    int k = 0;
-    bool estConverged = false;
-    for (int j = 0; j < m && its < maxit; ++j) {
-      double* vj = V + (long long)j * N;
-      double* zj = Z + (long long)j * N;
-      double* w = V + (long long)(j + 1) * N;
-      M(s, vj, zj);
-      A(s, zj, w);
-      // classical Gram-Schmidt, twice
-      R.multidot(s, N, j + 1, V, w, h.data());
-      Coef32 c{};
-      for (int i = 0; i <= j; ++i) c.c[i] = h[(std::size_t)i];
-      hipLaunchKernelGGL(k_multi_axpy, blocksFor(N), 256, 0, s, N, j + 1, V, c, w, -1.0);
-      NODALS_LAUNCHED();
-      // second pass fused with ||w||^2; after it ||w'||^2 = ||w||^2 - ||h2||^2
-      // (V orthonormal, small correction), so no separate norm reduction.
-      R.multidot(s, N, j + 1, V, w, h2.data(), true);
-      double hh = 0.0;
-      for (int i = 0; i <= j; ++i) {
-        c.c[i] = h2[(std::size_t)i];
-        h[(std::size_t)i] += h2[(std::size_t)i];
-        hh += h2[(std::size_t)i] * h2[(std::size_t)i];
+    hipLaunchKernelGGL(k_scale_dev, nb, 256, 0, s, N, st + fg::kScal, r, V);
+    NODALS_LAUNCHED();
+  };
+  auto step = [&](int j) {
+    double* vj = V + (long long)j * N;
+    double* zj = Z + (long long)j * N;
+    double* w = V + (long long)(j + 1) * N;
+    M(s, vj, zj);
+    A(s, zj, w);
+    // classical Gram-Schmidt twice; pass 2's dots share a kernel with pass 1's
+    // update, and pass 2's update shares one with the normalisation
+    R.multidotDev(s, N, j + 1, V, w, st + fg::kH1);
+    hipLaunchKernelGGL(k_gs_update_dots, kReduceParts, 256, 0, s, N, j + 1, V, st + fg::kH1, w, R.partials());
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_arnoldi_givens, 1, 256, 0, s, j, kReduceParts, R.partials(), st, hs);
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_gs_update_scale, nb, 256, 0, s, N, j + 1, V, st + fg::kH2, st + fg::kScal, w);
+    NODALS_LAUNCHED();
+    NODALS_HIP(hipEventRecord(W.stepEv[(std::size_t)j], s));
+  };
+
+  KrylovResult out;
+  start(rhs, true);
+  NODALS_HIP(hipEventRecord(W.stepEv[32], s));
+  double r0 = -1.0, target = 0.0;
+  int its = 0;
+  bool firstCycle = true;
+  std::vector<double> y((std::size_t)m);
+  double prevEst = 0.0;  // residual at the cycle start (r0, then the true residual at each restart)
+  while (true) {
+    const int limit = std::min(m, maxit - its);
+    int queued = 0, checked = -1, k = 0;
+    bool done = false;
+    auto check = [&](int q) {
+      NODALS_HIP(hipEventSynchronize(W.stepEv[(std::size_t)q]));
+      require(hs[fg::hBad + q] == 0.0, "FGMRES Givens breakdown", kErrBreakdown);
+      k = q + 1;
+      checked = q;
+      if (hs[fg::hEst + q] <= target || hs[fg::hZero + q] != 0.0) done = true;
+    };
+    // Residual estimate expected after the oldest unchecked step, from the
+    // last observed contraction; when it is predicted to converge, the host
+    // waits for it instead of queueing one more step.
+    auto predictsConverged = [&]() {
+      if (checked < 0) return false;
+      const double e1 = hs[fg::hEst + checked];
+      const double e0 = checked > 0 ? hs[fg::hEst + checked - 1] : prevEst;
+      return e0 > 0.0 && e1 * (e1 / e0) <= target;
+    };
+    while (!done) {
+      if (checked + 1 <= queued - 1 && predictsConverged()) {
+        check(checked + 1);
+        continue;
       }
-      hipLaunchKernelGGL(k_multi_axpy, blocksFor(N), 256, 0, s, N, j + 1, V, c, w, -1.0);
-      NODALS_LAUNCHED();
-      const double hn = std::sqrt(std::max(h2[(std::size_t)j + 1] - hh, 0.0));
-      require(std::isfinite(hn), "FGMRES Arnoldi norm non-finite", kErrNonFinite);
-      // Hessenberg column j with the previous rotations applied.
-      for (int i = 0; i <= j; ++i) H[(std::size_t)i * m + j] = h[(std::size_t)i];
-      H[(std::size_t)(j + 1) * m + j] = hn;
-      for (int i = 0; i < j; ++i) {
-        const double a = H[(std::size_t)i * m + j], b = H[(std::size_t)(i + 1) * m + j];
-        H[(std::size_t)i * m + j] = cs[(std::size_t)i] * a + sn[(std::size_t)i] * b;
-        H[(std::size_t)(i + 1) * m + j] = -sn[(std::size_t)i] * a + cs[(std::size_t)i] * b;
-      }
-      const double a = H[(std::size_t)j * m + j], b = H[(std::size_t)(j + 1) * m + j];
-      const double d = std::hypot(a, b);
-      require(d > 0 && std::isfinite(d), "FGMRES Givens breakdown", kErrBreakdown);
-      cs[(std::size_t)j] = a / d;
-      sn[(std::size_t)j] = b / d;
-      H[(std::size_t)j * m + j] = d;
-      H[(std::size_t)(j + 1) * m + j] = 0.0;
-      g[(std::size_t)j + 1] = -sn[(std::size_t)j] * g[(std::size_t)j];
-      g[(std::size_t)j] = cs[(std::size_t)j] * g[(std::size_t)j];
-      ++its;
-      k = j + 1;
-      if (std::abs(g[(std::size_t)j + 1]) <= target || hn == 0.0) {
-        estConverged = true;
+      if (queued < limit) {
+        step(queued);
+        ++queued;
+        if (firstCycle && queued == 1) {
+          // r0 is known once the start kernels have run (step 0 is in flight)
+          NODALS_HIP(hipEventSynchronize(W.stepEv[32]));
+          r0 = hs[fg::hR0];
+          require(std::isfinite(r0), "FGMRES initial norm non-finite", kErrNonFinite);
+          out.r0 = r0;
+          if (r0 <= atol) {
+            NODALS_HIP(hipStreamSynchronize(s));
+            out.ok = true;
+            out.rel = 0.0;
+            return out;
+          }
+          target = std::max(atol, rtol * r0);
+          prevEst = r0;
+        }
+      } else if (checked == queued - 1) {
         break;
       }
-      hipLaunchKernelGGL(k_scale_into<double>, blocksFor(N), 256, 0, s, N, 1.0 / hn, w, w);
-      NODALS_LAUNCHED();
+      const int q = checked + 1;
+      if (q > queued - 1) continue;
+      // check the oldest unchecked step when a newer one is already queued
+      // or when no more can be queued
+      if (queued - 1 > q || queued == limit) check(q);
     }
-    // x += Z y with H y = g (upper triangular k x k)
+    its += k;
+    firstCycle = false;
+    // x += Z y, H y = g (upper triangular k x k, rotated H in host memory)
     for (int i = k - 1; i >= 0; --i) {
-      double t = g[(std::size_t)i];
-      for (int l = i + 1; l < k; ++l) t -= H[(std::size_t)i * m + l] * y[(std::size_t)l];
-      y[(std::size_t)i] = t / H[(std::size_t)i * m + i];
+      double t = hs[fg::hG + i];
+      for (int l = i + 1; l < k; ++l) t -= hs[fg::hH + i * 32 + l] * y[(std::size_t)l];
+      y[(std::size_t)i] = t / hs[fg::hH + i * 32 + i];
     }
     Coef32 c{};
     for (int i = 0; i < k; ++i) c.c[i] = y[(std::size_t)i];
-    hipLaunchKernelGGL(k_multi_axpy, blocksFor(N), 256, 0, s, N, k, Z, c, W.x.data(), 1.0);
+    hipLaunchKernelGGL(k_multi_axpy, nb, 256, 0, s, N, k, Z, c, W.x.data(), 1.0);
     NODALS_LAUNCHED();
     // true FP64 residual
     A(s, W.x.data(), W.r.data());
-    hipLaunchKernelGGL(k_xpby<double>, blocksFor(N), 256, 0, s, N, rhs, -1.0, W.r.data());
+    hipLaunchKernelGGL(k_xpby<double>, nb, 256, 0, s, N, rhs, -1.0, W.r.data());
     NODALS_LAUNCHED();
-    beta = std::sqrt(R.dot(s, N, W.r.data(), W.r.data()));
+    const double beta = std::sqrt(R.dot(s, N, W.r.data(), W.r.data()));
     require(std::isfinite(beta), "FGMRES residual non-finite", kErrNonFinite);
     out.its = its;
     out.rel = beta / r0;
@@ -237,7 +420,8 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
       return out;
     }
     if (its >= maxit) return out;
-    (void)estConverged;
+    start(W.r.data(), false);
+    prevEst = beta;
   }
 }
 

@@ -83,6 +83,26 @@ static __global__ void k_multi_axpy(long long n, int k, const double* __restrict
   for (int j = 0; j < k; ++j) s += c.c[j] * V[(long long)j * n + i];
   y[i] += sign * s;
 }
+// Same with the coefficients read from device memory (k <= 32), so the
+// update can follow a device-side reduction without a host round trip.
+static __global__ void __launch_bounds__(256) k_multi_axpy_dev(long long n, int k, const double* __restrict__ V,
+                                                                const double* __restrict__ coef,
+                                                                double* __restrict__ y, double sign) {
+  __shared__ double c[32];
+  if (threadIdx.x < 32) c[threadIdx.x] = (int)threadIdx.x < k ? coef[threadIdx.x] : 0.0;
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; ++j) s += c[j] * V[(long long)j * n + i];
+  y[i] += sign * s;
+}
+// y = scal[0] * x with the scalar in device memory.
+static __global__ void k_scale_dev(long long n, const double* __restrict__ scal, const double* __restrict__ x,
+                                   double* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = scal[0] * x[i];
+}
 
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double blockSum256(double v, double* sh) {
@@ -109,10 +129,12 @@ __global__ void __launch_bounds__(256) k_dot_part(long long n, const TA* __restr
 
 // K dot products against one vector: part[k*gridDim + block] = sum V_k . w;
 // with NORM, entry k holds w . w (one pass over w for Gram-Schmidt + norm).
+// Each wavefront reduces all its sums with cross-lane shuffles, then one
+// barrier and one LDS pass combine the four wavefronts.
 template <int KMAX, bool NORM>
 __global__ void __launch_bounds__(256) k_multidot_part(long long n, int k, const double* __restrict__ V,
                                                        const double* __restrict__ w, double* __restrict__ part) {
-  __shared__ double sh[4];
+  __shared__ double sh[4][KMAX + 1];
   double s[KMAX + 1];
 #pragma unroll
   for (int j = 0; j <= KMAX; ++j) s[j] = 0.0;
@@ -123,14 +145,19 @@ __global__ void __launch_bounds__(256) k_multidot_part(long long n, int k, const
       if (j < k) s[j] += V[(long long)j * n + i] * wi;
     if (NORM) s[KMAX] += wi * wi;
   }
-  for (int j = 0; j < k; ++j) {
-    const double t = blockSum256(s[j], sh);
-    if (threadIdx.x == 0) part[(long long)j * gridDim.x + blockIdx.x] = t;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j <= KMAX; ++j) {
+    if (j < k || (NORM && j == KMAX)) {
+      const double t = groupSum(s[j], 64);
+      if (lane == 0) sh[wv][j] = t;
+    }
   }
-  if (NORM) {
-    const double t = blockSum256(s[KMAX], sh);
-    if (threadIdx.x == 0) part[(long long)k * gridDim.x + blockIdx.x] = t;
-  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < k) part[(long long)t * gridDim.x + blockIdx.x] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
+  if (NORM && t == k)
+    part[(long long)k * gridDim.x + blockIdx.x] = sh[0][KMAX] + sh[1][KMAX] + sh[2][KMAX] + sh[3][KMAX];
 }
 
 // out[b] = sum of part[b*nparts .. b*nparts+nparts)
@@ -360,6 +387,27 @@ class Reducer {
     NODALS_LAUNCHED();
     NODALS_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < outs; ++j) out[j] = host_[j];
+  }
+  // Device-resident forms: results land in device memory (out[0..]) and
+  // the stream is not synchronised.
+  template <class TA, class TB>
+  void dotDev(hipStream_t s, long long n, const TA* x, const TB* y, double* out) {
+    hipLaunchKernelGGL((k_dot_part<TA, TB>), kReduceParts, 256, 0, s, n, x, y, part_.data());
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_final, 1, 256, 0, s, kReduceParts, part_.data(), out);
+    NODALS_LAUNCHED();
+  }
+  void multidotDev(hipStream_t s, long long n, int k, const double* V, const double* w, double* out,
+                   bool withNorm = false) {
+    const int outs = k + (withNorm ? 1 : 0);
+    if (outs > maxVec_ || k > 32) fail("multidot width exceeds reducer capacity", kErrInternal);
+    if (withNorm)
+      hipLaunchKernelGGL((k_multidot_part<32, true>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
+    else
+      hipLaunchKernelGGL((k_multidot_part<32, false>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_reduce_final, outs, 256, 0, s, kReduceParts, part_.data(), out);
+    NODALS_LAUNCHED();
   }
   double* partials() { return part_.data(); }
   double* hostOut() { return host_; }
