@@ -180,6 +180,21 @@ int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res) {
   });
 }
 
+// CF/PMIS split data: cf_state<l> (uint8, 1 = C, 2 = F), cf_cid<l> (int32
+// coarse id or -1) and cf_rounds (int32, one per split).
+static void copyCf(const nodals::AmgHierarchy& H, const std::string& n, void* host, int64_t count) {
+  if (n == "cf_rounds") {
+    copyOut(std::vector<int32_t>(H.cfRounds.begin(), H.cfRounds.end()), host, count);
+    return;
+  }
+  const bool st = n.rfind("cf_state", 0) == 0;
+  nodals::require(st || n.rfind("cf_cid", 0) == 0, "unknown field " + n);
+  const std::size_t l = (std::size_t)std::stoi(n.substr(st ? 8 : 6));
+  nodals::require(l < H.cfState.size(), "CF split index out of range (or not a CF hierarchy)");
+  if (st) copyOut(H.cfState[l], host, count);
+  else copyOut(H.cfCoarseId[l], host, count);
+}
+
 int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
   return guarded([&]() {
     auto& s = solver(h);
@@ -232,6 +247,10 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
       const std::size_t l = (std::size_t)std::stoi(n.substr(3));
       nodals::require(l < s.hierarchy().agg.size(), "aggregate level out of range");
       copyOut(s.hierarchy().agg[l].id, host, count);
+    } else if (n.rfind("cf_", 0) == 0) {
+      copyCf(s.hierarchy(), n, host, count);
+    } else if (n == "exact_fine_val") {
+      copyOut(nodals::exactFineCsr(s.setup(), s.fineTopo(), s.setup().initRau).val, host, count);
     } else if (n.rfind("P", 0) == 0 || n.rfind("A", 0) == 0) {
       // P<l>_row|col|val and A<l>_row|col|val of the host hierarchy
       const auto u = n.find('_');
@@ -372,6 +391,8 @@ int nodals_sys_get(nodals_sys* h, const char* name, void* host, int64_t count) {
       const std::size_t l = (std::size_t)std::stoi(n.substr(3));
       nodals::require(l < H.agg.size(), "aggregate level out of range");
       copyOut(H.agg[l].id, host, count);
+    } else if (n.rfind("cf_", 0) == 0) {
+      copyCf(H, n, host, count);
     } else if (n == "terminal_inv") {
       copyOut(H.terminalInv, host, count);
     } else if (n == "level_lambda") {

@@ -151,7 +151,6 @@ H8Options parseH8Args(int argc, const char* const* argv) {
   require(!o.dgInlet, "--dg-inlet 1 is not supported by this build");
   require(!o.weakWall, "--weak-wall 1 is not supported by this build");
   require(!o.supg, "--supg 1 is not supported by this build");
-  require(o.amgHierarchy == "sa", "--amg-hierarchy cf is not supported by this build; use sa");
   return o;
 }
 
@@ -178,7 +177,19 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   ao.powerIts = powerIts;
   ao.safety = opt_.amgLambdaSafety;
   ao.lowFraction = opt_.amgLambdaLowFraction;
-  H_ = buildSaHierarchy(M_, S_, F_, S_.initRau, ao);
+  if (opt_.amgHierarchy == "cf") {
+    CfOptions co;
+    co.theta = opt_.cfTheta;
+    co.pmax = opt_.cfPmax;
+    co.interp = opt_.cfInterp;
+    co.aggressiveFirst = opt_.cfAggressiveFirst != 0;
+    co.terminal = opt_.amgTerminal;
+    co.powerIts = powerIts;
+    co.safety = opt_.amgLambdaSafety;
+    H_ = buildCfHierarchy(S_, F_, S_.initRau, co);
+  } else {
+    H_ = buildSaHierarchy(M_, S_, F_, S_.initRau, ao);
+  }
   nc_ = S_.nc;
   nv_ = S_.nv;
   if (!withDevice) {

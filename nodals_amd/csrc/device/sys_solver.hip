@@ -34,6 +34,11 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
     else if (k == "terminal") ao_.terminal = std::stoi(v);
     else if (k == "power_its") ao_.powerIts = std::stoi(v);
     else if (k == "cheb_degree") so_.chebDegree = std::stoi(v);
+    else if (k == "hierarchy") hierarchy_ = v;
+    else if (k == "theta") cf_.theta = std::stod(v);
+    else if (k == "pmax") cf_.pmax = std::stoi(v);
+    else if (k == "interp") cf_.interp = v;
+    else if (k == "aggressive") cf_.aggressiveFirst = std::stoi(v) != 0;
     else fail("unknown sys option " + k);
   }
   require(precision_ == "fp64" || precision_ == "fp32" || precision_ == "bf16", "precision must be fp64|fp32|bf16");
@@ -59,7 +64,15 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
     require(A_.diag[(std::size_t)i] > 0.0, "sys: non-positive or missing diagonal at row " + std::to_string(i));
   }
   const auto t0 = std::chrono::steady_clock::now();
-  H_ = buildSaHierarchyCsr(A_, ao_);
+  require(hierarchy_ == "sa" || hierarchy_ == "cf", "hierarchy must be sa or cf");
+  if (hierarchy_ == "cf") {
+    cf_.terminal = ao_.terminal;
+    cf_.powerIts = so_.kind == "cheb2" ? ao_.powerIts : 0;
+    cf_.safety = ao_.safety;
+    H_ = buildCfHierarchyCsr(A_, cf_);
+  } else {
+    H_ = buildSaHierarchyCsr(A_, ao_);
+  }
   const auto t1 = std::chrono::steady_clock::now();
   setupHostMs_ = std::chrono::duration<double, std::milli>(t1 - t0).count();
   n_ = n;

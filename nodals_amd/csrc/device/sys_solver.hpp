@@ -44,7 +44,8 @@ class SysSolver {
   void apply(hipStream_t s, const double* x, double* y);
   void precond(hipStream_t s, const double* r, double* z);
 
-  std::string precision_ = "fp64";
+  std::string precision_ = "fp64", hierarchy_ = "sa";
+  CfOptions cf_;
   dev::SmootherOptions so_;
   AmgOptions ao_;
   CsrD A_;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
 #include <unordered_map>
 #include <utility>
 
@@ -514,5 +515,311 @@ AmgHierarchy buildSaHierarchyCsr(const CsrD& A0, const AmgOptions& o) {
   finishHierarchy(H, o);
   return H;
 }
+
+CsrD galerkinProduct(const CsrD& A, const Transfer& P) { return galerkin(A, P); }
+
+// ------------------------------------------------------------------ CF/PMIS
+
+CsrD exactFineCsr(const H8Setup& S, const FineTopo& F, const std::vector<double>& rau) {
+  // Per cell, per free basis a, per incidence entry of its DOF, add
+  // rAU_g (B[c][a] . B[c2][b]) into the slot recorded for that contribution
+  // (g8_cf_hierarchy_host.hpp:116-166).
+  require((int)rau.size() == F.nv, "CF fine CSR rAU size mismatch");
+  CsrD A;
+  A.n = F.n;
+  A.row.assign(F.row.begin(), F.row.end());
+  A.col = F.col;
+  A.val.assign(F.col.size(), 0.0);
+  A.diag.assign((std::size_t)F.n, 0.0);
+  const double* B = S.bcoef.data();
+  for (int c = 0; c < F.n; ++c) {
+    const int rb = F.row[(std::size_t)c];
+    const CellPlan& cp = S.cells[(std::size_t)c];
+    int q = F.contribOff[(std::size_t)c];
+    for (int a = 0; a < 8; ++a) {
+      const int g = cp.ref[a];
+      if (g < 0) continue;
+      const double rv = rau[(std::size_t)g];
+      const double* ba = B + (std::size_t)c * 24 + 3 * a;
+      for (std::int32_t k = F.incOff[(std::size_t)g]; k < F.incOff[(std::size_t)g + 1]; ++k, ++q) {
+        const std::uint32_t pk = F.packed[(std::size_t)k];
+        const int c2 = (int)(pk >> 3), b = (int)(pk & 7u);
+        const double* bb = B + (std::size_t)c2 * 24 + 3 * b;
+        const double dot = ba[0] * bb[0] + ba[1] * bb[1] + ba[2] * bb[2];
+        A.val[(std::size_t)(rb + (int)F.slot[(std::size_t)q])] += rv * dot;
+      }
+    }
+    const double d = A.val[(std::size_t)F.diagPos[(std::size_t)c]];
+    require(d > 0.0 && std::isfinite(d), "CF fine CSR non-positive diagonal");
+    A.diag[(std::size_t)c] = d;
+  }
+  return A;
+}
+
+std::uint64_t cfHash64(std::uint64_t x) {
+  // SplitMix64 output function (Steele, Lea and Flood, OOPSLA 2014): the
+  // state advanced by the golden-ratio increment, then two xor-shift-multiply
+  // rounds and a final xor-shift.
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+namespace {
+
+double csrEntry(const CsrD& A, int i, int j) {
+  const auto b = A.col.begin() + A.row[(std::size_t)i];
+  const auto e = A.col.begin() + A.row[(std::size_t)i + 1];
+  const auto it = std::lower_bound(b, e, j);
+  return (it == e || *it != j) ? 0.0 : A.val[(std::size_t)(it - A.col.begin())];
+}
+
+}  // namespace
+
+CfSplit pmisSplit(const CsrD& A, double theta) {
+  // g8_cf_hierarchy_host.hpp:169-293.
+  const int n = A.n;
+  CfSplit R;
+  R.n = n;
+  R.strong.resize((std::size_t)n);
+  R.graph.resize((std::size_t)n);
+  for (int i = 0; i < n; ++i) {
+    double mx = 0.0;
+    for (std::int64_t k = A.row[(std::size_t)i]; k < A.row[(std::size_t)i + 1]; ++k) {
+      const int j = A.col[(std::size_t)k];
+      const double a = A.val[(std::size_t)k];
+      if (j != i && a < 0.0) mx = std::max(mx, -a);
+    }
+    const double thr = theta * mx;
+    if (thr > 0.0) {
+      for (std::int64_t k = A.row[(std::size_t)i]; k < A.row[(std::size_t)i + 1]; ++k) {
+        const int j = A.col[(std::size_t)k];
+        const double a = A.val[(std::size_t)k];
+        if (j != i && a < 0.0 && -a >= thr) R.strong[(std::size_t)i].push_back(j);
+      }
+    }
+    // CSR columns are sorted and unique, so each strong row is too.
+  }
+  // Symmetric union graph, sorted and unique.
+  for (int i = 0; i < n; ++i)
+    for (int j : R.strong[(std::size_t)i]) {
+      R.graph[(std::size_t)i].push_back(j);
+      R.graph[(std::size_t)j].push_back(i);
+    }
+  for (auto& g : R.graph) {
+    std::sort(g.begin(), g.end());
+    g.erase(std::unique(g.begin(), g.end()), g.end());
+  }
+  // Priority: larger union degree, then larger hash of (index + 0x63021),
+  // then larger index.
+  std::vector<std::uint64_t> h((std::size_t)n);
+  for (int i = 0; i < n; ++i) h[(std::size_t)i] = cfHash64((std::uint64_t)i + 0x63021ull);
+  auto higher = [&](int a, int b) {
+    const std::size_t da = R.graph[(std::size_t)a].size(), db = R.graph[(std::size_t)b].size();
+    if (da != db) return da > db;
+    if (h[(std::size_t)a] != h[(std::size_t)b]) return h[(std::size_t)a] > h[(std::size_t)b];
+    return a > b;
+  };
+  R.state.assign((std::size_t)n, 0);
+  std::vector<int> winners;
+  int undecided = n;
+  while (undecided > 0) {
+    winners.clear();
+    for (int i = 0; i < n; ++i) {
+      if (R.state[(std::size_t)i] != 0) continue;
+      bool win = true;
+      for (int j : R.graph[(std::size_t)i])
+        if (R.state[(std::size_t)j] == 0 && higher(j, i)) {
+          win = false;
+          break;
+        }
+      if (win) winners.push_back(i);
+    }
+    require(!winners.empty(), "PMIS made no progress", kErrInternal);
+    for (int i : winners) {
+      R.state[(std::size_t)i] = 1;
+      --undecided;
+    }
+    for (int i : winners)
+      for (int j : R.graph[(std::size_t)i])
+        if (R.state[(std::size_t)j] == 0) {
+          R.state[(std::size_t)j] = 2;
+          --undecided;
+        }
+    ++R.rounds;
+    require(R.rounds <= n, "PMIS excessive rounds", kErrInternal);
+  }
+  R.coarseId.assign((std::size_t)n, -1);
+  for (int i = 0; i < n; ++i)
+    if (R.state[(std::size_t)i] == 1) R.coarseId[(std::size_t)i] = R.nC++;
+  return R;
+}
+
+Transfer cfTransfer(const CsrD& A, const CfSplit& S, int pmax, const std::string& interp) {
+  // g8_cf_hierarchy_host.hpp:625-785.  Weights accumulate per coarse target
+  // in visiting order; candidates are then ordered by (weight desc, target
+  // asc), cut to pmax, normalised to unit sum in that order and stored by
+  // target.
+  require(interp == "direct" || interp == "exti", "CF interpolation must be direct or exti");
+  Transfer P;
+  P.nFine = A.n;
+  P.nCoarse = S.nC;
+  P.row.assign((std::size_t)A.n + 1, 0);
+  std::map<int, double> acc;
+  std::vector<Entry> cand;
+  const bool exti = interp == "exti";
+  for (int i = 0; i < A.n; ++i) {
+    cand.clear();
+    if (S.state[(std::size_t)i] == 1) {
+      cand.push_back({S.coarseId[(std::size_t)i], 1.0});
+    } else {
+      acc.clear();
+      for (int j : S.strong[(std::size_t)i]) {
+        if (S.state[(std::size_t)j] != 1) continue;
+        const double a = csrEntry(A, i, j);
+        const double w = a < 0.0 ? -a : std::abs(a);
+        if (w > 0.0) acc[S.coarseId[(std::size_t)j]] += w;
+      }
+      if (exti) {
+        for (int k : S.strong[(std::size_t)i]) {
+          if (S.state[(std::size_t)k] != 2) continue;
+          const double aik = csrEntry(A, i, k);
+          const double path = aik < 0.0 ? -aik : 0.0;
+          if (!(path > 0.0)) continue;
+          double denom = 0.0;
+          for (int j : S.strong[(std::size_t)k]) {
+            if (S.state[(std::size_t)j] != 1) continue;
+            const double akj = csrEntry(A, k, j);
+            if (akj < 0.0) denom += -akj;
+          }
+          if (!(denom > 0.0)) continue;
+          for (int j : S.strong[(std::size_t)k]) {
+            if (S.state[(std::size_t)j] != 1) continue;
+            const double akj = csrEntry(A, k, j);
+            if (akj < 0.0) acc[S.coarseId[(std::size_t)j]] += path * ((-akj) / denom);
+          }
+        }
+      }
+      if (acc.empty()) {
+        for (int j : S.graph[(std::size_t)i]) {
+          if (S.state[(std::size_t)j] != 1) continue;
+          const double w = std::abs(csrEntry(A, i, j));
+          if (w > 0.0) acc[S.coarseId[(std::size_t)j]] += w;
+        }
+      }
+      require(!acc.empty(), "CF interpolation row has no C target");
+      for (const auto& kv : acc)
+        if (kv.second > 0.0) cand.push_back(kv);
+      require(!cand.empty(), "CF interpolation empty candidates");
+      std::sort(cand.begin(), cand.end(), [](const Entry& a, const Entry& b) {
+        return a.second != b.second ? a.second > b.second : a.first < b.first;
+      });
+      if (pmax > 0 && (int)cand.size() > pmax) cand.resize((std::size_t)pmax);
+      double sum = 0.0;
+      for (const Entry& e : cand) sum += e.second;
+      require(sum > 0.0 && std::isfinite(sum), "CF interpolation invalid weight sum");
+      for (Entry& e : cand) e.second /= sum;
+      std::sort(cand.begin(), cand.end(), [](const Entry& a, const Entry& b) { return a.first < b.first; });
+    }
+    for (const Entry& e : cand) {
+      P.col.push_back(e.first);
+      P.val.push_back(e.second);
+    }
+    P.row[(std::size_t)i + 1] = (std::int64_t)P.col.size();
+  }
+  return P;
+}
+
+Transfer composeTransfers(const Transfer& P01, const Transfer& P12) {
+  // Exact product P01 P12, exact zeros dropped, no pruning
+  // (g8_cf_hierarchy_host.hpp:1100-1195).
+  require(P01.nCoarse == P12.nFine, "transfer composition size mismatch");
+  Transfer Q;
+  Q.nFine = P01.nFine;
+  Q.nCoarse = P12.nCoarse;
+  Q.row.assign((std::size_t)Q.nFine + 1, 0);
+  std::map<int, double> acc;
+  for (int i = 0; i < Q.nFine; ++i) {
+    acc.clear();
+    for (std::int64_t a = P01.row[(std::size_t)i]; a < P01.row[(std::size_t)i + 1]; ++a) {
+      const int mid = P01.col[(std::size_t)a];
+      const double w0 = P01.val[(std::size_t)a];
+      for (std::int64_t b = P12.row[(std::size_t)mid]; b < P12.row[(std::size_t)mid + 1]; ++b)
+        acc[P12.col[(std::size_t)b]] += w0 * P12.val[(std::size_t)b];
+    }
+    require(!acc.empty(), "composed interpolation empty row");
+    for (const auto& kv : acc)
+      if (kv.second != 0.0) {
+        Q.col.push_back(kv.first);
+        Q.val.push_back(kv.second);
+      }
+    Q.row[(std::size_t)i + 1] = (std::int64_t)Q.col.size();
+  }
+  return Q;
+}
+
+namespace {
+
+AmgHierarchy cfFromFine(CsrD A, const CfOptions& o) {
+  // g8_cf_hierarchy_host.hpp:1584-1783.
+  AmgHierarchy H;
+  H.fineDiag = A.diag;
+  H.fineLambda = o.powerIts > 0 ? powerCsr(A, o.powerIts, o.safety) : 1.0;
+  auto split = [&](const CsrD& M) {
+    CfSplit S = pmisSplit(M, o.theta);
+    H.cfState.push_back(S.state);
+    H.cfCoarseId.push_back(S.coarseId);
+    H.cfRounds.push_back(S.rounds);
+    return S;
+  };
+  auto lambdaOf = [&](const CsrD& M) { return o.powerIts > 0 ? powerCsr(M, o.powerIts, o.safety) : 1.0; };
+  int logical = 0;
+  if (o.aggressiveFirst && A.n > o.terminal) {
+    const CfSplit s0 = split(A);
+    Transfer P01 = cfTransfer(A, s0, o.pmax, o.interp);
+    CsrD A1 = galerkin(A, P01);
+    if (A1.n <= o.terminal) {
+      H.P.push_back(std::move(P01));
+      H.A.push_back(std::move(A1));
+    } else {
+      const CfSplit s1 = split(A1);
+      Transfer P12 = cfTransfer(A1, s1, o.pmax, o.interp);
+      CsrD A2 = galerkin(A1, P12);
+      H.P.push_back(composeTransfers(P01, P12));
+      H.A.push_back(std::move(A2));
+    }
+    if (H.A.back().n > o.terminal) {
+      H.levelLambda.assign(H.A.size(), 0.0);
+      H.levelLambda[0] = lambdaOf(H.A.back());
+    }
+    A = H.A.back();
+    logical = 1;
+  }
+  while (H.A.empty() || H.A.back().n > o.terminal) {
+    const CfSplit s = split(A);
+    H.P.push_back(cfTransfer(A, s, o.pmax, o.interp));
+    H.A.push_back(galerkin(A, H.P.back()));
+    if (H.A.back().n <= o.terminal) break;
+    if (H.levelLambda.size() < H.A.size()) H.levelLambda.resize(H.A.size(), 0.0);
+    H.levelLambda[H.A.size() - 1] = lambdaOf(H.A.back());
+    A = H.A.back();
+    ++logical;
+    require(logical <= 20, "CF hierarchy excessive levels");
+  }
+  if (H.levelLambda.size() < H.A.size()) H.levelLambda.resize(H.A.size(), 1.0);
+  H.terminalN = H.A.back().n;
+  H.terminalInv = denseInverseCholesky(H.A.back());
+  return H;
+}
+
+}  // namespace
+
+AmgHierarchy buildCfHierarchy(const H8Setup& S, const FineTopo& F, const std::vector<double>& rau,
+                              const CfOptions& o) {
+  return cfFromFine(exactFineCsr(S, F, rau), o);
+}
+
+AmgHierarchy buildCfHierarchyCsr(const CsrD& A0, const CfOptions& o) { return cfFromFine(A0, o); }
 
 }  // namespace nodals

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "h8_setup.hpp"
@@ -47,7 +48,45 @@ struct AmgHierarchy {
   std::vector<double> levelLambda;  // per explicit level, 0 for the terminal
   std::vector<double> terminalInv;  // dense row-major inverse of A.back()
   int terminalN = 0;
+  // CF/PMIS only: the C/F state (1 = C, 2 = F) and coarse id of every split
+  // made, in construction order (an aggressive first stage adds two).
+  std::vector<std::vector<std::uint8_t>> cfState;
+  std::vector<std::vector<std::int32_t>> cfCoarseId;
+  std::vector<int> cfRounds;
 };
+
+struct CfOptions {
+  double theta = 0.25;
+  int pmax = 4;
+  std::string interp = "direct";  // direct | exti
+  bool aggressiveFirst = false;
+  int terminal = 1000;
+  int powerIts = 0;
+  double safety = 1.5;
+};
+
+// PMIS C/F split of A under the classical negative-coupling strength test.
+struct CfSplit {
+  int n = 0, nC = 0, rounds = 0;
+  std::vector<std::uint8_t> state;  // 1 = C, 2 = F
+  std::vector<std::int32_t> coarseId;
+  std::vector<std::vector<std::int32_t>> strong, graph;
+};
+std::uint64_t cfHash64(std::uint64_t x);
+CfSplit pmisSplit(const CsrD& A, double theta);
+Transfer cfTransfer(const CsrD& A, const CfSplit& S, int pmax, const std::string& interp);
+Transfer composeTransfers(const Transfer& P01, const Transfer& P12);
+CsrD galerkinProduct(const CsrD& A, const Transfer& P);
+
+// Exact assembled fine Schur CSR B diag(rau) B^T in the reference summation
+// order (g8_cf_hierarchy_host.hpp:116-166).
+CsrD exactFineCsr(const H8Setup& S, const FineTopo& F, const std::vector<double>& rau);
+
+// CF/PMIS hierarchy of the fine Schur operator (g8_cf_hierarchy_host.hpp:
+// 1584-1783); of an explicit CSR operator with buildCfHierarchyCsr.
+AmgHierarchy buildCfHierarchy(const H8Setup& S, const FineTopo& F, const std::vector<double>& rau,
+                              const CfOptions& o);
+AmgHierarchy buildCfHierarchyCsr(const CsrD& A0, const CfOptions& o);
 
 using Graph = std::vector<std::vector<std::int32_t>>;
 

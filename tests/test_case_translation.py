@@ -144,11 +144,12 @@ def test_translated_argv_accepted_by_solver_parser(tmp_path):
     from nodals_amd._lib import H8
 
     mesh = meshgen.write_pipe_case(tmp_path / "pipe", core=4, ring=2, nslab=4)
-    cp = m._read_case(CASE)
-    m._case_overrides(cp, [f"solver.mesh={mesh}", "pressure_amg.hierarchy=sa"])
-    h = H8(m.build_gpu_options(cp), device=False)
-    assert h.info["cells"] == (16 + 32) * 4 * 6
-    h.close()
+    for hierarchy in ("cf", "sa"):  # cf (PMIS, ext+i, pmax 8) is the translator default
+        cp = m._read_case(CASE)
+        m._case_overrides(cp, [f"solver.mesh={mesh}", f"pressure_amg.hierarchy={hierarchy}"])
+        h = H8(m.build_gpu_options(cp), device=False)
+        assert h.info["cells"] == (16 + 32) * 4 * 6
+        h.close()
 
 
 def _fake_solver(tmp_path: Path, body: str) -> Path:

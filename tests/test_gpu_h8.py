@@ -274,3 +274,47 @@ def test_converged_fields_match_oracle(mesh_dir):
     assert abs(O.pressure_drop_fit(case, p) - O.pressure_drop_fit(case, ref["p"])) <= 1e-3 * abs(
         O.pressure_drop_fit(case, ref["p"]))
     h.close()
+
+
+@pytest.mark.parametrize("interp", ["exti", "direct"])
+def test_cf_vcycle_matches_oracle(mesh_dir, interp):
+    """V-cycle on the CF/PMIS hierarchy (the translator's default) equals the
+    oracle V-cycle on the same hierarchy."""
+    from nodals_amd._lib import H8
+
+    print(f"\n[gpu] H8 create small pipe amg-hierarchy=cf interp={interp}", flush=True)
+    h = H8(["--mesh", mesh_dir["small"], *BASE, "--amg-smoother", "jacobi", "--pressure-solver", "pcg",
+            "--amg-hierarchy", "cf", "--cf-interp", interp, "--cf-pmax", "8", "--amg-terminal", "100"])
+    case = O.case_from_product(h)
+    assert len(case.hier["P"]) >= 2
+    U, p = _random_state(case, 11)
+    _set_state(h, U, p)
+    h.assemble(0.5)
+    h.pressure_refresh()
+    Af, fdiag = O.fine_refresh(case, h.get("rau", case.nv))
+    vc = O.VCycle(case.hier, O.Smoother(kind="jacobi", jacobi_omega=0.7))
+    vc.set_fine(Af, fdiag)
+    r = np.random.default_rng(3).standard_normal(case.nc)
+    assert rel(h.precond(r), vc.apply(r)) < 1e-12
+    h.close()
+
+
+def test_cf_hierarchy_converges_to_same_fields(mesh_dir):
+    """The preconditioner does not change the converged SIMPLE fields: CF/PMIS
+    (ext+i, pmax 8) and SA give the same velocity and pressure to 1e-6."""
+    from nodals_amd._lib import H8
+
+    args = ["--mesh", mesh_dir["small"], "--run-mode", "converge", "--max-outer", "3000", "--simple-tol", "1e-9",
+            "--amg-smoother", "jacobi", "--pressure-solver", "fgmres", "--p-rtol", "1e-2", "--p-max", "200",
+            "--verbose", "0"]
+    fields = {}
+    for hier in ("sa", "cf"):
+        print(f"\n[gpu] SIMPLE converge small pipe amg-hierarchy={hier}", flush=True)
+        extra = ["--amg-hierarchy", hier] + (["--cf-interp", "exti", "--cf-pmax", "8"] if hier == "cf" else [])
+        h = H8(args + extra)
+        r = h.run()
+        assert r["converged"] == 1, (hier, r)
+        fields[hier] = [h.get(f"u{d}", h.info["nv"]) for d in range(3)] + [h.get("p", h.info["cells"])]
+        h.close()
+    for a, b in zip(fields["cf"], fields["sa"]):
+        assert rel(a, b) < 1e-6
