@@ -165,6 +165,38 @@ int nodals_sys_set_rhs(nodals_sys* s, const double* b);
  * algorithmic bytes of one launch. */
 int nodals_sys_time_spmv(nodals_sys* s, int32_t level, int32_t reps, double* ms_avg, double* bytes);
 
+/* ---------------------------------------------------------------- SST ---
+ * Pointwise k-omega SST-2003 and wall-law evaluations on the GPU, batched
+ * over n points (host arrays in and out, copied through device memory).
+ *
+ * nodals_sst_local replaces nodals::sst2003m::evaluate_host /
+ * evaluate_at_distance (turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/
+ * modules/physics/sst2003m_host.hpp:5-10, sst2003m_distance_core.hpp:9-22):
+ * wall distance from dist[n] (floored), or when pipe_xy[2n] is given, the
+ * pipe distance radius - |(x, y)|.  gu[n][9] row-major du_i/dx_j, k[n],
+ * omega[n], grad_k[n][3], grad_omega[n][3].  out[n][14] = nut F1 F2 sigmaK
+ * sigmaW beta gamma strain pk srcOmega reactK reactOmega omegaProduction
+ * omegaCross; floored[n][2] = k, omega floored. */
+int nodals_sst_local(int64_t n, const double* dist, const double* pipe_xy, double radius, double nu,
+                     const double* gu, const double* k, const double* omega, const double* grad_k,
+                     const double* grad_omega, double k_floor, double omega_floor, int32_t deviatoric,
+                     double* out, int32_t* floored);
+/* Spalding friction velocity per sample (nodals::wall::spalding,
+ * modules/wall/portable/spalding_core.hpp:86-230): out[n][3] = u_tau y+ beta;
+ * ok[n]. */
+int nodals_wall_spalding(int64_t n, const double* slip, const double* y, double nu, double* out, int32_t* ok);
+/* Five-point least-squares Spalding fit per wall face (fit_profile5_host,
+ * modules/wall/portable/wall_host.hpp:11-12): U, U_signed, y are [n][5];
+ * out[n][6] = u_tau y+ beta u_mid fit_rms point_u_tau_rel_span; valid[n]. */
+int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, const double* y, double nu,
+                         double* out, int32_t* valid);
+/* OpenFOAM-style omega wall target (evaluate_of_auto_host,
+ * modules/wall/portable/wall_host.hpp:5-9): in[n][15] = y nu Ut uTau k omega
+ * kFloor omegaFloor kappa E beta1 reBlend prodLimit pk nut; shared[n];
+ * out[n][8] = omegaTarget omegaVis omegaLog lamFrac rey yPlusK pkAuto
+ * nutSpalding. */
+int nodals_wall_omega_target(int64_t n, const double* in, const int32_t* shared, double* out);
+
 #ifdef __cplusplus
 }
 #endif

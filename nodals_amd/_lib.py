@@ -52,6 +52,7 @@ EXPORTS = [
     "nodals_sys_create", "nodals_sys_create_host", "nodals_sys_get", "nodals_sys_destroy", "nodals_sys_info",
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
+    "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
 ]
 
 
@@ -85,6 +86,11 @@ def lib() -> C.CDLL:
     L.nodals_h8_precond.argtypes = [vp, vp, vp]
     L.nodals_h8_time_kernel.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nodals_h8_probe.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.nodals_sst_local.argtypes = [C.c_int64, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp, vp, C.c_double,
+                                   C.c_double, C.c_int32, vp, vp]
+    L.nodals_wall_spalding.argtypes = [C.c_int64, vp, vp, C.c_double, vp, vp]
+    L.nodals_wall_profile5.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
+    L.nodals_wall_omega_target.argtypes = [C.c_int64, vp, vp, vp]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -307,3 +313,57 @@ class System:
         ms, by = C.c_double(), C.c_double()
         _check(lib().nodals_sys_time_spmv(self._h, level, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+
+# ---------------------------------------------------------------- SST / wall
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+SST_FIELDS = ("nut", "F1", "F2", "sigmaK", "sigmaW", "beta", "gamma", "strain", "pk", "srcOmega", "reactK",
+              "reactOmega", "omegaProduction", "omegaCross")
+
+
+def sst_local(gu, k, omega, grad_k, grad_omega, nu, radius, dist=None, pipe_xy=None, k_floor=1e-8,
+              omega_floor=1e-3, deviatoric=False):
+    """SST-2003 local state at n points on the GPU: (out[n,14], floored[n,2])."""
+    n = int(np.asarray(k).size)
+    gu, k, omega = _f64(gu, (n, 9)), _f64(k), _f64(omega)
+    gk, gw = _f64(grad_k, (n, 3)), _f64(grad_omega, (n, 3))
+    d = None if dist is None else _f64(dist)
+    xy = None if pipe_xy is None else _f64(pipe_xy, (n, 2))
+    out = np.empty((n, 14))
+    fl = np.empty((n, 2), dtype=np.int32)
+    _check(lib().nodals_sst_local(n, None if d is None else _ptr(d), None if xy is None else _ptr(xy), radius, nu,
+                                  _ptr(gu), _ptr(k), _ptr(omega), _ptr(gk), _ptr(gw), k_floor, omega_floor,
+                                  1 if deviatoric else 0, _ptr(out), _ptr(fl)))
+    return out, fl
+
+
+def wall_spalding(slip, y, nu):
+    n = int(np.asarray(slip).size)
+    s, yy = _f64(slip), _f64(y)
+    out = np.empty((n, 3))
+    ok = np.empty(n, dtype=np.int32)
+    _check(lib().nodals_wall_spalding(n, _ptr(s), _ptr(yy), nu, _ptr(out), _ptr(ok)))
+    return out, ok
+
+
+def wall_profile5(U, Us, y, nu):
+    n = int(np.asarray(U).size // 5)
+    U, Us, y = _f64(U, (n, 5)), _f64(Us, (n, 5)), _f64(y, (n, 5))
+    out = np.empty((n, 6))
+    valid = np.empty(n, dtype=np.int32)
+    _check(lib().nodals_wall_profile5(n, _ptr(U), _ptr(Us), _ptr(y), nu, _ptr(out), _ptr(valid)))
+    return out, valid
+
+
+def wall_omega_target(inp, shared):
+    n = int(np.asarray(shared).size)
+    inp = _f64(inp, (n, 15))
+    sh = np.ascontiguousarray(shared, dtype=np.int32)
+    out = np.empty((n, 8))
+    _check(lib().nodals_wall_omega_target(n, _ptr(inp), _ptr(sh), _ptr(out)))
+    return out
+

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "h8_solver.hpp"
+#include "sst_kernels.hpp"
 #include "sys_solver.hpp"
 
 struct nodals_h8 {
@@ -471,6 +472,29 @@ int nodals_sys_time_spmv(nodals_sys* h, int32_t level, int32_t reps, double* ms_
     const double ms = sys(h).timeSpmv(level, reps, bytes);
     if (ms_avg) *ms_avg = ms;
   });
+}
+
+int nodals_sst_local(int64_t n, const double* dist, const double* pipe_xy, double radius, double nu,
+                     const double* gu, const double* k, const double* omega, const double* grad_k,
+                     const double* grad_omega, double k_floor, double omega_floor, int32_t deviatoric,
+                     double* out, int32_t* floored) {
+  return guarded([&]() {
+    nodals::sstLocalBatch(n, dist, pipe_xy, radius, nu, gu, k, omega, grad_k, grad_omega, k_floor, omega_floor,
+                          deviatoric, out, floored);
+  });
+}
+
+int nodals_wall_spalding(int64_t n, const double* slip, const double* y, double nu, double* out, int32_t* ok) {
+  return guarded([&]() { nodals::spaldingBatch(n, slip, y, nu, out, ok); });
+}
+
+int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, const double* y, double nu,
+                         double* out, int32_t* valid) {
+  return guarded([&]() { nodals::profile5Batch(n, U, U_signed, y, nu, out, valid); });
+}
+
+int nodals_wall_omega_target(int64_t n, const double* in, const int32_t* shared, double* out) {
+  return guarded([&]() { nodals::omegaWallBatch(n, in, shared, out); });
 }
 
 }  // extern "C"

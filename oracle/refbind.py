@@ -109,3 +109,73 @@ class RefCase:
     def pressure_drop(self, p: np.ndarray) -> float:
         p = np.ascontiguousarray(p, dtype=np.float64)
         return float(self.L.ref_pressure_drop(self.h, p.ctypes.data))
+
+
+REF_SST_LIB = Path(__file__).resolve().parent / "_ref" / "libnodals_ref_sst.so"
+
+
+def sst_available() -> bool:
+    return REF_SST_LIB.exists()
+
+
+class RefSst:
+    """The reference's portable SST-2003 and wall-law cores
+    (modules/physics/sst2003m_host.cpp, modules/wall/portable/wall_host.cpp),
+    compiled from the reference tree, looped over arrays."""
+
+    def __init__(self):
+        L = C.CDLL(str(REF_SST_LIB))
+        vp = C.c_void_p
+        L.ref_sst_local.argtypes = [C.c_longlong, C.c_int, vp, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp, vp,
+                                    C.c_double, C.c_double, C.c_int, vp, vp]
+        L.ref_spalding.argtypes = [C.c_longlong, vp, vp, C.c_double, vp, vp]
+        L.ref_profile5.argtypes = [C.c_longlong, vp, vp, vp, C.c_double, vp, vp]
+        L.ref_of_auto.argtypes = [C.c_longlong, vp, vp, vp]
+        self.L = L
+
+    @staticmethod
+    def _p(a):
+        return C.c_void_p(a.ctypes.data) if a is not None else None
+
+    def sst_local(self, gu, k, omega, grad_k, grad_omega, nu, radius, dist=None, pipe_xy=None, k_floor=1e-8,
+                  omega_floor=1e-3, deviatoric=False):
+        n = int(np.asarray(k).size)
+        f = lambda a, s=None: None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(s or -1)
+        gu, k, omega, gk, gw = f(gu, (n, 9)), f(k), f(omega), f(grad_k, (n, 3)), f(grad_omega, (n, 3))
+        pipe = pipe_xy is not None
+        x = y = d = None
+        if pipe:
+            xy = f(pipe_xy, (n, 2))
+            x, y = np.ascontiguousarray(xy[:, 0]), np.ascontiguousarray(xy[:, 1])
+        else:
+            d = f(dist)
+        out = np.empty((n, 14))
+        fl = np.empty((n, 2), dtype=np.int32)
+        self.L.ref_sst_local(n, 1 if pipe else 0, self._p(x), self._p(y), self._p(d), radius, nu, self._p(gu),
+                             self._p(k), self._p(omega), self._p(gk), self._p(gw), k_floor, omega_floor,
+                             1 if deviatoric else 0, self._p(out), self._p(fl))
+        return out, fl
+
+    def spalding(self, slip, y, nu):
+        n = int(np.asarray(slip).size)
+        s, yy = np.ascontiguousarray(slip, dtype=np.float64), np.ascontiguousarray(y, dtype=np.float64)
+        out = np.empty((n, 3))
+        ok = np.empty(n, dtype=np.int32)
+        self.L.ref_spalding(n, self._p(s), self._p(yy), nu, self._p(out), self._p(ok))
+        return out, ok
+
+    def profile5(self, U, Us, y, nu):
+        n = int(np.asarray(U).size // 5)
+        a = [np.ascontiguousarray(v, dtype=np.float64).reshape(n, 5) for v in (U, Us, y)]
+        out = np.empty((n, 6))
+        valid = np.empty(n, dtype=np.int32)
+        self.L.ref_profile5(n, *(self._p(v) for v in a), nu, self._p(out), self._p(valid))
+        return out, valid
+
+    def of_auto(self, inp, shared):
+        n = int(np.asarray(shared).size)
+        i = np.ascontiguousarray(inp, dtype=np.float64).reshape(n, 15)
+        sh = np.ascontiguousarray(shared, dtype=np.int32)
+        out = np.empty((n, 8))
+        self.L.ref_of_auto(n, self._p(i), self._p(sh), self._p(out))
+        return out
