@@ -48,6 +48,11 @@ typedef struct {
   double loop_ms;
   double assembly_ms, momentum_ms, continuity_ms, pressure_ms, pupdate_ms;
   double setup_ms;
+  /* Extensions (no reference counterpart): total pressure Krylov
+   * iterations, learned warm starts tried, dropped (fallback), samples
+   * recorded. */
+  int64_t pressure_its;
+  int32_t warm_starts, fallbacks, recorded, reserved2;
 } nodals_h8_run_result;
 
 /* Last error message of the calling thread ("" if none). */
@@ -164,6 +169,45 @@ int nodals_sys_set_rhs(nodals_sys* s, const double* b);
  * operator; 0..: explicit AMG levels in the hierarchy precision), with the
  * algorithmic bytes of one launch. */
 int nodals_sys_time_spmv(nodals_sys* s, int32_t level, int32_t reps, double* ms_avg, double* bytes);
+
+/* ---------------------------------------------------------------- GNN ---
+ * Learned pressure-correction warm start (no reference counterpart; the
+ * reference's only data-driven pressure aid is the snapshot POD of
+ * turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/pod_balanced.inc).
+ * A 4-layer message-passing network, width 64, on the graph of face-adjacent
+ * cells, with BF16-MFMA forward and backward kernels (FP32 accumulation) and
+ * Adam; it predicts each SIMPLE iteration's pressure correction from the
+ * continuity residual, the two previous corrections and the wall distance.
+ * The FGMRES pressure solve then starts from the prediction and stops at the
+ * same tolerance relative to ||b||; a prediction that does not lower the
+ * residual is dropped.  Training samples are recorded from the solver's own
+ * SIMPLE iterations (features, converged correction). */
+typedef struct nodals_gnn nodals_gnn;
+/* A network on the cell graph of an H8 context, weights from seed. */
+int nodals_gnn_create_for_h8(nodals_h8* h, uint32_t seed, nodals_gnn** out);
+int nodals_gnn_destroy(nodals_gnn* g);
+/* info[4] = cells, padded rows, parameter count, recorded samples. */
+int nodals_gnn_info(nodals_gnn* g, int64_t info[4]);
+int nodals_gnn_get_params(nodals_gnn* g, float* host, int64_t count);
+int nodals_gnn_set_params(nodals_gnn* g, const float* host, int64_t count);
+/* Clears the training set and sizes it for `capacity` samples. */
+int nodals_gnn_reserve(nodals_gnn* g, int32_t capacity);
+/* Reads recorded sample i: x[cells][16], t[cells]. */
+int nodals_gnn_sample(nodals_gnn* g, int32_t i, float* x, float* t);
+/* y = network(x) for x[cells][16] (host arrays). */
+int nodals_gnn_forward(nodals_gnn* g, const float* x, float* y);
+/* Gradient of mean((y - t)^2) over the cells w.r.t. every parameter
+ * (grad[params]) and the loss, without updating. */
+int nodals_gnn_gradient(nodals_gnn* g, const float* x, const float* t, float* grad, double* loss);
+/* Adam epochs over the recorded samples in a seeded shuffled order; mean
+ * loss of the last epoch. */
+int nodals_gnn_train(nodals_gnn* g, int32_t epochs, float lr, uint32_t seed, double* last_loss);
+int nodals_gnn_evaluate(nodals_gnn* g, double* loss);
+/* Attach g to an H8 context (NULL detaches): warm_start != 0 predicts every
+ * pressure correction (needs --pressure-solver fgmres); record_every > 0
+ * records a sample every that many SIMPLE iterations until the set is
+ * full.  g must stay alive while attached. */
+int nodals_h8_attach_gnn(nodals_h8* h, nodals_gnn* g, int32_t warm_start, int32_t record_every);
 
 /* ---------------------------------------------------------------- SST ---
  * Pointwise k-omega SST-2003 and wall-law evaluations on the GPU, batched

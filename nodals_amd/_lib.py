@@ -34,10 +34,12 @@ class RunResult(C.Structure):
                 ("final_rel_cont", C.c_double), ("avg_pressure_its", C.c_double), ("pressure_drop", C.c_double),
                 ("exact_pressure_drop", C.c_double), ("loop_ms", C.c_double), ("assembly_ms", C.c_double),
                 ("momentum_ms", C.c_double), ("continuity_ms", C.c_double), ("pressure_ms", C.c_double),
-                ("pupdate_ms", C.c_double), ("setup_ms", C.c_double)]
+                ("pupdate_ms", C.c_double), ("setup_ms", C.c_double), ("pressure_its", C.c_int64),
+                ("warm_starts", C.c_int32), ("fallbacks", C.c_int32), ("recorded", C.c_int32),
+                ("reserved2", C.c_int32)]
 
     def as_dict(self) -> Dict[str, float]:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 _lib: Optional[C.CDLL] = None
@@ -53,6 +55,9 @@ EXPORTS = [
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
     "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
+    "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
+    "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
+    "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn",
 ]
 
 
@@ -91,6 +96,18 @@ def lib() -> C.CDLL:
     L.nodals_wall_spalding.argtypes = [C.c_int64, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_profile5.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_omega_target.argtypes = [C.c_int64, vp, vp, vp]
+    L.nodals_gnn_create_for_h8.argtypes = [vp, C.c_uint32, C.POINTER(vp)]
+    L.nodals_gnn_destroy.argtypes = [vp]
+    L.nodals_gnn_info.argtypes = [vp, vp]
+    L.nodals_gnn_get_params.argtypes = [vp, vp, C.c_int64]
+    L.nodals_gnn_set_params.argtypes = [vp, vp, C.c_int64]
+    L.nodals_gnn_reserve.argtypes = [vp, C.c_int32]
+    L.nodals_gnn_sample.argtypes = [vp, C.c_int32, vp, vp]
+    L.nodals_gnn_forward.argtypes = [vp, vp, vp]
+    L.nodals_gnn_gradient.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.nodals_gnn_train.argtypes = [vp, C.c_int32, C.c_float, C.c_uint32, C.POINTER(C.c_double)]
+    L.nodals_gnn_evaluate.argtypes = [vp, C.POINTER(C.c_double)]
+    L.nodals_h8_attach_gnn.argtypes = [vp, vp, C.c_int32, C.c_int32]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -202,6 +219,16 @@ class H8:
         ms, by = C.c_double(), C.c_double()
         _check(lib().nodals_h8_time_kernel(self._h, name.encode(), reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def gnn_graph(self):
+        """(off, nbr) of the face-adjacent cell graph the network runs on."""
+        nc = self.info["cells"]
+        off = self.get("cell_nbr_off", nc + 1, np.int32)
+        return off, self.get("cell_nbr", int(off[-1]), np.int32)
+
+    def attach_gnn(self, gnn, warm_start: bool = True, record_every: int = 0) -> None:
+        _check(lib().nodals_h8_attach_gnn(self._h, gnn._h if gnn is not None else None, 1 if warm_start else 0,
+                                          record_every))
 
     def probe_start(self):
         _check(lib().nodals_h8_probe(self._h, 1, None, None))
@@ -366,4 +393,74 @@ def wall_omega_target(inp, shared):
     out = np.empty((n, 8))
     _check(lib().nodals_wall_omega_target(n, _ptr(inp), _ptr(sh), _ptr(out)))
     return out
+
+
+# ---------------------------------------------------------------- learned warm start
+class Gnn:
+    """Message-passing pressure-correction network on an H8 context's cell graph."""
+
+    IN = 16
+
+    def __init__(self, h8: "H8", seed: int = 1):
+        self._h = C.c_void_p()
+        _check(lib().nodals_gnn_create_for_h8(h8._h, seed, C.byref(self._h)))
+        info = np.zeros(4, dtype=np.int64)
+        _check(lib().nodals_gnn_info(self._h, _ptr(info)))
+        self.n, self.rows, self.nparams = int(info[0]), int(info[1]), int(info[2])
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nodals_gnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def samples(self) -> int:
+        info = np.zeros(4, dtype=np.int64)
+        _check(lib().nodals_gnn_info(self._h, _ptr(info)))
+        return int(info[3])
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(lib().nodals_gnn_get_params(self._h, _ptr(out), self.nparams))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().nodals_gnn_set_params(self._h, _ptr(p), p.size))
+
+    def reserve(self, capacity: int) -> None:
+        _check(lib().nodals_gnn_reserve(self._h, capacity))
+
+    def sample(self, i: int):
+        x = np.empty((self.n, self.IN), dtype=np.float32)
+        t = np.empty(self.n, dtype=np.float32)
+        _check(lib().nodals_gnn_sample(self._h, i, _ptr(x), _ptr(t)))
+        return x, t
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.n, self.IN)
+        y = np.empty(self.n, dtype=np.float32)
+        _check(lib().nodals_gnn_forward(self._h, _ptr(x), _ptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(self.n, self.IN)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().nodals_gnn_gradient(self._h, _ptr(x), _ptr(t), _ptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, epochs: int, lr: float = 1e-3, seed: int = 1) -> float:
+        loss = C.c_double()
+        _check(lib().nodals_gnn_train(self._h, epochs, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def evaluate(self) -> float:
+        loss = C.c_double()
+        _check(lib().nodals_gnn_evaluate(self._h, C.byref(loss)))
+        return loss.value
 

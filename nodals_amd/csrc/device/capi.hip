@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "h8_solver.hpp"
+#include "gnn.hpp"
 #include "sst_kernels.hpp"
 #include "sys_solver.hpp"
 
@@ -16,6 +17,9 @@ struct nodals_h8 {
 };
 struct nodals_sys {
   std::unique_ptr<nodals::SysSolver> s;
+};
+struct nodals_gnn {
+  std::unique_ptr<nodals::GnnModel> g;
 };
 
 namespace {
@@ -178,6 +182,11 @@ int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res) {
     res->pressure_ms = r.pressureMs;
     res->pupdate_ms = r.pupdateMs;
     res->setup_ms = r.setupMs;
+    res->pressure_its = r.pressureIts;
+    res->warm_starts = r.warmStarts;
+    res->fallbacks = r.fallbacks;
+    res->recorded = r.recorded;
+    res->reserved2 = 0;
   });
 }
 
@@ -250,6 +259,9 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
       copyOut(s.hierarchy().agg[l].id, host, count);
     } else if (n.rfind("cf_", 0) == 0) {
       copyCf(s.hierarchy(), n, host, count);
+    } else if (n == "cell_nbr_off" || n == "cell_nbr") {
+      const nodals::CellGraph g = s.cellGraph();
+      copyOut(n == "cell_nbr" ? g.nbr : g.off, host, count);
     } else if (n == "exact_fine_val") {
       copyOut(nodals::exactFineCsr(s.setup(), s.fineTopo(), s.setup().initRau).val, host, count);
     } else if (n.rfind("P", 0) == 0 || n.rfind("A", 0) == 0) {
@@ -495,6 +507,135 @@ int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, con
 
 int nodals_wall_omega_target(int64_t n, const double* in, const int32_t* shared, double* out) {
   return guarded([&]() { nodals::omegaWallBatch(n, in, shared, out); });
+}
+
+static nodals::GnnModel& gnnOf(nodals_gnn* g) {
+  nodals::require(g != nullptr && g->g != nullptr, "null GNN handle");
+  return *g->g;
+}
+
+int nodals_gnn_create_for_h8(nodals_h8* h, uint32_t seed, nodals_gnn** out) {
+  return guarded([&]() {
+    nodals::require(out != nullptr, "null output handle");
+    *out = nullptr;
+    auto& s = solver(h);
+    nodals::require(s.hasDevice(), "H8 context was created without a device (host setup only)");
+    auto g = std::make_unique<nodals_gnn>();
+    g->g = std::make_unique<nodals::GnnModel>(s.cellGraph(), seed);
+    *out = g.release();
+  });
+}
+
+int nodals_gnn_destroy(nodals_gnn* g) {
+  return guarded([&]() { delete g; });
+}
+
+int nodals_gnn_info(nodals_gnn* g, int64_t info[4]) {
+  return guarded([&]() {
+    auto& m = gnnOf(g);
+    nodals::require(info != nullptr, "null info");
+    info[0] = m.n();
+    info[1] = m.paddedRows();
+    info[2] = m.paramCount();
+    info[3] = m.samples();
+  });
+}
+
+int nodals_gnn_get_params(nodals_gnn* g, float* host, int64_t count) {
+  return guarded([&]() {
+    auto& m = gnnOf(g);
+    nodals::require(host && count == m.paramCount(), "parameter count mismatch");
+    m.getParams(host);
+  });
+}
+
+int nodals_gnn_set_params(nodals_gnn* g, const float* host, int64_t count) {
+  return guarded([&]() {
+    auto& m = gnnOf(g);
+    nodals::require(host && count == m.paramCount(), "parameter count mismatch");
+    m.setParams(host);
+  });
+}
+
+int nodals_gnn_reserve(nodals_gnn* g, int32_t capacity) {
+  return guarded([&]() { gnnOf(g).reserveSamples(capacity); });
+}
+
+int nodals_gnn_sample(nodals_gnn* g, int32_t i, float* x, float* t) {
+  return guarded([&]() {
+    auto& m = gnnOf(g);
+    nodals::require(i >= 0 && i < m.samples() && x && t, "sample index out of range");
+    NODALS_HIP(hipStreamSynchronize(m.stream()));
+    NODALS_HIP(hipMemcpy(x, m.sampleX(i), sizeof(float) * (std::size_t)m.n() * nodals::GnnModel::kIn,
+                         hipMemcpyDeviceToHost));
+    NODALS_HIP(hipMemcpy(t, m.sampleT(i), sizeof(float) * (std::size_t)m.n(), hipMemcpyDeviceToHost));
+  });
+}
+
+namespace {
+// Host rows (n x 16) into a padded device copy.
+nodals::DBuf<float> gnnRows(nodals::GnnModel& m, const float* x, int width) {
+  nodals::DBuf<float> d;
+  d.alloc((std::size_t)m.paddedRows() * width);
+  nodals::dev::fill<float>(m.stream(), (long long)m.paddedRows() * width, d.data(), 0.0f);
+  NODALS_HIP(hipMemcpyAsync(d.data(), x, sizeof(float) * (std::size_t)m.n() * width, hipMemcpyHostToDevice,
+                            m.stream()));
+  return d;
+}
+}  // namespace
+
+int nodals_gnn_forward(nodals_gnn* g, const float* x, float* y) {
+  return guarded([&]() {
+    auto& m = gnnOf(g);
+    nodals::require(x && y, "null arrays");
+    nodals::DBuf<float> X = gnnRows(m, x, nodals::GnnModel::kIn);
+    m.forward(m.stream(), X.data(), nullptr);
+    NODALS_HIP(hipMemcpyAsync(y, m.outputDevice(), sizeof(float) * (std::size_t)m.n(), hipMemcpyDeviceToHost,
+                              m.stream()));
+    NODALS_HIP(hipStreamSynchronize(m.stream()));
+  });
+}
+
+int nodals_gnn_gradient(nodals_gnn* g, const float* x, const float* t, float* grad, double* loss) {
+  return guarded([&]() {
+    auto& m = gnnOf(g);
+    nodals::require(x && t && grad, "null arrays");
+    nodals::DBuf<float> X = gnnRows(m, x, nodals::GnnModel::kIn);
+    nodals::DBuf<float> T = gnnRows(m, t, 1);
+    nodals::DBuf<float> L;
+    L.alloc(1);
+    m.forward(m.stream(), X.data(), nullptr);
+    m.backward(m.stream(), X.data(), T.data(), L.data());
+    float lh = 0;
+    NODALS_HIP(hipMemcpyAsync(grad, m.gradDevice(), sizeof(float) * (std::size_t)m.paramCount(),
+                              hipMemcpyDeviceToHost, m.stream()));
+    NODALS_HIP(hipMemcpyAsync(&lh, L.data(), sizeof(float), hipMemcpyDeviceToHost, m.stream()));
+    NODALS_HIP(hipStreamSynchronize(m.stream()));
+    if (loss) *loss = (double)lh / m.n();
+  });
+}
+
+int nodals_gnn_train(nodals_gnn* g, int32_t epochs, float lr, uint32_t seed, double* last_loss) {
+  return guarded([&]() {
+    const double l = gnnOf(g).train(epochs, lr, seed, nullptr);
+    if (last_loss) *last_loss = l;
+  });
+}
+
+int nodals_gnn_evaluate(nodals_gnn* g, double* loss) {
+  return guarded([&]() {
+    const double l = gnnOf(g).evaluate();
+    if (loss) *loss = l;
+  });
+}
+
+int nodals_h8_attach_gnn(nodals_h8* h, nodals_gnn* g, int32_t warm_start, int32_t record_every) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    if (g && warm_start)
+      nodals::require(s.options().pressureSolver == "fgmres", "the learned warm start needs --pressure-solver fgmres");
+    s.attachGnn(g ? g->g.get() : nullptr, warm_start != 0, record_every);
+  });
 }
 
 }  // extern "C"

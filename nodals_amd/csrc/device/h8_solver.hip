@@ -7,12 +7,85 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <thread>
 
 #include "h8_kernels.hpp"
 
 namespace nodals {
 
 using namespace dev;
+
+namespace {
+
+// Partial sums of r^2 and (r/diag)^2 per block: part[2][gridDim].
+__global__ void __launch_bounds__(256) k_gnn_norms(int n, const double* __restrict__ r, const double* __restrict__ d,
+                                                   double* __restrict__ part) {
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double ri = r[i], ji = ri / d[i];
+    a += ri * ri;
+    b += ji * ji;
+  }
+  a = blockSum256(a, sh);
+  b = blockSum256(b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[gridDim.x + blockIdx.x] = b;
+  }
+}
+
+// scal[0] = 1/rms(r), scal[1] = 1/rms(r/diag) (1 where the norm is zero)
+// from the two block-partial rows.
+__global__ void __launch_bounds__(256) k_gnn_scales(int n, int parts, const double* __restrict__ part,
+                                                    double* __restrict__ scal) {
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < parts; i += 256) {
+    a += part[i];
+    b += part[parts + i];
+  }
+  a = blockSum256(a, sh);
+  b = blockSum256(b, sh);
+  if (threadIdx.x == 0) {
+    scal[0] = a > 0.0 ? sqrt((double)n / a) : 1.0;
+    scal[1] = b > 0.0 ? sqrt((double)n / b) : 1.0;
+  }
+}
+
+// Network input rows: r/rms(r), (r/diag), dp_{n-1}, dp_{n-2} scaled by
+// 1/rms(r/diag), wall distance / max, 1; the other ten columns zero.
+__global__ void k_gnn_features(int n, const double* __restrict__ r, const double* __restrict__ d,
+                               const double* __restrict__ dp1, const double* __restrict__ dp2,
+                               const double* __restrict__ wd, const double* __restrict__ scal, float* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sr = scal[0], sj = scal[1];
+  float* x = X + (long long)i * 16;
+  x[0] = (float)(r[i] * sr);
+  x[1] = (float)(r[i] / d[i] * sj);
+  x[2] = (float)(dp1[i] * sj);
+  x[3] = (float)(dp2[i] * sj);
+  x[4] = (float)wd[i];
+  x[5] = 1.0f;
+  for (int k = 6; k < 16; ++k) x[k] = 0.0f;
+}
+
+// Training target t = dp / rms(r/diag) (FP32).
+__global__ void k_gnn_target(int n, const double* __restrict__ dp, const double* __restrict__ scal,
+                             float* __restrict__ t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = (float)(dp[i] * scal[1]);
+}
+
+// Initial guess x0 = y rms(r/diag).
+__global__ void k_gnn_guess(int n, const float* __restrict__ y, const double* __restrict__ scal,
+                            double* __restrict__ x0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x0[i] = (double)y[i] / scal[1];
+}
+
+}  // namespace
 
 // ------------------------------------------------------------------ options
 H8Options parseH8Args(int argc, const char* const* argv) {
@@ -722,8 +795,100 @@ double H8Solver::pressurePrecondPower(int its, double safety) {
   return safety * est;
 }
 
-KrylovResult H8Solver::pressureSolve(const std::string& solver, double rtol, double atol, int maxit) {
+CellGraph H8Solver::cellGraph() const {
+  // Cells sharing a face.
+  CellGraph g;
+  g.n = nc_ > 0 ? nc_ : M_.nCells();
+  std::vector<std::vector<int>> adj((std::size_t)g.n);
+  for (int f = 0; f < M_.nInternal(); ++f) {
+    const int a = M_.owner[(std::size_t)f], b = M_.neighbour[(std::size_t)f];
+    adj[(std::size_t)a].push_back(b);
+    adj[(std::size_t)b].push_back(a);
+  }
+  g.off.assign((std::size_t)g.n + 1, 0);
+  for (int c = 0; c < g.n; ++c) {
+    std::sort(adj[(std::size_t)c].begin(), adj[(std::size_t)c].end());
+    g.off[(std::size_t)c + 1] = g.off[(std::size_t)c] + (int)adj[(std::size_t)c].size();
+    g.nbr.insert(g.nbr.end(), adj[(std::size_t)c].begin(), adj[(std::size_t)c].end());
+  }
+  return g;
+}
+
+void H8Solver::attachGnn(GnnModel* g, bool warmStart, int recordEvery) {
   requireDevice();
+  require(recordEvery >= 0, "record interval must be >= 0");
+  if (g) require(g->n() == nc_, "network graph size differs from the cell count");
+  gnn_ = g;
+  gnnWarm_ = g && warmStart;
+  gnnEvery_ = g ? recordEvery : 0;
+  if (!g || wallD_.size() == (std::size_t)nc_) return;
+  // Wall distance of each cell centroid: nearest wall-face centroid,
+  // normalised by the largest such distance.
+  const int wp = M_.patchIndex(opt_.wall);
+  std::vector<std::array<double, 3>> wc;
+  if (wp >= 0) {
+    const Patch& P = M_.patches[(std::size_t)wp];
+    for (int f = P.start; f < P.start + P.count; ++f) {
+      std::array<double, 3> c{0, 0, 0};
+      for (int v : M_.faces[(std::size_t)f])
+        for (int d = 0; d < 3; ++d) c[(std::size_t)d] += M_.points[(std::size_t)v][(std::size_t)d] / 3.0;
+      wc.push_back(c);
+    }
+  }
+  std::vector<double> wd((std::size_t)nc_, 1.0);
+  if (!wc.empty()) {
+    const int nt = hostThreads(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+      pool.emplace_back([&, t]() {
+        for (int c = t; c < nc_; c += nt) {
+          std::array<double, 3> x{0, 0, 0};
+          for (int v : M_.tets[(std::size_t)c])
+            for (int d = 0; d < 3; ++d) x[(std::size_t)d] += M_.points[(std::size_t)v][(std::size_t)d] * 0.25;
+          double best = 1e300;
+          for (const auto& w : wc) {
+            const double dx = x[0] - w[0], dy = x[1] - w[1], dz = x[2] - w[2];
+            best = std::min(best, dx * dx + dy * dy + dz * dz);
+          }
+          wd[(std::size_t)c] = std::sqrt(best);
+        }
+      });
+    for (auto& th : pool) th.join();
+    double mx = 0.0;
+    for (double v : wd) mx = std::max(mx, v);
+    if (mx > 0.0)
+      for (double& v : wd) v /= mx;
+  }
+  wallD_.upload(wd);
+  for (DBuf<double>* b : {&dp1_, &dp2_, &x0_}) {
+    b->alloc((std::size_t)nc_);
+    fill<double>(stream_, nc_, b->data(), 0.0);
+  }
+  gnnScal_.alloc(4);
+  gnnPart_.alloc(2 * (std::size_t)kReduceParts);
+  gnnX_.alloc((std::size_t)g->paddedRows() * GnnModel::kIn);
+  fill<float>(stream_, (long long)g->paddedRows() * GnnModel::kIn, gnnX_.data(), 0.0f);
+  sync();
+}
+
+void H8Solver::gnnFeatures(float* X) {
+  hipLaunchKernelGGL(k_gnn_norms, kReduceParts, 256, 0, stream_, nc_, cont_.data(), fdiag_.data(), gnnPart_.data());
+  NODALS_LAUNCHED();
+  hipLaunchKernelGGL(k_gnn_scales, 1, 256, 0, stream_, nc_, kReduceParts, gnnPart_.data(), gnnScal_.data());
+  NODALS_LAUNCHED();
+  hipLaunchKernelGGL(k_gnn_features, blocksFor(nc_), 256, 0, stream_, nc_, cont_.data(), fdiag_.data(), dp1_.data(),
+                     dp2_.data(), wallD_.data(), gnnScal_.data(), X);
+  NODALS_LAUNCHED();
+}
+
+KrylovResult H8Solver::pressureSolve(const std::string& solver, double rtol, double atol, int maxit) {
+  return pressureSolveFrom(solver, rtol, atol, maxit, nullptr);
+}
+
+KrylovResult H8Solver::pressureSolveFrom(const std::string& solver, double rtol, double atol, int maxit,
+                                         const double* x0) {
+  requireDevice();
+  require(!x0 || solver == "fgmres", "an initial pressure guess needs the fgmres pressure solver");
   DevOp A = [this](hipStream_t s, const double* x, double* y) { fineApply(s, x, y); };
   DevOp M = [this](hipStream_t s, const double* r, double* z) { precondApply(s, r, z); };
   KrylovResult r;
@@ -735,7 +900,7 @@ KrylovResult H8Solver::pressureSolve(const std::string& solver, double rtol, dou
     r = richardsonCheb(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_, true, 1.0, chebLambda_,
                        opt_.pressureChebLowFraction, opt_.pressureChebDegree);
   else if (solver == "fgmres")
-    r = fgmres(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, opt_.pressureRestart, kw_);
+    r = fgmres(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, opt_.pressureRestart, kw_, x0);
   else fail("unknown pressure solver " + solver);
   return r;
 }
@@ -778,6 +943,10 @@ H8RunResult H8Solver::run() {
   bool converged = false, pressureAll = true, finiteAll = true;
   double cont0 = -1.0, contRel = 1.0;
   long long sumP = 0;
+  if (gnn_) {  // correction history starts empty for every run
+    fill<double>(stream_, nc_, dp1_.data(), 0.0);
+    fill<double>(stream_, nc_, dp2_.data(), 0.0);
+  }
   std::array<double, 3> lastAudit{{NAN, NAN, NAN}};
   int finalIt = 0;
   const auto wall0 = std::chrono::steady_clock::now();
@@ -807,10 +976,39 @@ H8RunResult H8Solver::run() {
       contRel = cn / std::max(cont0, 1e-300);
       NODALS_HIP(hipEventRecord(ev[3], stream_));
       refresh();
-      const KrylovResult pr = pressureSolve(o.pressureSolver, o.pRtol, o.pAtol, o.pMax);
+      // learned warm start and sample recording
+      int slot = -1;
+      const double* x0 = nullptr;
+      if (gnn_) {
+        if (gnnEvery_ > 0 && (it - 1) % gnnEvery_ == 0) slot = gnn_->takeSlot();
+        if (slot >= 0 || gnnWarm_) {
+          float* X = slot >= 0 ? gnn_->sampleX(slot) : gnnX_.data();
+          gnnFeatures(X);
+          if (gnnWarm_) {
+            gnn_->forward(stream_, X, nullptr);
+            hipLaunchKernelGGL(k_gnn_guess, blocksFor(nc_), 256, 0, stream_, nc_, gnn_->outputDevice(),
+                               gnnScal_.data(), x0_.data());
+            NODALS_LAUNCHED();
+            x0 = x0_.data();
+          }
+        }
+      }
+      const KrylovResult pr = pressureSolveFrom(o.pressureSolver, o.pRtol, o.pAtol, o.pMax, x0);
       pressureAll = pressureAll && pr.ok;
       require(pr.ok, "H8 pressure solve failed to reach its target", kErrNotConverged);
       sumP += pr.its;
+      if (pr.warm) ++R.warmStarts;
+      if (pr.fallback) ++R.fallbacks;
+      if (gnn_) {
+        if (slot >= 0) {
+          hipLaunchKernelGGL(k_gnn_target, blocksFor(nc_), 256, 0, stream_, nc_, kw_.x.data(), gnnScal_.data(),
+                             gnn_->sampleT(slot));
+          NODALS_LAUNCHED();
+          ++R.recorded;
+        }
+        copy<double>(stream_, nc_, dp1_.data(), dp2_.data());
+        copy<double>(stream_, nc_, kw_.x.data(), dp1_.data());
+      }
       NODALS_HIP(hipEventRecord(ev[4], stream_));
       pressureUpdate(o.alphaP);
       NODALS_HIP(hipEventRecord(ev[5], stream_));
@@ -853,6 +1051,7 @@ H8RunResult H8Solver::run() {
   R.converged = converged ? 1 : 0;
   R.finalRelCont = contRel;
   R.avgPressureIts = finalIt ? (double)sumP / finalIt : 0.0;
+  R.pressureIts = sumP;
   const std::vector<double> p = getCellField("p");
   R.pressureDrop = pressureDropFit(S_, p);
   R.pass = o.runMode == "fixed10" ? (finalIt == 10 && pressureAll && finiteAll) : converged;

@@ -18,6 +18,7 @@
 #include "../host/amg_setup.hpp"
 #include "../host/h8_setup.hpp"
 #include "amg.hpp"
+#include "gnn.hpp"
 #include "krylov.hpp"
 
 namespace nodals {
@@ -70,6 +71,8 @@ struct H8RunResult {
   double loopMs = 0;
   double assemblyMs = 0, momentumMs = 0, continuityMs = 0, pressureMs = 0, pupdateMs = 0;
   double setupMs = 0;
+  long long pressureIts = 0;  // total Krylov iterations of the run
+  int warmStarts = 0, fallbacks = 0, recorded = 0;
 };
 
 class H8Solver {
@@ -91,6 +94,9 @@ class H8Solver {
   double continuity();
   void refresh();
   dev::KrylovResult pressureSolve(const std::string& solver, double rtol, double atol, int maxit);
+  // The same from an initial guess (device, cells; fgmres only).
+  dev::KrylovResult pressureSolveFrom(const std::string& solver, double rtol, double atol, int maxit,
+                                      const double* x0);
   void pressureUpdate(double alphaP);
   double rauSnapshotRel();
   H8RunResult run();
@@ -124,6 +130,13 @@ class H8Solver {
   // probeStop returns the launch count and the summed launch durations.
   void probeStart();
   void probeStop(long long* launches, double* totalMs);
+  // Learned warm start: the cell graph and normalised wall distance the
+  // network sees, and attaching a network to predict each pressure
+  // correction (warm) and/or to record (features, converged correction)
+  // samples every `recordEvery` SIMPLE iterations (0: off).  The network
+  // must outlive its attachment; attach nullptr to detach.
+  CellGraph cellGraph() const;
+  void attachGnn(GnnModel* g, bool warmStart, int recordEvery);
   double* devScratchA() { return tmpA_.data(); }
   double* devScratchB() { return tmpB_.data(); }
   void sync();
@@ -144,6 +157,9 @@ class H8Solver {
   void fineApply(hipStream_t s, const double* x, double* y);
   double pressurePrecondPower(int its, double safety);
   void probeHarvest();
+  // Features of the current pressure system into X (device, M x 16 FP32);
+  // the scaling 1/rms(r/diag) lands in gnnScal_[1].
+  void gnnFeatures(float* X);
 
   H8Options opt_;
   TetMesh M_;
@@ -184,6 +200,11 @@ class H8Solver {
   std::unique_ptr<dev::DevAmg<bf16, float>> amgb_;
   double setupMs_ = 0;
   double chebLambda_ = 1.0;
+  GnnModel* gnn_ = nullptr;
+  bool gnnWarm_ = false;
+  int gnnEvery_ = 0;
+  DBuf<double> dp1_, dp2_, wallD_, x0_, gnnScal_, gnnPart_;
+  DBuf<float> gnnX_;
   static constexpr int kProbePairs = 512;
   std::vector<hipEvent_t> probeEv_;
   int probeUsed_ = 0;

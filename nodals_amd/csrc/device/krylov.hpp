@@ -24,6 +24,8 @@ struct KrylovResult {
   double rel = 1.0;
   bool ok = false;
   double r0 = 0.0;
+  bool warm = false;      // started from a given initial guess
+  bool fallback = false;  // the guess did not reduce the residual and was dropped
 };
 
 // y = A x and z = M r on length-n FP64 device vectors.
@@ -297,8 +299,12 @@ inline KrylovResult richardsonCheb(hipStream_t s, Reducer& R, int n, const DevOp
 // j+1 of H and g[j+1..j+2], none of which the k = j+1 solution uses.
 // At the end of each cycle the host solves the k x k triangular system,
 // applies x += Z y and recomputes the true FP64 residual.
+// With an initial guess x0 the convergence target stays relative to ||b||
+// (the same rtol as from zero), and a guess whose residual is not below
+// ||b|| is dropped (the solve then starts from zero).
 inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, const DevOp& M, const double* rhs,
-                           double rtol, double atol, int maxit, int restart, KrylovWork& W) {
+                           double rtol, double atol, int maxit, int restart, KrylovWork& W,
+                           const double* x0 = nullptr) {
   require(restart >= 1 && restart <= 31, "FGMRES restart must be in [1,31]");
   W.ensure(n, restart);
   const int m = restart;
@@ -308,7 +314,6 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
   double* hs = W.fgHost;
   double* V = W.V.data();
   double* Z = W.Z.data();
-  fill<double>(s, n, W.x.data(), 0.0);
   // cycle start from r (first cycle r = rhs, x = 0)
   auto start = [&](const double* r, bool first) {
     R.dotDev(s, N, r, r, st + fg::kBeta2);
@@ -336,13 +341,42 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
   };
 
   KrylovResult out;
-  start(rhs, true);
-  NODALS_HIP(hipEventRecord(W.stepEv[32], s));
   double r0 = -1.0, target = 0.0;
   int its = 0;
   bool firstCycle = true;
   std::vector<double> y((std::size_t)m);
   double prevEst = 0.0;  // residual at the cycle start (r0, then the true residual at each restart)
+  if (x0) {
+    out.warm = true;
+    const double bn = std::sqrt(R.dot(s, N, rhs, rhs));
+    require(std::isfinite(bn), "FGMRES right-hand side non-finite", kErrNonFinite);
+    out.r0 = r0 = bn;
+    target = std::max(atol, rtol * bn);
+    copy<double>(s, n, x0, W.x.data());
+    A(s, W.x.data(), W.r.data());
+    hipLaunchKernelGGL(k_xpby<double>, nb, 256, 0, s, N, rhs, -1.0, W.r.data());
+    NODALS_LAUNCHED();
+    double rn = std::sqrt(R.dot(s, N, W.r.data(), W.r.data()));
+    if (!(rn < bn) || !std::isfinite(rn)) {
+      out.fallback = true;
+      fill<double>(s, n, W.x.data(), 0.0);
+      copy<double>(s, n, rhs, W.r.data());
+      rn = bn;
+    }
+    if (bn <= atol || rn <= target) {
+      NODALS_HIP(hipStreamSynchronize(s));
+      out.ok = true;
+      out.rel = bn > 0.0 ? rn / bn : 0.0;
+      return out;
+    }
+    start(W.r.data(), false);
+    prevEst = rn;
+    firstCycle = false;
+  } else {
+    fill<double>(s, n, W.x.data(), 0.0);
+    start(rhs, true);
+    NODALS_HIP(hipEventRecord(W.stepEv[32], s));
+  }
   while (true) {
     const int limit = std::min(m, maxit - its);
     int queued = 0, checked = -1, k = 0;

@@ -88,3 +88,30 @@ def test_sys_rejects_bad_input():
     col = np.array([0, 1], dtype=np.int32)
     rc = L.nodals_sys_create(2, row.ctypes.data, col.ctypes.data, val.ctypes.data, b"precision=fp16", C.byref(h))
     assert rc == 1 and b"precision" in L.nodals_last_error()
+
+
+def test_cell_graph_is_the_face_adjacency(mesh_dir):
+    """The learned warm start's graph: cells sharing a face, symmetric."""
+    import numpy as np
+    from nodals_amd._lib import H8
+
+    h = H8(["--mesh", mesh_dir["small"], "--max-outer", "10"], device=False)
+    off, nbr = h.gnn_graph()
+    nc = h.info["cells"]
+    ref = h.get("cell_ref", nc * 8).reshape(nc, 8)
+    pairs = set()
+    for i in range(nc):
+        for k in range(off[i], off[i + 1]):
+            pairs.add((i, int(nbr[k])))
+    assert all((j, i) in pairs for (i, j) in pairs)
+    assert np.diff(off).max() <= 4 and np.diff(off).min() >= 1
+    # every shared free face bubble (DOFs 4..7 of a cell) links exactly two cells
+    faces = {}
+    for c in range(nc):
+        for a in range(4, 8):
+            if ref[c, a] >= 0:
+                faces.setdefault(int(ref[c, a]), []).append(c)
+    for cs in faces.values():
+        if len(cs) == 2:
+            assert (cs[0], cs[1]) in pairs
+    h.close()
