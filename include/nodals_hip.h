@@ -176,8 +176,9 @@ int nodals_sys_time_spmv(nodals_sys* s, int32_t level, int32_t reps, double* ms_
  * turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/pod_balanced.inc).
  * A 4-layer message-passing network, width 64, on the graph of face-adjacent
  * cells, with BF16-MFMA forward and backward kernels (FP32 accumulation) and
- * Adam; it predicts each SIMPLE iteration's pressure correction from the
- * continuity residual, the two previous corrections and the wall distance.
+ * Adam; it predicts each SIMPLE iteration's pressure correction (from the
+ * second on) from the continuity residual, the two previous corrections and
+ * the wall distance, in units of the previous correction's RMS.
  * The FGMRES pressure solve then starts from the prediction and stops at the
  * same tolerance relative to ||b||; a prediction that does not lower the
  * residual is dropped.  Training samples are recorded from the solver's own

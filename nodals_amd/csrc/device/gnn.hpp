@@ -3,7 +3,7 @@
 // the wall distance, on the graph of face-adjacent cells (the P0 pressure
 // unknowns).
 //
-//   h0 = relu(x W0 + b0)                            x: 16 features (6 used)
+//   h0 = relu(x W0 + b0)                            x: 16 features (8 used)
 //   h_l = h_{l-1} + relu([h_{l-1} | mean_N h_{l-1}] W_l + b_l),  l = 1..4
 //   y = h4 w + b
 // Width 64, BF16 weights and GEMM inputs, FP32 accumulation and FP32
@@ -25,7 +25,7 @@ struct CellGraph {
 class GnnModel {
  public:
   static constexpr int kIn = 16, kHid = 64, kCat = 128, kLayers = 4;
-  static constexpr int kFeatures = 6;  // used input columns
+  static constexpr int kFeatures = 8;  // used input columns
 
   GnnModel(const CellGraph& g, unsigned seed);
   ~GnnModel();

@@ -17,72 +17,87 @@ using namespace dev;
 
 namespace {
 
-// Partial sums of r^2 and (r/diag)^2 per block: part[2][gridDim].
+// Partial sums of r^2, (r/diag)^2, dp1^2 and dp2^2 per block:
+// part[4][gridDim].
 __global__ void __launch_bounds__(256) k_gnn_norms(int n, const double* __restrict__ r, const double* __restrict__ d,
+                                                   const double* __restrict__ dp1, const double* __restrict__ dp2,
                                                    double* __restrict__ part) {
   __shared__ double sh[4];
-  double a = 0.0, b = 0.0;
+  double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const double ri = r[i], ji = ri / d[i];
     a += ri * ri;
     b += ji * ji;
+    c += dp1[i] * dp1[i];
+    e += dp2[i] * dp2[i];
   }
   a = blockSum256(a, sh);
   b = blockSum256(b, sh);
+  c = blockSum256(c, sh);
+  e = blockSum256(e, sh);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
     part[gridDim.x + blockIdx.x] = b;
+    part[2 * gridDim.x + blockIdx.x] = c;
+    part[3 * gridDim.x + blockIdx.x] = e;
   }
 }
 
-// scal[0] = 1/rms(r), scal[1] = 1/rms(r/diag) (1 where the norm is zero)
-// from the two block-partial rows.
+// Feature scales from the block partials:
+//   scal[0] = 1/rms(r), scal[1] = 1/rms(r/diag), scal[2] = 1/rms(dp1) (the
+//   scale of the prediction), scal[3] = log10(rms(r/diag)/rms(dp1))/4,
+//   scal[4] = log10(rms(dp2)/rms(dp1))/4 clamped to [-1, 1] (-1 without dp2).
 __global__ void __launch_bounds__(256) k_gnn_scales(int n, int parts, const double* __restrict__ part,
                                                     double* __restrict__ scal) {
   __shared__ double sh[4];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < parts; i += 256) {
-    a += part[i];
-    b += part[parts + i];
+  double q[4];
+  for (int k = 0; k < 4; ++k) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < parts; i += 256) v += part[k * parts + i];
+    q[k] = blockSum256(v, sh);
   }
-  a = blockSum256(a, sh);
-  b = blockSum256(b, sh);
   if (threadIdx.x == 0) {
-    scal[0] = a > 0.0 ? sqrt((double)n / a) : 1.0;
-    scal[1] = b > 0.0 ? sqrt((double)n / b) : 1.0;
+    const double rr = sqrt(q[0] / n), rj = sqrt(q[1] / n), r1 = sqrt(q[2] / n), r2 = sqrt(q[3] / n);
+    scal[0] = rr > 0.0 ? 1.0 / rr : 1.0;
+    scal[1] = rj > 0.0 ? 1.0 / rj : 1.0;
+    scal[2] = r1 > 0.0 ? 1.0 / r1 : scal[1];
+    scal[3] = (rj > 0.0 && r1 > 0.0) ? fmin(fmax(log10(rj / r1) / 4.0, -2.0), 2.0) : 0.0;
+    scal[4] = (r2 > 0.0 && r1 > 0.0) ? fmin(fmax(log10(r2 / r1) / 4.0, -1.0), 1.0) : -1.0;
   }
 }
 
-// Network input rows: r/rms(r), (r/diag), dp_{n-1}, dp_{n-2} scaled by
-// 1/rms(r/diag), wall distance / max, 1; the other ten columns zero.
+// Network input rows: r/rms(r), (r/diag)/rms(r/diag), dp_{n-1} and dp_{n-2}
+// over rms(dp_{n-1}), wall distance / max, 1, and the two global log-ratio
+// features; the other eight columns zero.
 __global__ void k_gnn_features(int n, const double* __restrict__ r, const double* __restrict__ d,
                                const double* __restrict__ dp1, const double* __restrict__ dp2,
                                const double* __restrict__ wd, const double* __restrict__ scal, float* __restrict__ X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double sr = scal[0], sj = scal[1];
   float* x = X + (long long)i * 16;
-  x[0] = (float)(r[i] * sr);
-  x[1] = (float)(r[i] / d[i] * sj);
-  x[2] = (float)(dp1[i] * sj);
-  x[3] = (float)(dp2[i] * sj);
+  x[0] = (float)(r[i] * scal[0]);
+  x[1] = (float)(r[i] / d[i] * scal[1]);
+  x[2] = (float)(dp1[i] * scal[2]);
+  x[3] = (float)(dp2[i] * scal[2]);
   x[4] = (float)wd[i];
   x[5] = 1.0f;
-  for (int k = 6; k < 16; ++k) x[k] = 0.0f;
+  x[6] = (float)scal[3];
+  x[7] = (float)scal[4];
+  for (int k = 8; k < 16; ++k) x[k] = 0.0f;
 }
 
-// Training target t = dp / rms(r/diag) (FP32).
+// Training target t = dp / rms(dp_{n-1}) (FP32).
 __global__ void k_gnn_target(int n, const double* __restrict__ dp, const double* __restrict__ scal,
                              float* __restrict__ t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) t[i] = (float)(dp[i] * scal[1]);
+  if (i < n) t[i] = (float)(dp[i] * scal[2]);
 }
 
-// Initial guess x0 = y rms(r/diag).
+// Initial guess x0 = y rms(dp_{n-1}).
 __global__ void k_gnn_guess(int n, const float* __restrict__ y, const double* __restrict__ scal,
                             double* __restrict__ x0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x0[i] = (double)y[i] / scal[1];
+  if (i < n) x0[i] = (double)y[i] / scal[2];
 }
 
 }  // namespace
@@ -864,15 +879,16 @@ void H8Solver::attachGnn(GnnModel* g, bool warmStart, int recordEvery) {
     b->alloc((std::size_t)nc_);
     fill<double>(stream_, nc_, b->data(), 0.0);
   }
-  gnnScal_.alloc(4);
-  gnnPart_.alloc(2 * (std::size_t)kReduceParts);
+  gnnScal_.alloc(8);
+  gnnPart_.alloc(4 * (std::size_t)kReduceParts);
   gnnX_.alloc((std::size_t)g->paddedRows() * GnnModel::kIn);
   fill<float>(stream_, (long long)g->paddedRows() * GnnModel::kIn, gnnX_.data(), 0.0f);
   sync();
 }
 
 void H8Solver::gnnFeatures(float* X) {
-  hipLaunchKernelGGL(k_gnn_norms, kReduceParts, 256, 0, stream_, nc_, cont_.data(), fdiag_.data(), gnnPart_.data());
+  hipLaunchKernelGGL(k_gnn_norms, kReduceParts, 256, 0, stream_, nc_, cont_.data(), fdiag_.data(), dp1_.data(),
+                     dp2_.data(), gnnPart_.data());
   NODALS_LAUNCHED();
   hipLaunchKernelGGL(k_gnn_scales, 1, 256, 0, stream_, nc_, kReduceParts, gnnPart_.data(), gnnScal_.data());
   NODALS_LAUNCHED();
@@ -979,8 +995,8 @@ H8RunResult H8Solver::run() {
       // learned warm start and sample recording
       int slot = -1;
       const double* x0 = nullptr;
-      if (gnn_) {
-        if (gnnEvery_ > 0 && (it - 1) % gnnEvery_ == 0) slot = gnn_->takeSlot();
+      if (gnn_ && it >= 2) {  // the network needs a previous correction
+        if (gnnEvery_ > 0 && (it - 2) % gnnEvery_ == 0) slot = gnn_->takeSlot();
         if (slot >= 0 || gnnWarm_) {
           float* X = slot >= 0 ? gnn_->sampleX(slot) : gnnX_.data();
           gnnFeatures(X);

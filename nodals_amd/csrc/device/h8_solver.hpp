@@ -158,7 +158,7 @@ class H8Solver {
   double pressurePrecondPower(int its, double safety);
   void probeHarvest();
   // Features of the current pressure system into X (device, M x 16 FP32);
-  // the scaling 1/rms(r/diag) lands in gnnScal_[1].
+  // the prediction scale 1/rms(dp_{n-1}) lands in gnnScal_[2].
   void gnnFeatures(float* X);
 
   H8Options opt_;

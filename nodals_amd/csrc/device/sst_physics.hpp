@@ -17,10 +17,29 @@
 // reference to rounding.
 #pragma once
 
+// The same functions also compile as plain host C++ (the checker builds
+// them with g++ to compare against the reference cores bit for bit).
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define NODALS_SST_FN __host__ __device__ inline
+#else
+#include <cmath>
+#define NODALS_SST_FN inline
+#endif
 
 namespace nodals {
 namespace sst {
+#if !defined(__HIPCC__)
+using std::exp;
+using std::expm1;
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::isfinite;
+using std::log;
+using std::sqrt;
+using std::tanh;
+#endif
 
 // SST-2003 model constants (modules/sst/sst_constants.inc).
 constexpr double kBetaStar = 0.09, kA1 = 0.31;
@@ -35,19 +54,19 @@ struct Local {
 
 // Wall distance of the straight-pipe case: radius - r, floored
 // (modules/sst/sst_pipe_distance.inc).
-__host__ __device__ inline double pipeDistance(double x, double y, double radius) {
+NODALS_SST_FN double pipeDistance(double x, double y, double radius) {
   const double r = sqrt(x * x + y * y);
   return fmax(radius - r, fmax(1e-10 * radius, 1e-12));
 }
 
-__host__ __device__ inline double floorDistance(double d, double radius) {
+NODALS_SST_FN double floorDistance(double d, double radius) {
   return fmax(d, fmax(1e-10 * radius, 1e-12));
 }
 
 // Local SST state at one point from the velocity gradient gu (row-major
 // du_i/dx_j), k, omega and their gradients, at wall distance d (already
 // floored).
-__host__ __device__ inline Local local(double d, double nu, const double* gu, double kRaw, double omRaw,
+NODALS_SST_FN Local local(double d, double nu, const double* gu, double kRaw, double omRaw,
                                        const double* gk, const double* gw, double kFloor, double omFloor,
                                        bool deviatoric) {
   Local q{};
@@ -107,7 +126,7 @@ constexpr double kE = 9.025013499434122;  // exp(kappa B)
 // y+ of Spalding's law at u+; the bracket e^x - 1 - x - x^2/2 - x^3/6 is
 // taken from its Taylor series x^4/24 + x^5/120 + x^6/720 + x^7/5040 when
 // |x| < 1e-3 (cancellation) and from expm1 otherwise.
-__host__ __device__ inline double yPlusOf(double up, double kap, double B) {
+NODALS_SST_FN double yPlusOf(double up, double kap, double B) {
   const double x = kap * up;
   double rem;
   if (fabs(x) < 1e-3) {
@@ -120,7 +139,7 @@ __host__ __device__ inline double yPlusOf(double up, double kap, double B) {
 }
 
 // d y+ / d u+.
-__host__ __device__ inline double dYPlusOf(double up, double kap, double B) {
+NODALS_SST_FN double dYPlusOf(double up, double kap, double B) {
   const double x = kap * up;
   double drem;
   if (fabs(x) < 1e-3) {
@@ -133,8 +152,8 @@ __host__ __device__ inline double dYPlusOf(double up, double kap, double B) {
 }
 
 // g(u+) = u+ y+(u+) - Re_y, whose root gives u+ for Re_y = y U / nu, and g'.
-__host__ __device__ inline double spaldingResidual(double up, double reY) { return up * yPlusOf(up, kKappa, kB) - reY; }
-__host__ __device__ inline double spaldingResidualDeriv(double up) {
+NODALS_SST_FN double spaldingResidual(double up, double reY) { return up * yPlusOf(up, kKappa, kB) - reY; }
+NODALS_SST_FN double spaldingResidualDeriv(double up) {
   return yPlusOf(up, kKappa, kB) + up * dYPlusOf(up, kKappa, kB);
 }
 
@@ -142,7 +161,7 @@ __host__ __device__ inline double spaldingResidualDeriv(double up) {
 // started from the viscous (sqrt Re_y) or log-law fixed-point guess, six
 // safeguarded Newton steps (a step leaving the bracket falls back to its
 // midpoint).  Returns false where the law has no valid root.
-__host__ __device__ inline bool spaldingRoot(double slip, double y, double nu, double& ut, double& yp, double& beta) {
+NODALS_SST_FN bool spaldingRoot(double slip, double y, double nu, double& ut, double& yp, double& beta) {
   const double U = fabs(slip);
   const double reY = y * U / nu;
   if (!(y > 0) || !(nu > 0) || !isfinite(U)) return false;
@@ -189,7 +208,7 @@ __host__ __device__ inline bool spaldingRoot(double slip, double y, double nu, d
 }
 
 // Mean squared Spalding residual of five samples at friction velocity ut.
-__host__ __device__ inline double profileObjective(const double* U, const double* y, double nu, double ut) {
+NODALS_SST_FN double profileObjective(const double* U, const double* y, double nu, double ut) {
   if (!(ut > 0.0) || !(nu > 0.0) || !isfinite(ut)) return 1.0e300;
   double J = 0.0;
   for (int m = 0; m < 5; ++m) {
@@ -201,7 +220,7 @@ __host__ __device__ inline double profileObjective(const double* U, const double
 }
 
 // Objective, gradient and Gauss-Newton Hessian in one pass.
-__host__ __device__ inline bool profileGN(const double* U, const double* y, double nu, double ut, double& J, double& g,
+NODALS_SST_FN bool profileGN(const double* U, const double* y, double nu, double ut, double& J, double& g,
                                           double& h) {
   if (!(nu > 0.0 && ut > 0.0) || !isfinite(ut)) return false;
   double sum = 0.0;
@@ -229,7 +248,7 @@ struct Profile {
 // (speeds U, signed tangential speeds Us, heights y increasing): Gauss-Newton
 // in a bracket set from the pointwise Spalding roots, with a halving line
 // search; a golden-section search in log(ut) if Gauss-Newton fails.
-__host__ __device__ inline Profile fitProfile5(const double* U, const double* Us, const double* y, double nu) {
+NODALS_SST_FN Profile fitProfile5(const double* U, const double* Us, const double* y, double nu) {
   Profile p{0, 0, 0, 0, 0, 0, 0};
   if (!(nu > 0.0)) return p;
   double minRoot = 1.0e300, maxRoot = 0.0, sumRoot = 0.0, Umax = 0.0;
@@ -331,7 +350,7 @@ struct OmegaWall {
 // log-layer u*/(sqrt(Cmu) kappa y) blended by exp(-Re_y / reBlend), with the
 // matching production blend and limiter; in shared mode u* is the Spalding
 // friction velocity and kappa, E those of the momentum wall law.
-__host__ __device__ inline OmegaWall omegaWall(double yd, double nu, double Ut, double uTau, double kRaw, double omRaw,
+NODALS_SST_FN OmegaWall omegaWall(double yd, double nu, double Ut, double uTau, double kRaw, double omRaw,
                                                double pk, double kFloor, double omFloor, double kappa, double E,
                                                double beta1, double reBlend, double prodLimit, bool shared) {
   OmegaWall q{};
@@ -369,3 +388,5 @@ __host__ __device__ inline OmegaWall omegaWall(double yd, double nu, double Ut, 
 
 }  // namespace sst
 }  // namespace nodals
+
+#undef NODALS_SST_FN

@@ -179,3 +179,37 @@ class RefSst:
         out = np.empty((n, 8))
         self.L.ref_of_auto(n, self._p(i), self._p(sh), self._p(out))
         return out
+
+
+PORT_SST_LIB = Path(__file__).resolve().parent / "_ref" / "libnodals_sst_port.so"
+
+
+class PortSst(RefSst):
+    """Host build of the product's pointwise SST functions with the RefSst
+    interface (distance form only for the local state)."""
+
+    def __init__(self):
+        L = C.CDLL(str(PORT_SST_LIB))
+        vp = C.c_void_p
+        L.port_sst_local.argtypes = [C.c_longlong, vp, C.c_double, C.c_double, vp, vp, vp, vp, vp, C.c_double,
+                                     C.c_double, C.c_int, vp, vp]
+        for f in ("port_spalding",):
+            getattr(L, f).argtypes = [C.c_longlong, vp, vp, C.c_double, vp, vp]
+        L.port_profile5.argtypes = [C.c_longlong, vp, vp, vp, C.c_double, vp, vp]
+        L.port_of_auto.argtypes = [C.c_longlong, vp, vp, vp]
+        self.L = L
+        L.ref_spalding = L.port_spalding
+        L.ref_profile5 = L.port_profile5
+        L.ref_of_auto = L.port_of_auto
+
+    def sst_local(self, gu, k, omega, grad_k, grad_omega, nu, radius, dist=None, pipe_xy=None, k_floor=1e-8,
+                  omega_floor=1e-3, deviatoric=False):
+        n = int(np.asarray(k).size)
+        f = lambda a, s=None: np.ascontiguousarray(a, dtype=np.float64).reshape(s or -1)
+        out = np.empty((n, 14))
+        fl = np.empty((n, 2), dtype=np.int32)
+        d = f(dist)
+        arrs = [f(gu, (n, 9)), f(k), f(omega), f(grad_k, (n, 3)), f(grad_omega, (n, 3))]
+        self.L.port_sst_local(n, self._p(d), radius, nu, *(self._p(a) for a in arrs), k_floor, omega_floor,
+                              1 if deviatoric else 0, self._p(out), self._p(fl))
+        return out, fl

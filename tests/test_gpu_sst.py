@@ -2,7 +2,10 @@
 portable cores compiled from the reference tree (oracle/_ref).  The device
 evaluates the same formulas in the same order; transcendental functions come
 from the device math library and products may fuse into FMAs, so values
-agree to 1e-11 relative (Newton roots and fits included), flags exactly."""
+agree to 1e-11 relative (Newton roots included), flags exactly; the
+five-point fit, an iterative minimisation with a relative stopping rule,
+agrees to 1e-6.  (A host build of the same device functions reproduces the
+reference fit bit for bit; the difference is the device math library.)"""
 import numpy as np
 import pytest
 
@@ -61,8 +64,12 @@ def test_profile5_matches_reference():
     assert np.array_equal(gv, rv)
     m = rv.astype(bool)
     assert m.sum() > 0.9 * m.size
-    for j in range(6):
-        assert close(got[m, j], ref[m, j], 1e-9) == 0, j
+    # The fit stops when a Gauss-Newton step changes u_tau by < 1e-10
+    # relative on an objective that is flat at its minimum, so device
+    # rounding can end it one step earlier or later: u_tau (and beta, y+)
+    # agree to 1e-6, the fit residual and u_mid tighter.
+    for j, tol in zip(range(6), (1e-6, 1e-6, 2e-6, 1e-12, 1e-6, 1e-6)):
+        assert close(got[m, j], ref[m, j], tol) == 0, j
 
 
 def test_omega_wall_target_matches_reference():

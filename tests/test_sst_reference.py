@@ -39,3 +39,27 @@ def test_sst_local_closed_forms():
     assert np.allclose(nut, 0.31 * k / np.maximum(0.31 * om, S * F2), rtol=1e-12)
     assert np.allclose(out[:, 5], F1 * 0.075 + (1 - F1) * 0.0828, rtol=1e-14)
     assert (fl[20:30, 0] == 1).all() and (fl[30:40, 1] == 1).all()
+
+
+@pytest.mark.skipif(not refbind.PORT_SST_LIB.exists(), reason="host build of the product SST functions missing")
+def test_product_sst_functions_reproduce_reference_bitwise_on_host():
+    """The pointwise functions the GPU kernels run, compiled for the host,
+    give the reference cores' results bit for bit on the seeded sets."""
+    from tests.sst_cases import omega_inputs, profile_inputs
+
+    R, P = refbind.RefSst(), refbind.PortSst()
+    c = sst_inputs()
+    for dev in (False, True):
+        a = R.sst_local(c["gu"], c["k"], c["omega"], c["grad_k"], c["grad_omega"], c["nu"], c["radius"],
+                        dist=c["dist"], deviatoric=dev)
+        b = P.sst_local(c["gu"], c["k"], c["omega"], c["grad_k"], c["grad_omega"], c["nu"], c["radius"],
+                        dist=c["dist"], deviatoric=dev)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    slip, y, nu = spalding_inputs()
+    a, b = R.spalding(slip, y, nu), P.spalding(slip, y, nu)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+    U, Us, yy, nu = profile_inputs()
+    a, b = R.profile5(U, Us, yy, nu), P.profile5(U, Us, yy, nu)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+    inp, sh = omega_inputs()
+    assert np.array_equal(R.of_auto(inp, sh), P.of_auto(inp, sh))
