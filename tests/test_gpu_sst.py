@@ -3,8 +3,8 @@ portable cores compiled from the reference tree (oracle/_ref).  The device
 evaluates the same formulas in the same order; transcendental functions come
 from the device math library and products may fuse into FMAs, so values
 agree to 1e-11 relative (Newton roots included), flags exactly; the
-five-point fit, an iterative minimisation with a relative stopping rule,
-agrees to 1e-6.  (A host build of the same device functions reproduces the
+five-point fit, a minimisation with rounding-decided branches, agrees to
+the resolution of its golden-section fallback (3e-5).  (A host build of the same device functions reproduces the
 reference fit bit for bit; the difference is the device math library.)"""
 import numpy as np
 import pytest
@@ -64,12 +64,15 @@ def test_profile5_matches_reference():
     assert np.array_equal(gv, rv)
     m = rv.astype(bool)
     assert m.sum() > 0.9 * m.size
-    # The fit stops when a Gauss-Newton step changes u_tau by < 1e-10
-    # relative on an objective that is flat at its minimum, so device
-    # rounding can end it one step earlier or later: u_tau (and beta, y+)
-    # agree to 1e-6, the fit residual and u_mid tighter.
-    for j, tol in zip(range(6), (1e-6, 1e-6, 2e-6, 1e-12, 1e-6, 1e-6)):
-        assert close(got[m, j], ref[m, j], tol) == 0, j
+    # The fit is a minimisation of an objective that is flat at its
+    # minimum: whether a Gauss-Newton step still lowers it is decided by
+    # rounding, and where it does not, the fit falls back to a 28-step
+    # golden-section search in log(u_tau) (resolution ~1e-5).  Device and
+    # host rounding can take different branches, so u_tau, y+ and beta agree
+    # to 3e-5, the fit residual to 1e-4, u_mid exactly.
+    for j, tol in zip(range(6), (3e-5, 3e-5, 6e-5, 0.0, 1e-4, 1e-4)):
+        d = np.abs(got[m, j] - ref[m, j]) / np.maximum(np.abs(ref[m, j]), 1e-300)
+        assert close(got[m, j], ref[m, j], tol) == 0, (j, float(d.max()), int((d > 1e-9).sum()))
 
 
 def test_omega_wall_target_matches_reference():
