@@ -204,10 +204,13 @@ int nodals_gnn_gradient(nodals_gnn* g, const float* x, const float* t, float* gr
  * loss of the last epoch. */
 int nodals_gnn_train(nodals_gnn* g, int32_t epochs, float lr, uint32_t seed, double* last_loss);
 int nodals_gnn_evaluate(nodals_gnn* g, double* loss);
-/* Attach g to an H8 context (NULL detaches): warm_start != 0 predicts every
- * pressure correction (needs --pressure-solver fgmres); record_every > 0
- * records a sample every that many SIMPLE iterations until the set is
- * full.  g must stay alive while attached. */
+/* Attach g to an H8 context (NULL detaches).  warm_start 1: every pressure
+ * solve (from the second SIMPLE iteration) starts from the prediction at its
+ * residual-minimising multiple (never above the zero start's residual);
+ * 2: the prediction is added as the first FGMRES direction; 0: off.  Needs
+ * --pressure-solver fgmres.  record_every > 0 records a sample every that
+ * many SIMPLE iterations until the set is full.  g must stay alive while
+ * attached. */
 int nodals_h8_attach_gnn(nodals_h8* h, nodals_gnn* g, int32_t warm_start, int32_t record_every);
 
 /* ---------------------------------------------------------------- SST ---

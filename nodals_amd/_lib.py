@@ -226,9 +226,12 @@ class H8:
         off = self.get("cell_nbr_off", nc + 1, np.int32)
         return off, self.get("cell_nbr", int(off[-1]), np.int32)
 
-    def attach_gnn(self, gnn, warm_start: bool = True, record_every: int = 0) -> None:
-        _check(lib().nodals_h8_attach_gnn(self._h, gnn._h if gnn is not None else None, 1 if warm_start else 0,
-                                          record_every))
+    def attach_gnn(self, gnn, warm_start=True, record_every: int = 0) -> None:
+        """warm_start: True or "guess" (the prediction, at its residual-optimal
+        multiple, as the initial guess), "direction" (as an added first FGMRES
+        direction), False (off)."""
+        mode = {True: 1, "guess": 1, "direction": 2, False: 0, None: 0}[warm_start]
+        _check(lib().nodals_h8_attach_gnn(self._h, gnn._h if gnn is not None else None, mode, record_every))
 
     def probe_start(self):
         _check(lib().nodals_h8_probe(self._h, 1, None, None))

@@ -634,7 +634,7 @@ int nodals_h8_attach_gnn(nodals_h8* h, nodals_gnn* g, int32_t warm_start, int32_
     auto& s = solver(h);
     if (g && warm_start)
       nodals::require(s.options().pressureSolver == "fgmres", "the learned warm start needs --pressure-solver fgmres");
-    s.attachGnn(g ? g->g.get() : nullptr, warm_start != 0, record_every);
+    s.attachGnn(g ? g->g.get() : nullptr, warm_start, record_every);
   });
 }
 

@@ -149,7 +149,8 @@ void GnnModel::forward(hipStream_t s, const float* X, float* y) {
     NODALS_LAUNCHED();
     h = H_[l].data();
   }
-  hipLaunchKernelGGL(k_decode, grid1(M_), 256, 0, s, M_, h, P_.data() + ow_, P_.data() + obout_, y_.data());
+  hipLaunchKernelGGL(k_decode, grid1(M_), 256, 0, s, M_, h, P_.data() + ow_, P_.data() + obout_, X, kSkip,
+                     y_.data());
   NODALS_LAUNCHED();
   if (y && y != y_.data()) copy<float>(s, M_, y_.data(), y);
 }

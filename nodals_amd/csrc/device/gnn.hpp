@@ -5,7 +5,7 @@
 //
 //   h0 = relu(x W0 + b0)                            x: 16 features (8 used)
 //   h_l = h_{l-1} + relu([h_{l-1} | mean_N h_{l-1}] W_l + b_l),  l = 1..4
-//   y = h4 w + b
+//   y = h4 w + b + x[2]        (x[2]: the previous correction, scaled)
 // Width 64, BF16 weights and GEMM inputs, FP32 accumulation and FP32
 // master parameters (Adam).  The same object holds the training set
 // recorded from the solver's own SIMPLE iterations.
@@ -26,6 +26,7 @@ class GnnModel {
  public:
   static constexpr int kIn = 16, kHid = 64, kCat = 128, kLayers = 4;
   static constexpr int kFeatures = 8;  // used input columns
+  static constexpr int kSkip = 2;      // input column added to the output (previous correction)
 
   GnnModel(const CellGraph& g, unsigned seed);
   ~GnnModel();

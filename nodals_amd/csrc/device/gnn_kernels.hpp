@@ -190,14 +190,16 @@ __global__ void k_relu_back(int M, int n, const float* __restrict__ dH, const fl
   dZ[t] = (row < n && Z[t] > 0.0f) ? dH[t] : 0.0f;
 }
 
-// y[i] = sum_k bf16(H[i][k]) bf16(w[k]) + b.
+// y[i] = sum_k bf16(H[i][k]) bf16(w[k]) + b + X[i][skip]: the network
+// learns a correction to one input column (the previous pressure
+// correction), so an untrained network already predicts that column.
 __global__ void k_decode(int M, const float* __restrict__ H, const float* __restrict__ w, const float* __restrict__ b,
-                         float* __restrict__ y) {
+                         const float* __restrict__ X, int skip, float* __restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   float s = 0.0f;
   for (int k = 0; k < 64; ++k) s += bfRound(H[(long long)i * 64 + k]) * bfRound(w[k]);
-  y[i] = s + b[0];
+  y[i] = s + b[0] + X[(long long)i * 16 + skip];
 }
 
 // Squared error and its gradient: dy = 2 (y - t) / n on rows < n, partial

@@ -829,12 +829,14 @@ CellGraph H8Solver::cellGraph() const {
   return g;
 }
 
-void H8Solver::attachGnn(GnnModel* g, bool warmStart, int recordEvery) {
+void H8Solver::attachGnn(GnnModel* g, int warmMode, int recordEvery) {
   requireDevice();
   require(recordEvery >= 0, "record interval must be >= 0");
+  require(warmMode >= 0 && warmMode <= 2, "warm-start mode must be 0, 1 or 2");
   if (g) require(g->n() == nc_, "network graph size differs from the cell count");
   gnn_ = g;
-  gnnWarm_ = g && warmStart;
+  gnnWarm_ = g && warmMode != 0;
+  gnnAsGuess_ = warmMode == 1;
   gnnEvery_ = g ? recordEvery : 0;
   if (!g || wallD_.size() == (std::size_t)nc_) return;
   // Wall distance of each cell centroid: nearest wall-face centroid,
@@ -898,13 +900,13 @@ void H8Solver::gnnFeatures(float* X) {
 }
 
 KrylovResult H8Solver::pressureSolve(const std::string& solver, double rtol, double atol, int maxit) {
-  return pressureSolveFrom(solver, rtol, atol, maxit, nullptr);
+  return pressureSolveFrom(solver, rtol, atol, maxit, nullptr, nullptr);
 }
 
 KrylovResult H8Solver::pressureSolveFrom(const std::string& solver, double rtol, double atol, int maxit,
-                                         const double* x0) {
+                                         const double* x0, const double* z0) {
   requireDevice();
-  require(!x0 || solver == "fgmres", "an initial pressure guess needs the fgmres pressure solver");
+  require((!x0 && !z0) || solver == "fgmres", "an initial pressure guess needs the fgmres pressure solver");
   DevOp A = [this](hipStream_t s, const double* x, double* y) { fineApply(s, x, y); };
   DevOp M = [this](hipStream_t s, const double* r, double* z) { precondApply(s, r, z); };
   KrylovResult r;
@@ -916,7 +918,7 @@ KrylovResult H8Solver::pressureSolveFrom(const std::string& solver, double rtol,
     r = richardsonCheb(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_, true, 1.0, chebLambda_,
                        opt_.pressureChebLowFraction, opt_.pressureChebDegree);
   else if (solver == "fgmres")
-    r = fgmres(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, opt_.pressureRestart, kw_, x0);
+    r = fgmres(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, opt_.pressureRestart, kw_, x0, z0);
   else fail("unknown pressure solver " + solver);
   return r;
 }
@@ -994,7 +996,7 @@ H8RunResult H8Solver::run() {
       refresh();
       // learned warm start and sample recording
       int slot = -1;
-      const double* x0 = nullptr;
+      const double* guess = nullptr;
       if (gnn_ && it >= 2) {  // the network needs a previous correction
         if (gnnEvery_ > 0 && (it - 2) % gnnEvery_ == 0) slot = gnn_->takeSlot();
         if (slot >= 0 || gnnWarm_) {
@@ -1005,11 +1007,15 @@ H8RunResult H8Solver::run() {
             hipLaunchKernelGGL(k_gnn_guess, blocksFor(nc_), 256, 0, stream_, nc_, gnn_->outputDevice(),
                                gnnScal_.data(), x0_.data());
             NODALS_LAUNCHED();
-            x0 = x0_.data();
+            guess = x0_.data();
           }
         }
       }
-      const KrylovResult pr = pressureSolveFrom(o.pressureSolver, o.pRtol, o.pAtol, o.pMax, x0);
+      // the prediction enters as the first Krylov direction (or as the
+      // initial guess when the network is attached in guess mode)
+      const KrylovResult pr = pressureSolveFrom(o.pressureSolver, o.pRtol, o.pAtol, o.pMax,
+                                                gnnAsGuess_ ? guess : nullptr, gnnAsGuess_ ? nullptr : guess);
+      if (guess && !gnnAsGuess_) ++R.warmStarts;
       pressureAll = pressureAll && pr.ok;
       require(pr.ok, "H8 pressure solve failed to reach its target", kErrNotConverged);
       sumP += pr.its;

@@ -95,8 +95,9 @@ class H8Solver {
   void refresh();
   dev::KrylovResult pressureSolve(const std::string& solver, double rtol, double atol, int maxit);
   // The same from an initial guess (device, cells; fgmres only).
+  // x0: initial guess; z0: first Krylov direction (fgmres only).
   dev::KrylovResult pressureSolveFrom(const std::string& solver, double rtol, double atol, int maxit,
-                                      const double* x0);
+                                      const double* x0, const double* z0);
   void pressureUpdate(double alphaP);
   double rauSnapshotRel();
   H8RunResult run();
@@ -136,7 +137,9 @@ class H8Solver {
   // samples every `recordEvery` SIMPLE iterations (0: off).  The network
   // must outlive its attachment; attach nullptr to detach.
   CellGraph cellGraph() const;
-  void attachGnn(GnnModel* g, bool warmStart, int recordEvery);
+  // mode 1: prediction as the initial guess at its residual-optimal
+  // multiple; mode 2: as an added first Krylov direction.
+  void attachGnn(GnnModel* g, int warmMode, int recordEvery);
   double* devScratchA() { return tmpA_.data(); }
   double* devScratchB() { return tmpB_.data(); }
   void sync();
@@ -201,7 +204,7 @@ class H8Solver {
   double setupMs_ = 0;
   double chebLambda_ = 1.0;
   GnnModel* gnn_ = nullptr;
-  bool gnnWarm_ = false;
+  bool gnnWarm_ = false, gnnAsGuess_ = false;
   int gnnEvery_ = 0;
   DBuf<double> dp1_, dp2_, wallD_, x0_, gnnScal_, gnnPart_;
   DBuf<float> gnnX_;

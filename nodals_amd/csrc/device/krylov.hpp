@@ -302,9 +302,16 @@ inline KrylovResult richardsonCheb(hipStream_t s, Reducer& R, int n, const DevOp
 // With an initial guess x0 the convergence target stays relative to ||b||
 // (the same rtol as from zero), and a guess whose residual is not below
 // ||b|| is dropped (the solve then starts from zero).
+// The initial guess is used at its residual-minimising multiple
+// alpha = (b, A x0) / (A x0, A x0), so it never raises the residual; a guess
+// with alpha <= 0 is dropped.
+// With a first direction z0, the first cycle's Arnoldi steps take z0, then
+// M^-1 v_0, M^-1 v_1, ... (flexible GMRES admits any directions): the
+// space is the plain one augmented by z0, whose optimal multiple the
+// least-squares update picks; a poor z0 costs one step.
 inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, const DevOp& M, const double* rhs,
                            double rtol, double atol, int maxit, int restart, KrylovWork& W,
-                           const double* x0 = nullptr) {
+                           const double* x0 = nullptr, const double* z0 = nullptr) {
   require(restart >= 1 && restart <= 31, "FGMRES restart must be in [1,31]");
   W.ensure(n, restart);
   const int m = restart;
@@ -322,11 +329,17 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     hipLaunchKernelGGL(k_scale_dev, nb, 256, 0, s, N, st + fg::kScal, r, V);
     NODALS_LAUNCHED();
   };
+  bool useZ0 = z0 != nullptr;
   auto step = [&](int j) {
     double* vj = V + (long long)j * N;
     double* zj = Z + (long long)j * N;
     double* w = V + (long long)(j + 1) * N;
-    M(s, vj, zj);
+    if (useZ0) {
+      if (j == 0) copy<double>(s, n, z0, zj);
+      else M(s, vj - N, zj);  // lag the preconditioned basis by one
+    } else {
+      M(s, vj, zj);
+    }
     A(s, zj, w);
     // classical Gram-Schmidt twice; pass 2's dots share a kernel with pass 1's
     // update, and pass 2's update shares one with the normalisation
@@ -352,16 +365,22 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     require(std::isfinite(bn), "FGMRES right-hand side non-finite", kErrNonFinite);
     out.r0 = r0 = bn;
     target = std::max(atol, rtol * bn);
-    copy<double>(s, n, x0, W.x.data());
-    A(s, W.x.data(), W.r.data());
-    hipLaunchKernelGGL(k_xpby<double>, nb, 256, 0, s, N, rhs, -1.0, W.r.data());
-    NODALS_LAUNCHED();
-    double rn = std::sqrt(R.dot(s, N, W.r.data(), W.r.data()));
-    if (!(rn < bn) || !std::isfinite(rn)) {
+    // w = A x0; (b, w) and (w, w) in one pass
+    A(s, x0, W.q.data());
+    double bw[2];
+    R.multidot(s, N, 1, rhs, W.q.data(), bw, true);
+    const double alpha = bw[1] > 0.0 ? bw[0] / bw[1] : 0.0;
+    double rn = bn;
+    if (alpha > 0.0 && std::isfinite(alpha)) {
+      hipLaunchKernelGGL(k_scale_into<double>, nb, 256, 0, s, N, alpha, x0, W.x.data());
+      NODALS_LAUNCHED();
+      copy<double>(s, n, rhs, W.r.data());
+      axpy<double>(s, n, -alpha, W.q.data(), W.r.data());
+      rn = std::sqrt(std::max(bn * bn - bw[0] * alpha, 0.0));
+    } else {
       out.fallback = true;
       fill<double>(s, n, W.x.data(), 0.0);
       copy<double>(s, n, rhs, W.r.data());
-      rn = bn;
     }
     if (bn <= atol || rn <= target) {
       NODALS_HIP(hipStreamSynchronize(s));

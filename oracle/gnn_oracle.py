@@ -11,7 +11,7 @@ so there is nothing of the reference to follow: this restates the network
 the product defines,
     h0 = relu(bf(x) bf(W0) + b0)
     h_l = h_{l-1} + relu(bf([h_{l-1} | mean_N h_{l-1}]) bf(W_l) + b_l)
-    y = bf(h4) bf(w) + b,   loss = mean((y - t)^2)
+    y = bf(h4) bf(w) + b + x[:, 2],   loss = mean((y - t)^2)
 with bf() rounding to BF16 where the device feeds its matrix cores, and its
 backward pass (bf() applied to both GEMM operands there as well).  Pinned by
 central finite differences of the loss on a small graph with rounding
@@ -25,6 +25,7 @@ from typing import Dict, List, Tuple
 import numpy as np
 
 IN, HID, CAT, LAYERS = 16, 64, 128, 4
+SKIP = 2  # input column added to the output
 
 
 def bf16(a: np.ndarray) -> np.ndarray:
@@ -93,7 +94,7 @@ def forward(p: np.ndarray, g: Graph, x: np.ndarray, rounding: bool = True):
         Z = r(Xl) @ r(P[f"W{l + 1}"]) + P[f"b{l + 1}"]
         H = H + np.maximum(Z, 0.0)
         cache[f"X{l + 1}"], cache[f"Z{l + 1}"], cache[f"H{l + 1}"] = Xl, Z, H
-    y = r(H) @ r(P["w"]) + P["b"][0]
+    y = r(H) @ r(P["w"]) + P["b"][0] + np.asarray(x, dtype=np.float64)[:, SKIP]
     return y, cache
 
 
