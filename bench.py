@@ -20,6 +20,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -118,6 +120,48 @@ def traffic_per_launch() -> dict | None:
     return vals
 
 
+def learned_leg(mesh: str, steps: int, epochs: int) -> dict:
+    """Learned warm start + FP32 AMG on the same pipe: samples recorded from
+    the solver's own converged runs at Re 10, 15, 30, 45 (every 4th SIMPLE
+    iteration), trained on the GPU, then the timed Re = 20 solves start each
+    pressure solve from the network's prediction (its residual-optimal
+    multiple).  Recording and training are reported, not timed."""
+    from nodals_amd._lib import Gnn, H8
+
+    h = H8(solver_args(mesh, "fp32"))
+    g = Gnn(h, seed=1)
+    g.reserve(3000)
+    t0 = time.perf_counter()
+    for re in (10.0, 15.0, 30.0, 45.0):
+        hr = H8(solver_args(mesh, "fp32") + ["--re", str(re)])
+        hr.attach_gnn(g, warm_start=False, record_every=4)
+        hr.run()
+        hr.attach_gnn(None)
+        hr.close()
+    t1 = time.perf_counter()
+    l0 = g.evaluate()
+    last = g.train(epochs=epochs, lr=1e-3, seed=2)
+    t2 = time.perf_counter()
+    h.attach_gnn(g, warm_start="guess")
+    times, runs = [], []
+    for _ in range(max(1, steps)):
+        reset(h)
+        ts = time.perf_counter()
+        r = h.run()
+        times.append(time.perf_counter() - ts)
+        runs.append(r)
+        print(f"[bench] learned+fp32 step: {times[-1]:.3f} s, outer={r['outer']} pressure its={r['pressure_its']} "
+              f"guesses dropped={r['fallbacks']}/{r['warm_starts']}", file=sys.stderr, flush=True)
+    h.attach_gnn(None)
+    out = {"s": float(np.mean(times)), "outer": runs[-1]["outer"], "pressure_its": runs[-1]["pressure_its"],
+           "guesses": runs[-1]["warm_starts"], "guesses_dropped": runs[-1]["fallbacks"],
+           "pressure_drop": runs[-1]["pressure_drop"], "samples": g.samples, "record_s": t1 - t0,
+           "train_s": t2 - t1, "epochs": epochs, "loss_before": l0, "loss_after": last}
+    g.close()
+    h.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +171,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=500)
     ap.add_argument("--no-poisson", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only the fine SpMV and the copy calibration kernel (for counter runs)")
     a = ap.parse_args()
@@ -198,6 +244,10 @@ def main() -> int:
     dp32 = fp32["runs"][-1]["pressure_drop"]
     dp64 = fp64["runs"][-1]["pressure_drop"]
     poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
+    learned = None if a.no_learned else learned_leg(mesh, a.steps, a.learned_epochs)
+    if learned:
+        learned["pressure_its_fp32_amg"] = int(fp32["runs"][-1]["pressure_its"])
+        learned["speedup_vs_fp64"] = float(np.mean(fp64["times"])) / learned["s"]
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(mesh, int(fp64["runs"][-1]["outer"]), a.cpu_iters)
@@ -232,6 +282,7 @@ def main() -> int:
                 for k in ("assembly", "momentum", "continuity", "pressure", "pupdate")},
             "kernels": extra,
             "poisson_64": poisson,
+            "learned_fp32_amg": learned,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
