@@ -120,6 +120,33 @@ def traffic_per_launch() -> dict | None:
     return vals
 
 
+def cavity_leg(steps: int) -> dict:
+    """Lid-driven cavity (unit cube, 16^3 hexes x 6 Kuhn tets = 24,576 tets,
+    lid u = (1, 0, 0), Re = 100), plain FP64, SIMPLE to 1e-6."""
+    from nodals_amd import meshgen
+    from nodals_amd._lib import H8
+
+    d = ROOT / "_meshes" / "cavity_16"
+    if not (d / "faces").exists():
+        meshgen.write_cavity_case(d, n=16)
+    h = H8(["--mesh", str(d), "--lid", "lid", "--wall", "walls", "--re", "100", "--run-mode", "converge",
+            "--max-outer", "20000", "--simple-tol", "1e-6", "--alpha-u", "0.5", "--alpha-p", "0.5",
+            "--amg-smoother", "jacobi", "--amg-jacobi-omega", "0.7", "--pressure-solver", "fgmres", "--p-rtol", "0.1",
+            "--p-max", "200", "--amg-precision", "fp64", "--verbose", "0"])
+    times, r = [], None
+    for _ in range(max(1, steps)):
+        reset(h)
+        t0 = time.perf_counter()
+        r = h.run()
+        times.append(time.perf_counter() - t0)
+    print(f"[bench] cavity 16^3 fp64: {np.mean(times):.3f} s, outer={r['outer']} converged={r['converged']}",
+          file=sys.stderr, flush=True)
+    out = {"s": float(np.mean(times)), "outer": r["outer"], "converged": r["converged"], "cells": h.info["cells"],
+           "avg_pressure_its": r["avg_pressure_its"]}
+    h.close()
+    return out
+
+
 def learned_leg(mesh: str, steps: int, epochs: int) -> dict:
     """Learned warm start + FP32 AMG on the same pipe: samples recorded from
     the solver's own converged runs at Re 10, 15, 30, 45 (every 4th SIMPLE
@@ -172,6 +199,7 @@ def main() -> int:
     ap.add_argument("--spmv-reps", type=int, default=500)
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-cavity", action="store_true")
     ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only the fine SpMV and the copy calibration kernel (for counter runs)")
@@ -245,6 +273,7 @@ def main() -> int:
     dp64 = fp64["runs"][-1]["pressure_drop"]
     poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
     learned = None if a.no_learned else learned_leg(mesh, a.steps, a.learned_epochs)
+    cavity = None if a.no_cavity else cavity_leg(1)
     if learned:
         learned["pressure_its_fp32_amg"] = int(fp32["runs"][-1]["pressure_its"])
         learned["speedup_vs_fp64"] = float(np.mean(fp64["times"])) / learned["s"]
@@ -283,6 +312,7 @@ def main() -> int:
             "kernels": extra,
             "poisson_64": poisson,
             "learned_fp32_amg": learned,
+            "cavity_16_fp64": cavity,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

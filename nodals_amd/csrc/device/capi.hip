@@ -259,6 +259,8 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
       copyOut(s.hierarchy().agg[l].id, host, count);
     } else if (n.rfind("cf_", 0) == 0) {
       copyCf(s.hierarchy(), n, host, count);
+    } else if (n == "pin_cell") {
+      copyOut(std::vector<int32_t>{s.setup().pinCell}, host, count);
     } else if (n == "cell_nbr_off" || n == "cell_nbr") {
       const nodals::CellGraph g = s.cellGraph();
       copyOut(n == "cell_nbr" ? g.nbr : g.off, host, count);

@@ -247,11 +247,13 @@ __global__ void __launch_bounds__(256) k_mom_norms(int nLong, const int* __restr
 
 // Continuity residual r_c = fixedDiv_c + sum_a B[c][a] . u(ref a)
 // (h8_continuity_kernel, h8_precomputed_b.cuh:140-163), written negated as
-// the pressure right-hand side, with block partials of r^2.
+// the pressure right-hand side, with block partials of r^2.  A pinned cell
+// (pin >= 0, closed domains) gets a zero right-hand side; its residual still
+// counts in the norm.
 static __global__ void __launch_bounds__(256) k_continuity(int nc, const int* __restrict__ cref,
                                                     const double* __restrict__ bcoef,
                                                     const double* __restrict__ fixedDiv, const double* __restrict__ u,
-                                                    double* __restrict__ negCont, double* __restrict__ part) {
+                                                    int pin, double* __restrict__ negCont, double* __restrict__ part) {
   __shared__ double sh[4];
   double acc = 0.0;
   for (int c = blockIdx.x * 256 + threadIdx.x; c < nc; c += gridDim.x * 256) {
@@ -263,7 +265,7 @@ static __global__ void __launch_bounds__(256) k_continuity(int nc, const int* __
       if (g < 0) continue;
       s += B[3 * a] * u[3ll * g] + B[3 * a + 1] * u[3ll * g + 1] + B[3 * a + 2] * u[3ll * g + 2];
     }
-    negCont[c] = -s;
+    negCont[c] = c == pin ? 0.0 : -s;
     acc += s * s;
   }
   acc = blockSum256(acc, sh);
@@ -274,13 +276,16 @@ static __global__ void __launch_bounds__(256) k_continuity(int nc, const int* __
 //   val[k] = sum_{shared free DOF g} rAU_g * (B[c][a] . B[c2][b])
 // (h8_fine_csr_refresh_warp_kernel, h8_precomputed_b.cuh:37-91, restated as
 // a gather over the terms recorded at setup; no atomics).  Also writes the
-// diagonal, optional lower-precision copies and the row-L1 sums.
+// diagonal, optional lower-precision copies and the row-L1 sums.  With a
+// pinned cell (pin >= 0) the off-diagonal entries of its row and column are
+// zero.
 template <int L, bool F32, bool B16, bool L1>
 __global__ void __launch_bounds__(256) k_fine_refresh(
     int nc, const int* __restrict__ frow, const int* __restrict__ fcol, const int* __restrict__ fdiagPos,
     const int* __restrict__ nzOff, const unsigned* __restrict__ nzTerm, const double* __restrict__ bcoef,
-    const double* __restrict__ rau, double* __restrict__ val, float* __restrict__ valf, bf16* __restrict__ valb,
-    double* __restrict__ diag, float* __restrict__ diagf, double* __restrict__ l1d, float* __restrict__ l1f) {
+    const double* __restrict__ rau, int pin, double* __restrict__ val, float* __restrict__ valf,
+    bf16* __restrict__ valb, double* __restrict__ diag, float* __restrict__ diagf, double* __restrict__ l1d,
+    float* __restrict__ l1f) {
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int c = gid / L;
   const int lane = gid % L;
@@ -298,6 +303,7 @@ __global__ void __launch_bounds__(256) k_fine_refresh(
       const double dot = Bc[3 * a] * Bn[3 * b] + Bc[3 * a + 1] * Bn[3 * b + 1] + Bc[3 * a + 2] * Bn[3 * b + 2];
       v += rau[g] * dot;
     }
+    if ((c == pin || c2 == pin) && c != c2) v = 0.0;
     val[k] = v;
     if (F32) valf[k] = (float)v;
     if (B16) valb[k] = floatToBf16((float)v);

@@ -118,6 +118,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--wall") o.wall = next();
     else if (a == "--inlet") o.inlet = next();
     else if (a == "--outlet") o.outlet = next();
+    else if (a == "--lid") o.lid = next();
     else if (a == "--re") o.re = num(next());
     else if (a == "--bulk") o.bulk = num(next());
     else if (a == "--mixing-length") o.mixingLength = inum(next()) != 0;
@@ -250,6 +251,7 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   prm.wall = opt_.wall;
   prm.inlet = opt_.inlet;
   prm.outlet = opt_.outlet;
+  prm.lid = opt_.lid;
   prm.re = opt_.re;
   prm.bulk = opt_.bulk;
   prm.alphaU = opt_.alphaU;
@@ -265,7 +267,27 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   ao.powerIts = powerIts;
   ao.safety = opt_.amgLambdaSafety;
   ao.lowFraction = opt_.amgLambdaLowFraction;
-  if (opt_.amgHierarchy == "cf") {
+  if (S_.pinCell >= 0) {
+    // Closed domain: the Schur operator is singular (constants); the pinned
+    // exact fine CSR (row and column of the pinned cell decoupled) seeds the
+    // explicit-operator hierarchy.
+    CsrD A0 = exactFineCsr(S_, F_, S_.initRau);
+    pinCsr(A0, S_.pinCell);
+    if (opt_.amgHierarchy == "cf") {
+      CfOptions co;
+      co.theta = opt_.cfTheta;
+      co.pmax = opt_.cfPmax;
+      co.interp = opt_.cfInterp;
+      co.aggressiveFirst = opt_.cfAggressiveFirst != 0;
+      co.terminal = opt_.amgTerminal;
+      co.powerIts = powerIts;
+      co.safety = opt_.amgLambdaSafety;
+      H_ = buildCfHierarchyCsr(A0, co);
+    } else {
+      ao.powerIts = powerIts > 0 ? powerIts : ao.powerIts;
+      H_ = buildSaHierarchyCsr(A0, ao);
+    }
+  } else if (opt_.amgHierarchy == "cf") {
     CfOptions co;
     co.theta = opt_.cfTheta;
     co.pmax = opt_.cfPmax;
@@ -670,7 +692,7 @@ double H8Solver::continuity() {
   requireDevice();
   double* part = red_.partials();
   hipLaunchKernelGGL(k_continuity, kReduceParts, 256, 0, stream_, nc_, cref_.data(), bcoef_.data(), fixedDiv_.data(),
-                     u_.data(), cont_.data(), part);
+                     u_.data(), S_.pinCell, cont_.data(), part);
   NODALS_LAUNCHED();
   hipLaunchKernelGGL(k_reduce_final, 1, 256, 0, stream_, kReduceParts, part, red_.hostOut());
   NODALS_LAUNCHED();
@@ -685,8 +707,9 @@ void H8Solver::refresh() {
   const bool l1 = opt_.amgSmoother == "l1jacobi";
 #define NODALS_REFRESH(L, F, B, LL)                                                                                  \
   hipLaunchKernelGGL((k_fine_refresh<L, F, B, LL>), grid, 256, 0, stream_, nc_, frow_.data(), fcol_.data(),         \
-                     fdiagPos_.data(), nzOff_.data(), nzTerm_.data(), bcoef_.data(), rau_.data(), fval_.data(),     \
-                     fvalf_.data(), fvalb_.data(), fdiag_.data(), fdiagf_.data(), fl1_.data(), fl1f_.data())
+                     fdiagPos_.data(), nzOff_.data(), nzTerm_.data(), bcoef_.data(), rau_.data(), S_.pinCell,       \
+                     fval_.data(), fvalf_.data(), fvalb_.data(), fdiag_.data(), fdiagf_.data(), fl1_.data(),        \
+                     fl1f_.data())
 #define NODALS_REFRESH_P(L)                         \
   if (f32) {                                        \
     if (l1) NODALS_REFRESH(L, true, false, true);   \
@@ -1206,7 +1229,7 @@ double H8Solver::timeKernel(const std::string& kernel, int reps, double* bytes) 
   } else if (kernel == "continuity") {
     launch = [this]() {
       hipLaunchKernelGGL(k_continuity, kReduceParts, 256, 0, stream_, nc_, cref_.data(), bcoef_.data(),
-                         fixedDiv_.data(), u_.data(), cont_.data(), red_.partials());
+                         fixedDiv_.data(), u_.data(), S_.pinCell, cont_.data(), red_.partials());
       NODALS_LAUNCHED();
     };
     by = b.continuity;

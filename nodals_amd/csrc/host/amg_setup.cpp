@@ -556,6 +556,15 @@ CsrD exactFineCsr(const H8Setup& S, const FineTopo& F, const std::vector<double>
   return A;
 }
 
+void pinCsr(CsrD& A, int cell) {
+  require(cell >= 0 && cell < A.n, "pinned cell out of range");
+  for (int i = 0; i < A.n; ++i)
+    for (std::int64_t k = A.row[(std::size_t)i]; k < A.row[(std::size_t)i + 1]; ++k) {
+      const int j = A.col[(std::size_t)k];
+      if (i != j && (i == cell || j == cell)) A.val[(std::size_t)k] = 0.0;
+    }
+}
+
 std::uint64_t cfHash64(std::uint64_t x) {
   // SplitMix64 output function (Steele, Lea and Flood, OOPSLA 2014): the
   // state advanced by the golden-ratio increment, then two xor-shift-multiply

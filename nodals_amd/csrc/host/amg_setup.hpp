@@ -82,6 +82,10 @@ CsrD galerkinProduct(const CsrD& A, const Transfer& P);
 // order (g8_cf_hierarchy_host.hpp:116-166).
 CsrD exactFineCsr(const H8Setup& S, const FineTopo& F, const std::vector<double>& rau);
 
+// Zero the off-diagonal entries of row and column `cell` (pressure pin of a
+// closed domain); the pattern and the diagonal are kept.
+void pinCsr(CsrD& A, int cell);
+
 // CF/PMIS hierarchy of the fine Schur operator (g8_cf_hierarchy_host.hpp:
 // 1584-1783); of an explicit CSR operator with buildCfHierarchyCsr.
 AmgHierarchy buildCfHierarchy(const H8Setup& S, const FineTopo& F, const std::vector<double>& rau,

@@ -108,6 +108,31 @@ double triangleAvgUz(const PipeInfo& P, const V3 X[3]) {
   return 2.0 * P.bulk * (1.0 - (ax + ay) / (P.R * P.R));
 }
 
+PipeInfo makeCavity(const TetMesh& M, const H8Params& prm) {
+  PipeInfo P;
+  P.wall = M.patchIndex(prm.wall);
+  P.inlet = M.patchIndex(prm.lid);  // the lid carries the driving velocity
+  P.outlet = -1;
+  P.re = prm.re;
+  P.bulk = prm.bulk;
+  require(P.inlet >= 0, "lid patch not found (lid=" + prm.lid + ")");
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (const auto& x : M.points)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], x[(std::size_t)d]);
+      hi[d] = std::max(hi[d], x[(std::size_t)d]);
+    }
+  P.L = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+  P.zIn = lo[2];
+  P.zOut = hi[2];
+  require(P.L > 0 && prm.re > 0 && prm.bulk > 0, "invalid cavity geometry");
+  P.nu = prm.bulk * P.L / prm.re;
+  P.inletVelocity[0] = prm.bulk;
+  P.inletVelocity[1] = 0.0;
+  P.inletVelocity[2] = 0.0;
+  return P;
+}
+
 PipeInfo makePipe(const TetMesh& M, const H8Params& prm) {
   PipeInfo P;
   P.wall = M.patchIndex(prm.wall);
@@ -332,7 +357,9 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
   require(prm.alphaU > 0.0 && prm.alphaU <= 1.0, "alpha-u must be in (0,1]");
   const int nth = hostThreads(prm.threads);
   H8Setup S;
-  S.pipe = makePipe(M, prm);
+  S.cavity = !prm.lid.empty();
+  S.pipe = S.cavity ? makeCavity(M, prm) : makePipe(M, prm);
+  S.pinCell = S.cavity ? 0 : -1;
   const int np = M.nPoints(), nf = M.nFaces(), ni = M.nInternal(), nc = M.nCells();
   S.nc = nc;
   S.nPoints = np;
@@ -376,13 +403,27 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
     if (p == S.pipe.inlet)
       for (auto v : M.faces[(std::size_t)f]) onInlet[(std::size_t)v] = 1;
   }
-  for (int v = 0; v < np; ++v)
+  if (S.cavity) {
+    // Every lid vertex (its edges included) moves with the lid; each
+    // boundary face bubble takes (20/9)(exact face average - vertex mean)
+    // so that the P1+BF3 trace has the exact average on every face.
+    for (int v = 0; v < np; ++v)
+      if (onInlet[(std::size_t)v]) S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)v]][0] = S.pipe.inletVelocity[0];
+    for (int f = ni; f < nf; ++f) {
+      const auto& F = M.faces[(std::size_t)f];
+      const double exact = M.facePatch[(std::size_t)f] == S.pipe.inlet ? S.pipe.inletVelocity[0] : 0.0;
+      double vm = 0;
+      for (int j = 0; j < 3; ++j) vm += S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)F[(std::size_t)j]]][0] / 3.0;
+      S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)np + f]][0] = (20.0 / 9.0) * (exact - vm);
+    }
+  }
+  for (int v = 0; v < np && !S.cavity; ++v)
     if (onInlet[(std::size_t)v] && !onWall[(std::size_t)v]) {
       const int s = S.fixedSlot[(std::size_t)v];
       S.fixedValue[(std::size_t)s][2] =
           S.pipe.profileScale * idealUz(S.pipe, M.points[(std::size_t)v][0], M.points[(std::size_t)v][1]);
     }
-  {
+  if (!S.cavity) {
     const Patch& pin = M.patches[(std::size_t)S.pipe.inlet];
     for (int f = pin.start; f < pin.start + pin.count; ++f) {
       const auto& F = M.faces[(std::size_t)f];

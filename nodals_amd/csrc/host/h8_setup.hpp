@@ -75,6 +75,10 @@ struct FineTopo {
 
 struct H8Params {
   std::string wall = "patch_0_0", inlet = "patch_2_0", outlet = "patch_1_0";
+  // Lid-driven cavity instead of the pipe when non-empty: every boundary
+  // face is Dirichlet, the lid patch moves at (bulk, 0, 0), nu = bulk L / re
+  // with L the largest box extent, and the pressure is pinned in cell 0.
+  std::string lid;
   double re = 20.0, bulk = 1.0;
   double alphaU = 0.5;
   double rauScale = 1.0;
@@ -83,6 +87,8 @@ struct H8Params {
 
 struct H8Setup {
   PipeInfo pipe;
+  bool cavity = false;
+  int pinCell = -1;  // pressure row pinned (cavity), -1 for none
   int nc = 0, nv = 0, nFixed = 0;
   int nPoints = 0, nFaces = 0;
   int freeVertices = 0, freeFaces = 0, fixedVertices = 0, fixedFaces = 0;

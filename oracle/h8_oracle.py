@@ -115,6 +115,7 @@ class H8Case:
     init_rau: np.ndarray      # (nv,)
     cell_z: Optional[np.ndarray] = None
     zspan: float = 1.0
+    pin: int = -1             # pressure row pinned in a closed domain (-1: none)
     hier: Optional[dict] = None
     _fine: Optional[dict] = field(default=None, repr=False)
 
@@ -151,6 +152,7 @@ def case_from_product(h, with_hierarchy: bool = True) -> H8Case:
     )
     pipe = h.get("pipe", 5)
     c.zspan = float(pipe[4])
+    c.pin = int(h.get("pin_cell", 1)[0])
     if with_hierarchy:
         c.hier = hierarchy_from_product(h)
     return c
@@ -379,6 +381,12 @@ def fine_refresh(case: H8Case, rau: np.ndarray) -> Tuple[sp.csr_matrix, np.ndarr
     dot = np.einsum("nd,nd->n", case.bcoef[rc, ra], case.bcoef[c2, b])
     pos = T["row"][rc] + T["slot"]
     np.add.at(val, pos, rau[g] * dot)
+    if case.pin >= 0:
+        # closed domain: decouple the pinned cell's row and column, keep its
+        # diagonal (the product's pressure pin)
+        rows = np.repeat(np.arange(case.nc), np.diff(T["row"]))
+        off = (rows != T["col"]) & ((rows == case.pin) | (T["col"] == case.pin))
+        val[off] = 0.0
     A = sp.csr_matrix((val, T["col"], T["row"]), shape=(case.nc, case.nc))
     return A, val[T["diag_pos"]].copy()
 
@@ -660,6 +668,8 @@ def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work=
         Af, fdiag = fine_refresh(case, rau)
         vc.set_fine(Af, fdiag)
         rhs = -r
+        if case.pin >= 0:
+            rhs[case.pin] = 0.0
         if p_solver == "pcg":
             dp, pr = pcg(Af, vc.apply, rhs, p_rtol, p_atol, p_max)
         elif p_solver == "richardson":
