@@ -170,6 +170,19 @@ int nodals_sys_set_rhs(nodals_sys* s, const double* b);
  * algorithmic bytes of one launch. */
 int nodals_sys_time_spmv(nodals_sys* s, int32_t level, int32_t reps, double* ms_avg, double* bytes);
 
+/* k-omega SST volume terms of every cell of an H8 context's mesh (the
+ * reference kernel sst_tet_volume, turbulence/sst_gpu_v2/source/
+ * puretet_profile5_eq5_hxt5b/modules/transport/tet_volume_kernel.inc:1-30,
+ * with volume_coefficients, modules/transport/portable/transport_core.hpp:
+ * 14-25): Keast 24-point rule, SST-2003 local state at each point with the
+ * pipe wall distance, from the context's current velocity and the given
+ * k, omega (k_free/omega_free in reference free-DOF order, k_fixed/
+ * omega_fixed per fixed slot).  equation 0: k, 1: omega.  mat[cells][64]
+ * (local a, b), rhs[cells][8]. */
+int nodals_h8_sst_volume(nodals_h8* h, int32_t equation, const double* k_free, const double* k_fixed,
+                         const double* omega_free, const double* omega_fixed, double k_floor, double omega_floor,
+                         int32_t deviatoric, double* mat, double* rhs);
+
 /* ---------------------------------------------------------------- GNN ---
  * Learned pressure-correction warm start (no reference counterpart; the
  * reference's only data-driven pressure aid is the snapshot POD of

@@ -57,7 +57,7 @@ EXPORTS = [
     "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
-    "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn",
+    "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
 ]
 
 
@@ -108,6 +108,7 @@ def lib() -> C.CDLL:
     L.nodals_gnn_train.argtypes = [vp, C.c_int32, C.c_float, C.c_uint32, C.POINTER(C.c_double)]
     L.nodals_gnn_evaluate.argtypes = [vp, C.POINTER(C.c_double)]
     L.nodals_h8_attach_gnn.argtypes = [vp, vp, C.c_int32, C.c_int32]
+    L.nodals_h8_sst_volume.argtypes = [vp, C.c_int32, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int32, vp, vp]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -219,6 +220,17 @@ class H8:
         ms, by = C.c_double(), C.c_double()
         _check(lib().nodals_h8_time_kernel(self._h, name.encode(), reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def sst_volume(self, equation: int, k_free, k_fixed, omega_free, omega_fixed, k_floor=1e-8,
+                   omega_floor=1e-3, deviatoric=False):
+        """k-omega SST volume terms of every cell: (mat[nc,8,8], rhs[nc,8])."""
+        nc = self.info["cells"]
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (k_free, k_fixed, omega_free, omega_fixed)]
+        mat = np.empty((nc, 8, 8))
+        rhs = np.empty((nc, 8))
+        _check(lib().nodals_h8_sst_volume(self._h, equation, *(_ptr(v) for v in a), k_floor, omega_floor,
+                                          1 if deviatoric else 0, _ptr(mat), _ptr(rhs)))
+        return mat, rhs
 
     def gnn_graph(self):
         """(off, nbr) of the face-adjacent cell graph the network runs on."""

@@ -259,6 +259,19 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
       copyOut(s.hierarchy().agg[l].id, host, count);
     } else if (n.rfind("cf_", 0) == 0) {
       copyCf(s.hierarchy(), n, host, count);
+    } else if (n == "cell_xy") {
+      // (x, y) of each cell's four vertices relative to the pipe axis
+      const auto& M = s.mesh();
+      const auto& P = s.setup().pipe;
+      std::vector<double> v;
+      v.reserve((std::size_t)s.setup().nc * 8);
+      for (int c = 0; c < s.setup().nc; ++c)
+        for (int a = 0; a < 4; ++a) {
+          const auto& X = M.points[(std::size_t)M.tets[(std::size_t)c][(std::size_t)a]];
+          v.push_back(X[0] - P.cx);
+          v.push_back(X[1] - P.cy);
+        }
+      copyOut(v, host, count);
     } else if (n == "pin_cell") {
       copyOut(std::vector<int32_t>{s.setup().pinCell}, host, count);
     } else if (n == "cell_nbr_off" || n == "cell_nbr") {
@@ -509,6 +522,22 @@ int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, con
 
 int nodals_wall_omega_target(int64_t n, const double* in, const int32_t* shared, double* out) {
   return guarded([&]() { nodals::omegaWallBatch(n, in, shared, out); });
+}
+
+int nodals_h8_sst_volume(nodals_h8* h, int32_t equation, const double* k_free, const double* k_fixed,
+                         const double* omega_free, const double* omega_fixed, double k_floor, double omega_floor,
+                         int32_t deviatoric, double* mat, double* rhs) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    const std::size_t nv = (std::size_t)s.setup().nv, nf = (std::size_t)s.setup().nFixed;
+    nodals::require(k_free && omega_free && mat && rhs && (nf == 0 || (k_fixed && omega_fixed)), "null arrays");
+    std::vector<double> m, r;
+    s.sstVolume(equation, std::vector<double>(k_free, k_free + nv), std::vector<double>(k_fixed, k_fixed + nf),
+                std::vector<double>(omega_free, omega_free + nv), std::vector<double>(omega_fixed, omega_fixed + nf),
+                k_floor, omega_floor, deviatoric, m, r);
+    std::memcpy(mat, m.data(), m.size() * sizeof(double));
+    std::memcpy(rhs, r.data(), r.size() * sizeof(double));
+  });
 }
 
 static nodals::GnnModel& gnnOf(nodals_gnn* g) {

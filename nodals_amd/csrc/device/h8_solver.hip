@@ -10,6 +10,7 @@
 #include <thread>
 
 #include "h8_kernels.hpp"
+#include "sst_volume.hpp"
 
 namespace nodals {
 
@@ -909,6 +910,50 @@ void H8Solver::attachGnn(GnnModel* g, int warmMode, int recordEvery) {
   gnnX_.alloc((std::size_t)g->paddedRows() * GnnModel::kIn);
   fill<float>(stream_, (long long)g->paddedRows() * GnnModel::kIn, gnnX_.data(), 0.0f);
   sync();
+}
+
+void H8Solver::sstVolume(int equation, const std::vector<double>& kFree, const std::vector<double>& kFixed,
+                         const std::vector<double>& wFree, const std::vector<double>& wFixed, double kFloor,
+                         double omegaFloor, int deviatoric, std::vector<double>& mat, std::vector<double>& rhs) {
+  requireDevice();
+  require(equation == 0 || equation == 1, "SST equation must be 0 (k) or 1 (omega)");
+  require((int)kFree.size() == nv_ && (int)wFree.size() == nv_ && (int)kFixed.size() == nFix_ &&
+              (int)wFixed.size() == nFix_,
+          "SST volume: k/omega sizes must be free DOFs and fixed slots");
+  if (sstXY_.size() != (std::size_t)nc_ * 8) {
+    std::vector<double> xy((std::size_t)nc_ * 8);
+    for (int c = 0; c < nc_; ++c)
+      for (int a = 0; a < 4; ++a) {
+        const auto& X = M_.points[(std::size_t)M_.tets[(std::size_t)c][(std::size_t)a]];
+        xy[(std::size_t)c * 8 + 2 * a] = X[0] - S_.pipe.cx;
+        xy[(std::size_t)c * 8 + 2 * a + 1] = X[1] - S_.pipe.cy;
+      }
+    sstXY_.upload(xy);
+    sstQuad_.upload(keast24());
+  }
+  std::vector<double> kd((std::size_t)nv_), wd((std::size_t)nv_);
+  for (int i = 0; i < nv_; ++i) {
+    kd[(std::size_t)i] = kFree[(std::size_t)perm_[(std::size_t)i]];
+    wd[(std::size_t)i] = wFree[(std::size_t)perm_[(std::size_t)i]];
+  }
+  DBuf<double> dk, dw, dkf, dwf, dmat, drhs;
+  dk.upload(kd);
+  dw.upload(wd);
+  std::vector<double> kf(kFixed), wf(wFixed);
+  if (kf.empty()) kf.push_back(0.0);
+  if (wf.empty()) wf.push_back(0.0);
+  dkf.upload(kf);
+  dwf.upload(wf);
+  dmat.alloc((std::size_t)nc_ * 64);
+  drhs.alloc((std::size_t)nc_ * 8);
+  hipLaunchKernelGGL(k_sst_volume, (nc_ + 3) / 4, 256, 0, stream_, nc_, cref_.data(), cinvJ_.data(), cdet_.data(),
+                     sstXY_.data(), u_.data(), fixedVal_.data(), dk.data(), dkf.data(), dw.data(), dwf.data(),
+                     sstQuad_.data(), S_.pipe.R, S_.pipe.nu, equation, kFloor, omegaFloor, deviatoric, dmat.data(),
+                     drhs.data());
+  NODALS_LAUNCHED();
+  sync();
+  mat = dmat.host();
+  rhs = drhs.host();
 }
 
 void H8Solver::gnnFeatures(float* X) {

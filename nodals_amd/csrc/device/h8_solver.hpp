@@ -138,6 +138,12 @@ class H8Solver {
   // samples every `recordEvery` SIMPLE iterations (0: off).  The network
   // must outlive its attachment; attach nullptr to detach.
   CellGraph cellGraph() const;
+  // k-omega SST volume terms (equation 0: k, 1: omega) of every cell from
+  // the current velocity and the given k, omega (free DOFs in reference
+  // order, then fixed slots): mat[nc][64] element matrices, rhs[nc][8].
+  void sstVolume(int equation, const std::vector<double>& kFree, const std::vector<double>& kFixed,
+                 const std::vector<double>& wFree, const std::vector<double>& wFixed, double kFloor,
+                 double omegaFloor, int deviatoric, std::vector<double>& mat, std::vector<double>& rhs);
   // mode 1: prediction as the initial guess at its residual-optimal
   // multiple; mode 2: as an added first Krylov direction.
   void attachGnn(GnnModel* g, int warmMode, int recordEvery);
@@ -204,6 +210,7 @@ class H8Solver {
   std::unique_ptr<dev::DevAmg<bf16, float>> amgb_;
   double setupMs_ = 0;
   double chebLambda_ = 1.0;
+  DBuf<double> sstXY_, sstQuad_;
   GnnModel* gnn_ = nullptr;
   bool gnnWarm_ = false, gnnAsGuess_ = false;
   int gnnEvery_ = 0;
