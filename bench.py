@@ -120,6 +120,62 @@ def traffic_per_launch() -> dict | None:
     return vals
 
 
+# Published H8 numbers of the reference (RTX 3060, gpu/serial_cuda/README.md):
+# fixed-10 FP64 at 768,530 cells 100.373 ms per SIMPLE = 7.657 MIUPS
+# (:156-158); Re=20 convergence gate at 111,183 cells to simpleTol 1e-3:
+# 407 SIMPLE iterations, 13.006 ms per SIMPLE = 8.549 MIUPS (:144-152).
+REF_FIXED10_FP64_MIUPS = 7.657
+REF_GATE_MS_PER_SIMPLE, REF_GATE_ITS = 13.006, 407
+
+
+def reference_legs() -> dict:
+    """The reference's own H8 benchmarks with its binary defaults (PCG, p-rtol
+    0.5, p-max 20, Chebyshev(2) smoother with power-iteration spectrum, SA
+    hierarchy, fgs1), on generated O-grid pipes of the same cell counts:
+    fixed-10 FP64 at 768,768 cells (16x16 core, 10 rings, 143 slabs) and the
+    Re = 20 convergence gate to simpleTol 1e-3 at 112,200 cells (10x10 core,
+    6 rings, 55 slabs).  MIUPS = cells / (ms per SIMPLE) / 1000."""
+    from nodals_amd import meshgen
+    from nodals_amd._lib import H8
+
+    out = {}
+    d = ROOT / "_meshes" / "pipe_896x143"
+    if not (d / "faces").exists():
+        meshgen.write_pipe_case(d, core=16, ring=10, nslab=143)
+    for prec in ("fp64", "fp32"):
+        h = H8(["--mesh", str(d), "--run-mode", "fixed10", "--max-outer", "10", "--amg-precision", prec,
+                "--verbose", "0"])
+        h.run()  # warm-up
+        reset(h)
+        r = h.run()
+        ms = r["loop_ms"] / 10.0
+        out[f"fixed10_768k_{prec}"] = {"cells": h.info["cells"], "ms_per_simple": ms,
+                                       "miups": h.info["cells"] / ms / 1000.0,
+                                       "avg_pressure_its": r["avg_pressure_its"],
+                                       "stage_ms": {k: r[f"{k}_ms"] / 10.0 for k in
+                                                    ("assembly", "momentum", "continuity", "pressure", "pupdate")}}
+        print(f"[bench] fixed-10 {h.info['cells']} cells {prec}: {ms:.3f} ms/SIMPLE "
+              f"({out[f'fixed10_768k_{prec}']['miups']:.2f} MIUPS)", file=sys.stderr, flush=True)
+        h.close()
+    out["fixed10_768k_fp64"]["miups_vs_reference_rtx3060"] = out["fixed10_768k_fp64"]["miups"] / REF_FIXED10_FP64_MIUPS
+    d = ROOT / "_meshes" / "pipe_340x55"
+    if not (d / "faces").exists():
+        meshgen.write_pipe_case(d, core=10, ring=6, nslab=55)
+    h = H8(["--mesh", str(d), "--run-mode", "converge", "--max-outer", "5000", "--simple-tol", "1e-3",
+            "--verbose", "0"])
+    r = h.run()
+    ms = r["loop_ms"] / max(r["outer"], 1)
+    out["gate_112k_fp64"] = {"cells": h.info["cells"], "outer": r["outer"], "converged": r["converged"],
+                             "ms_per_simple": ms, "loop_s": r["loop_ms"] / 1e3,
+                             "pressure_drop": r["pressure_drop"], "exact_pressure_drop": r["exact_pressure_drop"],
+                             "reference_outer": REF_GATE_ITS, "reference_ms_per_simple": REF_GATE_MS_PER_SIMPLE,
+                             "speedup_per_simple_vs_reference_rtx3060": REF_GATE_MS_PER_SIMPLE / ms}
+    print(f"[bench] gate {h.info['cells']} cells fp64: {r['outer']} SIMPLE, {ms:.3f} ms/SIMPLE", file=sys.stderr,
+          flush=True)
+    h.close()
+    return out
+
+
 def cavity_leg(steps: int) -> dict:
     """Lid-driven cavity (unit cube, 16^3 hexes x 6 Kuhn tets = 24,576 tets,
     lid u = (1, 0, 0), Re = 100), plain FP64, SIMPLE to 1e-6."""
@@ -200,6 +256,7 @@ def main() -> int:
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cavity", action="store_true")
+    ap.add_argument("--no-reference-legs", action="store_true")
     ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only the fine SpMV and the copy calibration kernel (for counter runs)")
@@ -274,6 +331,7 @@ def main() -> int:
     poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
     learned = None if a.no_learned else learned_leg(mesh, a.steps, a.learned_epochs)
     cavity = None if a.no_cavity else cavity_leg(1)
+    refl = None if a.no_reference_legs else reference_legs()
     if learned:
         learned["pressure_its_fp32_amg"] = int(fp32["runs"][-1]["pressure_its"])
         learned["speedup_vs_fp64"] = float(np.mean(fp64["times"])) / learned["s"]
@@ -313,6 +371,7 @@ def main() -> int:
             "poisson_64": poisson,
             "learned_fp32_amg": learned,
             "cavity_16_fp64": cavity,
+            "reference_published_benchmarks": refl,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
