@@ -272,8 +272,21 @@ static __global__ void __launch_bounds__(256) k_continuity(int nc, const int* __
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// W[c][a] = rAU_g B[c][a] for the free DOF g of cell c's basis a (zero for
+// fixed ones): every term of the fine Schur refresh is then W[c][a].B[c2][b]
+// with no per-term rAU gather.
+static __global__ void k_fine_weights(int nc, const int* __restrict__ cref, const double* __restrict__ bcoef,
+                                      const double* __restrict__ rau, double* __restrict__ W) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nc * 8) return;
+  const int g = cref[t];
+  const double r = g >= 0 ? rau[g] : 0.0;
+  for (int d = 0; d < 3; ++d) W[3ll * t + d] = r * bcoef[3ll * t + d];
+}
+
 // Fine P0 Schur numeric refresh, one nonzero per lane:
 //   val[k] = sum_{shared free DOF g} rAU_g * (B[c][a] . B[c2][b])
+//          = sum (W[c][a] . B[c2][b])
 // (h8_fine_csr_refresh_warp_kernel, h8_precomputed_b.cuh:37-91, restated as
 // a gather over the terms recorded at setup; no atomics).  Also writes the
 // diagonal, optional lower-precision copies and the row-L1 sums.  With a
@@ -283,7 +296,7 @@ template <int L, bool F32, bool B16, bool L1>
 __global__ void __launch_bounds__(256) k_fine_refresh(
     int nc, const int* __restrict__ frow, const int* __restrict__ fcol, const int* __restrict__ fdiagPos,
     const int* __restrict__ nzOff, const unsigned* __restrict__ nzTerm, const double* __restrict__ bcoef,
-    const double* __restrict__ rau, int pin, double* __restrict__ val, float* __restrict__ valf,
+    const double* __restrict__ W, int pin, double* __restrict__ val, float* __restrict__ valf,
     bf16* __restrict__ valb, double* __restrict__ diag, float* __restrict__ diagf, double* __restrict__ l1d,
     float* __restrict__ l1f) {
   const int gid = blockIdx.x * 256 + threadIdx.x;
@@ -291,7 +304,7 @@ __global__ void __launch_bounds__(256) k_fine_refresh(
   const int lane = gid % L;
   if (c >= nc) return;
   const int rb = frow[c], re = frow[c + 1], dp = fdiagPos[c];
-  const double* Bc = bcoef + (long long)c * 24;
+  const double* Wc = W + (long long)c * 24;
   double sabs = 0.0;
   for (int k = rb + lane; k < re; k += L) {
     const int c2 = fcol[k];
@@ -299,9 +312,8 @@ __global__ void __launch_bounds__(256) k_fine_refresh(
     double v = 0.0;
     for (int t = nzOff[k]; t < nzOff[k + 1]; ++t) {
       const unsigned q = nzTerm[t];
-      const int g = (int)(q >> 6), a = (int)((q >> 3) & 7u), b = (int)(q & 7u);
-      const double dot = Bc[3 * a] * Bn[3 * b] + Bc[3 * a + 1] * Bn[3 * b + 1] + Bc[3 * a + 2] * Bn[3 * b + 2];
-      v += rau[g] * dot;
+      const int a = (int)((q >> 3) & 7u), b = (int)(q & 7u);
+      v += Wc[3 * a] * Bn[3 * b] + Wc[3 * a + 1] * Bn[3 * b + 1] + Wc[3 * a + 2] * Bn[3 * b + 2];
     }
     if ((c == pin || c2 == pin) && c != c2) v = 0.0;
     val[k] = v;

@@ -522,6 +522,9 @@ void H8Solver::upload() {
   fdiag_.alloc((std::size_t)nc_);
   fl1_.alloc((std::size_t)nc_);
   fineLanes_ = lanesFor((double)fineNnz_ / std::max(nc_, 1));
+  // the refresh does ~1.5 dependent term loads per nonzero: wider groups
+  refreshLanes_ = fineNnz_ > 48ll * nc_ ? 32 : 16;
+  fineW_.alloc((std::size_t)nc_ * 24);
   if (opt_.amgPrecision != "fp64") {
     fdiagf_.alloc((std::size_t)nc_);
     fl1f_.alloc((std::size_t)nc_);
@@ -703,12 +706,15 @@ double H8Solver::continuity() {
 
 void H8Solver::refresh() {
   requireDevice();
-  const int grid = blocksFor((long long)nc_ * fineLanes_);
+  hipLaunchKernelGGL(k_fine_weights, blocksFor((long long)nc_ * 8), 256, 0, stream_, nc_, cref_.data(), bcoef_.data(),
+                     rau_.data(), fineW_.data());
+  NODALS_LAUNCHED();
+  const int grid = blocksFor((long long)nc_ * refreshLanes_);
   const bool f32 = opt_.amgPrecision == "fp32", b16 = opt_.amgPrecision == "bf16";
   const bool l1 = opt_.amgSmoother == "l1jacobi";
 #define NODALS_REFRESH(L, F, B, LL)                                                                                  \
   hipLaunchKernelGGL((k_fine_refresh<L, F, B, LL>), grid, 256, 0, stream_, nc_, frow_.data(), fcol_.data(),         \
-                     fdiagPos_.data(), nzOff_.data(), nzTerm_.data(), bcoef_.data(), rau_.data(), S_.pinCell,       \
+                     fdiagPos_.data(), nzOff_.data(), nzTerm_.data(), bcoef_.data(), fineW_.data(), S_.pinCell,     \
                      fval_.data(), fvalf_.data(), fvalb_.data(), fdiag_.data(), fdiagf_.data(), fl1_.data(),        \
                      fl1f_.data())
 #define NODALS_REFRESH_P(L)                         \
@@ -722,7 +728,7 @@ void H8Solver::refresh() {
     if (l1) NODALS_REFRESH(L, false, false, true);  \
     else NODALS_REFRESH(L, false, false, false);    \
   }
-  switch (fineLanes_) {
+  switch (refreshLanes_) {
     case 4: NODALS_REFRESH_P(4); break;
     case 8: NODALS_REFRESH_P(8); break;
     case 16: NODALS_REFRESH_P(16); break;

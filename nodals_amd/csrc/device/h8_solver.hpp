@@ -181,7 +181,7 @@ class H8Solver {
   dev::KrylovWork kw_;
 
   std::vector<int> perm_, pinv_;  // device numbering: perm_[new] = reference row
-  int nc_ = 0, nv_ = 0, nFix_ = 0, fineLanes_ = 16;
+  int nc_ = 0, nv_ = 0, nFix_ = 0, fineLanes_ = 16, refreshLanes_ = 32;
   long long momNnz_ = 0, fineNnz_ = 0;
   std::vector<int> colorOff_, colorSplit_;  // colour c: [off, split) long rows, [split, off+1) short rows
   // Momentum rows with more than kLongRow nonzeros (vertex rows) get
@@ -200,7 +200,7 @@ class H8Solver {
   DBuf<double> p_, fixedDiv_, cont_, pcx_;
   DBuf<int> frow_, fcol_, fdiagPos_, nzOff_;
   DBuf<unsigned> nzTerm_;
-  DBuf<double> fval_, fdiag_, fl1_;
+  DBuf<double> fval_, fdiag_, fl1_, fineW_;  // fineW_: rAU-weighted B per cell basis
   DBuf<float> fvalf_, fdiagf_, fl1f_, rf_, zf_;
   DBuf<bf16> fvalb_;
   DBuf<double> tmpA_, tmpB_, calibA_, calibB_, rd_, zd_;
