@@ -132,6 +132,20 @@ class DevAmg {
 
   // z = M^{-1} b on the finest level.
   void apply(hipStream_t s, const TV* b, TV* z) {
+    if (fusedJacobi()) {
+      // pre-smoothing + residual, coarse correction, residual + post-
+      // smoothing: two fine passes instead of five kernels
+      const TV* den = opt_.kind == "l1jacobi" ? fine_.l1 : fine_.diag;
+      jacobiPreResid<TF, TV>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den, opt_.jacobiOmega,
+                             ftmp_.data(), fr_.data());
+      Level& C = L_[0];
+      restrictTo(s, P_[0], fr_.data(), C.b.data());
+      vcycle(s, 0, C.b.data(), C.x.data());
+      prolongAdd(s, P_[0], C.x.data(), ftmp_.data());
+      residJacobiPost<TF, TV>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den, opt_.jacobiOmega,
+                              ftmp_.data(), z);
+      return;
+    }
     smoothFine(s, b, z);
     spmv<TF, TV, true, false>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, z, b, fr_.data());
     Level& C = L_[0];
@@ -259,6 +273,18 @@ class DevAmg {
       NODALS_LAUNCHED();
       return;
     }
+    if (fusedJacobi()) {
+      const TV* den = opt_.kind == "l1jacobi" ? A.l1.data() : A.diag.data();
+      jacobiPreResid<TV, TV>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), b, den, opt_.jacobiOmega,
+                             A.tmp.data(), A.r.data());
+      Level& C = L_[(std::size_t)l + 1];
+      restrictTo(s, P_[(std::size_t)l + 1], A.r.data(), C.b.data());
+      vcycle(s, l + 1, C.b.data(), C.x.data());
+      prolongAdd(s, P_[(std::size_t)l + 1], C.x.data(), A.tmp.data());
+      residJacobiPost<TV, TV>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), b, den, opt_.jacobiOmega,
+                              A.tmp.data(), x);
+      return;
+    }
     smoothLevel(s, A, b, x);
     spmv<TV, TV, true, false>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), x, b, A.r.data());
     Level& C = L_[(std::size_t)l + 1];
@@ -268,6 +294,12 @@ class DevAmg {
     spmv<TV, TV, true, false>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), x, b, A.r.data());
     smoothLevel(s, A, A.r.data(), A.corr.data());
     axpy<TV>(s, A.n, 1.0, A.corr.data(), x);
+  }
+
+  // One Jacobi sweep before and after on every level: the fused kernels.
+  bool fusedJacobi() const {
+    return (opt_.kind == "jacobi" || opt_.kind == "l1jacobi") && opt_.jacobiFineSweeps == 1 &&
+           opt_.jacobiCoarseSweeps == 1;
   }
 
   SmootherOptions opt_;

@@ -233,16 +233,22 @@ __global__ void __launch_bounds__(256) k_mom_norms(int nLong, const int* __restr
                                                    const int* __restrict__ mrow, const int* __restrict__ mcol,
                                                    const double* __restrict__ aval, const double* __restrict__ brhs,
                                                    const double* __restrict__ u, double* __restrict__ part) {
-  __shared__ double sh[4];
+  __shared__ double sh[4][6];
   double acc[6] = {0, 0, 0, 0, 0, 0};
 #define NODALS_NORM_BODY(LN, q, lane, isLong) \
   mom_norm_row<LN>((isLong) ? longRows[q] : shortRows[q], lane, mrow, mcol, aval, brhs, u, acc)
   NODALS_TWO_CLASS(LL, LS, nLong, nShort, nbLong, NODALS_NORM_BODY)
 #undef NODALS_NORM_BODY
+  // all six sums with cross-lane shuffles per wavefront, one barrier
+#pragma unroll
   for (int q = 0; q < 6; ++q) {
-    const double t = blockSum256(acc[q], sh);
-    if (threadIdx.x == 0) part[(long long)q * gridDim.x + blockIdx.x] = t;
+    const double t = groupSum(acc[q], 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][q] = t;
   }
+  __syncthreads();
+  if (threadIdx.x < 6)
+    part[(long long)threadIdx.x * gridDim.x + blockIdx.x] =
+        sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
 // Continuity residual r_c = fixedDiv_c + sum_a B[c][a] . u(ref a)

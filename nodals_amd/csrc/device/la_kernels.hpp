@@ -190,6 +190,55 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int* __restrict__ row
   }
 }
 
+// Single-sweep Jacobi smoothing fused into the V-cycle residuals.
+// Pre-smoothing from zero and its residual in one pass:
+//   t = w b / d,   r = b - A t
+// (t_j of every neighbour is formed from b_j and d_j as the separate
+// smoother would, so the values equal the unfused sequence).
+template <int LANES, class TM, class TV>
+__global__ void __launch_bounds__(256) k_jacobi_pre_resid(int n, const int* __restrict__ row,
+                                                          const int* __restrict__ col, const TM* __restrict__ val,
+                                                          const TV* __restrict__ b, const TV* __restrict__ d,
+                                                          double omega, TV* __restrict__ t, TV* __restrict__ r) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / LANES;
+  const int lane = gid % LANES;
+  if (i >= n) return;
+  TV s = 0;
+  const int e = row[i + 1];
+  for (int k = row[i] + lane; k < e; k += LANES) {
+    const int j = col[k];
+    const TV tj = (TV)omega * b[j] / d[j];
+    s += ld<TV>(val, k) * tj;
+  }
+  s = groupSum(s, LANES);
+  if (lane == 0) {
+    t[i] = (TV)omega * b[i] / d[i];
+    r[i] = b[i] - s;
+  }
+}
+// Residual and post-smoothing correction in one pass:
+//   z = t + w (b - A t) / d      (z and t distinct)
+template <int LANES, class TM, class TV>
+__global__ void __launch_bounds__(256) k_resid_jacobi_post(int n, const int* __restrict__ row,
+                                                           const int* __restrict__ col, const TM* __restrict__ val,
+                                                           const TV* __restrict__ b, const TV* __restrict__ d,
+                                                           double omega, const TV* __restrict__ t,
+                                                           TV* __restrict__ z) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / LANES;
+  const int lane = gid % LANES;
+  if (i >= n) return;
+  TV s = 0;
+  const int e = row[i + 1];
+  for (int k = row[i] + lane; k < e; k += LANES) s += ld<TV>(val, k) * t[col[k]];
+  s = groupSum(s, LANES);
+  if (lane == 0) {
+    const TV r = b[i] - s;
+    z[i] = t[i] + (TV)omega * r / d[i];
+  }
+}
+
 // l1[r] = sum_k |a_rk|
 template <int LANES, class TM, class TV>
 __global__ void __launch_bounds__(256) k_row_l1(int n, const int* __restrict__ row, const TM* __restrict__ val,
@@ -288,6 +337,42 @@ void spmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const
     default: NODALS_SPMV_LAUNCH(64) break;
   }
 #undef NODALS_SPMV_LAUNCH
+  NODALS_LAUNCHED();
+}
+
+template <class TM, class TV>
+void jacobiPreResid(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TV* b,
+                    const TV* d, double omega, TV* t, TV* r) {
+  if (n <= 0) return;
+  const int grid = blocksFor((long long)n * lanes);
+#define NODALS_JPR(LN) \
+  hipLaunchKernelGGL((k_jacobi_pre_resid<LN, TM, TV>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, r)
+  switch (lanes) {
+    case 4: NODALS_JPR(4); break;
+    case 8: NODALS_JPR(8); break;
+    case 16: NODALS_JPR(16); break;
+    case 32: NODALS_JPR(32); break;
+    default: NODALS_JPR(64); break;
+  }
+#undef NODALS_JPR
+  NODALS_LAUNCHED();
+}
+
+template <class TM, class TV>
+void residJacobiPost(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TV* b,
+                     const TV* d, double omega, const TV* t, TV* z) {
+  if (n <= 0) return;
+  const int grid = blocksFor((long long)n * lanes);
+#define NODALS_RJP(LN) \
+  hipLaunchKernelGGL((k_resid_jacobi_post<LN, TM, TV>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, z)
+  switch (lanes) {
+    case 4: NODALS_RJP(4); break;
+    case 8: NODALS_RJP(8); break;
+    case 16: NODALS_RJP(16); break;
+    case 32: NODALS_RJP(32); break;
+    default: NODALS_RJP(64); break;
+  }
+#undef NODALS_RJP
   NODALS_LAUNCHED();
 }
 
