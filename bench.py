@@ -317,6 +317,7 @@ def main() -> int:
     _, by = h.time_kernel("fine_spmv", 1)
     achieved = by / (ms * 1e-3) / 1e9
     warm_ms, _ = h.time_kernel("fine_spmv", a.spmv_reps)
+    fine_lanes = int(h.get("fine_lanes", 1)[0])
     extra = {}
     for k in ("refresh", "mom_gs", "assemble", "continuity"):
         kms, kby = h.time_kernel(k, 100)
@@ -377,7 +378,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic["traffic_bytes_per_launch"],
-                     "kernel": "k_spmv<32,double,double,false,false> (fine P0 Schur SpMV, FP64)",
+                     "kernel": f"k_spmv<{fine_lanes},double,double,false,false> (fine P0 Schur SpMV, FP64)",
                      "kernel_ms": ms, "launches": launches, "bytes_per_launch": by,
                      "warm_loop_kernel_ms": warm_ms,
                      "traffic_calibration": None if traffic is None else

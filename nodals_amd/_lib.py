@@ -137,7 +137,7 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
 
 _DTYPES = {
     "mom_row": np.int64, "mom_col": np.int32, "color": np.int32, "g2free": np.int32, "fine_row": np.int32,
-    "fine_col": np.int32, "cell_ref": np.int32, "cell_slot": np.uint8, "pin_cell": np.int32,
+    "fine_col": np.int32, "cell_ref": np.int32, "cell_slot": np.uint8, "pin_cell": np.int32, "fine_lanes": np.int32,
 }
 
 

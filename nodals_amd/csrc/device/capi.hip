@@ -272,6 +272,8 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
           v.push_back(X[1] - P.cy);
         }
       copyOut(v, host, count);
+    } else if (n == "fine_lanes") {
+      copyOut(std::vector<int32_t>{s.fineLanes()}, host, count);
     } else if (n == "pin_cell") {
       copyOut(std::vector<int32_t>{s.setup().pinCell}, host, count);
     } else if (n == "cell_nbr_off" || n == "cell_nbr") {

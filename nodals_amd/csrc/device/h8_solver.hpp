@@ -118,6 +118,7 @@ class H8Solver {
   const AmgHierarchy& hierarchy() const { return H_; }
   const TetMesh& mesh() const { return M_; }
   int ncolors() const { return S_.coloring.ncolors; }
+  int fineLanes() const { return fineLanes_; }
 
   // Kernel-level handles for measurement: the fine CSR SpMV on device data.
   void fineSpmv(const double* x, double* y);
