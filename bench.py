@@ -203,6 +203,32 @@ def cavity_leg(steps: int) -> dict:
     return out
 
 
+def turb_leg(h, steps: int) -> dict:
+    """k-omega SST transport step (k then omega: convection, volume terms,
+    inlet flux, omega wall penalty at the third of five wall-normal samples,
+    row-L1 relaxation, 20 + 20 coloured Gauss-Seidel sweeps, floors) on the
+    converged pipe velocity of context h, with the reference's spiral-case
+    settings (alpha 0.3, sweep omega 0.7, penalty 1e6)."""
+    t0 = time.perf_counter()
+    h.turb_init([])
+    setup_s = time.perf_counter() - t0
+    for _ in range(2):
+        h.turb_step()
+    dev, wall = [], []
+    for _ in range(max(1, steps)):
+        ts = time.perf_counter()
+        r = h.turb_step()
+        wall.append(time.perf_counter() - ts)
+        dev.append(r["ms"])
+    out = {"ms_per_step_device": float(np.mean(dev)), "ms_per_step_wall": 1e3 * float(np.mean(wall)),
+           "steps": len(dev), "setup_s": setup_s, "scalar_dofs": h.turb["ns"], "vertex_rows": h.turb["nvert"],
+           "colors": h.turb["colors"], "wall_faces": h.turb["wall_faces"], "inlet_faces": h.turb["inlet_faces"],
+           "sweeps_per_equation": 20}
+    print(f"[bench] k-omega step: {out['ms_per_step_device']:.3f} ms device, {out['ms_per_step_wall']:.3f} ms wall "
+          f"({h.turb['colors']} colours, {h.turb['wall_faces']} wall faces)", file=sys.stderr, flush=True)
+    return out
+
+
 def learned_leg(mesh: str, steps: int, epochs: int) -> dict:
     """Learned warm start + FP32 AMG on the same pipe: samples recorded from
     the solver's own converged runs at Re 10, 15, 30, 45 (every 4th SIMPLE
@@ -257,6 +283,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cavity", action="store_true")
     ap.add_argument("--no-reference-legs", action="store_true")
+    ap.add_argument("--no-turb", action="store_true")
     ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only the fine SpMV and the copy calibration kernel (for counter runs)")
@@ -329,6 +356,7 @@ def main() -> int:
     outer = int(fp32["runs"][-1]["outer"])
     dp32 = fp32["runs"][-1]["pressure_drop"]
     dp64 = fp64["runs"][-1]["pressure_drop"]
+    turb = None if a.no_turb else turb_leg(h, 20)
     poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
     learned = None if a.no_learned else learned_leg(mesh, a.steps, a.learned_epochs)
     cavity = None if a.no_cavity else cavity_leg(1)
@@ -372,13 +400,14 @@ def main() -> int:
             "poisson_64": poisson,
             "learned_fp32_amg": learned,
             "cavity_16_fp64": cavity,
+            "k_omega_step_40k": turb,
             "reference_published_benchmarks": refl,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic["traffic_bytes_per_launch"],
-                     "kernel": f"k_spmv<{fine_lanes},double,double,false,false> (fine P0 Schur SpMV, FP64)",
+                     "kernel": f"k_schur_spmv<{fine_lanes}> (fine P0 Schur SpMV, FP64)",
                      "kernel_ms": ms, "launches": launches, "bytes_per_launch": by,
                      "warm_loop_kernel_ms": warm_ms,
                      "traffic_calibration": None if traffic is None else

@@ -183,6 +183,35 @@ int nodals_h8_sst_volume(nodals_h8* h, int32_t equation, const double* k_free, c
                          const double* omega_free, const double* omega_fixed, double k_floor, double omega_floor,
                          int32_t deviatoric, double* mat, double* rhs);
 
+/* k-omega SST transport step of an H8 context (masked-P1 state space: one
+ * scalar per mesh entity, vertices first then faces, face-bubble values
+ * zero).  Replaces step_turbulence (turbulence/sst_gpu_v2/source/
+ * puretet_profile5_eq5_hxt5b/modules/coupling/g2_step_initialization.inc,
+ * g2_step_convection.inc, g2_step_k.inc, g2_step_omega.inc) for tetrahedral
+ * pipes: convection at the context's current velocity, then k and omega in
+ * turn: volume terms, inlet flux (tet_inlet_flux.inc), omega wall penalty at
+ * the third of five wall-normal samples with the OF-auto target
+ * (puretet_of_auto_point3.cuh), row-L1 relaxation (scalar_relaxation.inc:
+ * 8-33), fixed coloured Gauss-Seidel sweeps (scalar_linear_solver.inc:
+ * 33-67 "fgs-fixed") and the floor.
+ * nodals_h8_turb_init takes the reference's SST flags (--sst-intensity,
+ * --sst-length-ratio, --sst-alpha-k, --sst-alpha-omega, --sst-k-fgs-sweeps,
+ * --sst-omega-fgs-sweeps, --sst-gs-omega, --sst-k-floor, --sst-omega-floor,
+ * --sst-deviatoric-production, --sst-omega-wall-mode of_auto|none,
+ * --sst-wall-penalty-gamma, --sst-profile-fit-y1..y5,
+ * --sst-wall-sample-scale, --sst-of-kappa, --sst-of-E, --sst-of-beta1,
+ * --sst-of-re-blend, --sst-of-production-limit; argv holds flags only) and
+ * sets k, omega to their inlet values.  k and omega are read and written
+ * with nodals_h8_get / nodals_h8_set ("k", "omega"). */
+typedef struct nodals_turb_step_result {
+  int32_t k_clips, omega_clips; /* vertex values raised to the floor */
+  int32_t wall_invalid;         /* wall points without a valid fit or target */
+  int32_t reserved;
+  double ms;                    /* device time of the step */
+} nodals_turb_step_result;
+int nodals_h8_turb_init(nodals_h8* h, int argc, const char* const* argv);
+int nodals_h8_turb_step(nodals_h8* h, nodals_turb_step_result* res);
+
 /* ---------------------------------------------------------------- GNN ---
  * Learned pressure-correction warm start (no reference counterpart; the
  * reference's only data-driven pressure aid is the snapshot POD of

@@ -58,6 +58,7 @@ EXPORTS = [
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
     "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
+    "nodals_h8_turb_init", "nodals_h8_turb_step",
 ]
 
 
@@ -109,6 +110,8 @@ def lib() -> C.CDLL:
     L.nodals_gnn_evaluate.argtypes = [vp, C.POINTER(C.c_double)]
     L.nodals_h8_attach_gnn.argtypes = [vp, vp, C.c_int32, C.c_int32]
     L.nodals_h8_sst_volume.argtypes = [vp, C.c_int32, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int32, vp, vp]
+    L.nodals_h8_turb_init.argtypes = [vp, C.c_int, vp]
+    L.nodals_h8_turb_step.argtypes = [vp, vp]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -138,7 +141,18 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
 _DTYPES = {
     "mom_row": np.int64, "mom_col": np.int32, "color": np.int32, "g2free": np.int32, "fine_row": np.int32,
     "fine_col": np.int32, "cell_ref": np.int32, "cell_slot": np.uint8, "pin_cell": np.int32, "fine_lanes": np.int32,
+    "mesh_tets": np.int32, "mesh_opp": np.int32, "mesh_faces": np.int32, "mesh_owner": np.int32,
+    "mesh_neighbour": np.int32, "mesh_face_patch": np.int32, "mesh_sizes": np.int64, "fixed_slot": np.int32,
 }
+
+
+class TurbStepResult(C.Structure):
+    _fields_ = [("k_clips", C.c_int32), ("omega_clips", C.c_int32), ("wall_invalid", C.c_int32),
+                ("reserved", C.c_int32), ("ms", C.c_double)]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {"k_clips": self.k_clips, "omega_clips": self.omega_clips, "wall_invalid": self.wall_invalid,
+                "ms": self.ms}
 
 
 class H8:
@@ -231,6 +245,21 @@ class H8:
         _check(lib().nodals_h8_sst_volume(self._h, equation, *(_ptr(v) for v in a), k_floor, omega_floor,
                                           1 if deviatoric else 0, _ptr(mat), _ptr(rhs)))
         return mat, rhs
+
+    def turb_init(self, args: Sequence[str] = ()) -> None:
+        """Set up the k-omega transport step (reference SST flags, e.g.
+        ["--sst-alpha-k", "0.3"]); k and omega start at their inlet values."""
+        a = [str(x).encode() for x in args]
+        arr = (C.c_char_p * max(len(a), 1))(*a) if a else None
+        _check(lib().nodals_h8_turb_init(self._h, len(a), arr))
+        st = self.get("turb_inlet_state", 7)
+        self.turb = {"k_in": st[0], "omega_in": st[1], "inlet_faces": int(st[2]), "wall_faces": int(st[3]),
+                     "ns": int(st[4]), "nvert": int(st[5]), "colors": int(st[6])}
+
+    def turb_step(self) -> Dict[str, float]:
+        r = TurbStepResult()
+        _check(lib().nodals_h8_turb_step(self._h, C.byref(r)))
+        return r.as_dict()
 
     def gnn_graph(self):
         """(off, nbr) of the face-adjacent cell graph the network runs on."""

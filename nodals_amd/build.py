@@ -34,8 +34,9 @@ HOST_FLAGS = COMMON + ["-ffp-contract=off"]
 DEVICE_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                          "-munsafe-fp-atomics"]
 
-HOST_SOURCES = ["host/mesh.cpp", "host/h8_setup.cpp", "host/amg_setup.cpp"]
-DEVICE_SOURCES = ["device/h8_solver.hip", "device/sys_solver.hip", "device/sst_kernels.hip", "device/gnn.hip", "device/capi.hip"]
+HOST_SOURCES = ["host/mesh.cpp", "host/h8_setup.cpp", "host/amg_setup.cpp", "host/turb_setup.cpp"]
+DEVICE_SOURCES = ["device/h8_solver.hip", "device/sys_solver.hip", "device/sst_kernels.hip", "device/gnn.hip", "device/turb_step.hip",
+                   "device/capi.hip"]
 
 
 def _run(cmd, verbose):

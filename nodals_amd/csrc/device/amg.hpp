@@ -37,6 +37,7 @@ template <class TF, class TV>
 struct FineView {
   int n = 0;
   int lanes = 16;
+  double meanRow = 8.0;  // nonzeros per row (lanes of the per-colour passes)
   const int* row = nullptr;
   const int* col = nullptr;
   const TF* val = nullptr;
@@ -61,6 +62,7 @@ class DevAmg {
  public:
   struct Level {
     int n = 0, lanes = 8;
+    double meanRow = 1.0;
     DBuf<int> row, col;
     DBuf<TV> val, diag, l1, b, x, r, tmp, corr;
     double lambda = 1.0;
@@ -86,7 +88,8 @@ class DevAmg {
       const CsrD& A = H.A[l];
       Level& Lv = L_[l];
       Lv.n = A.n;
-      Lv.lanes = lanesFor(A.n ? (double)A.col.size() / A.n : 1.0);
+      Lv.meanRow = A.n ? (double)A.col.size() / A.n : 1.0;
+      Lv.lanes = lanesFor(Lv.meanRow, A.n);
       std::vector<int> r32(A.row.begin(), A.row.end());
       Lv.row.upload(r32);
       Lv.col.upload(A.col);
@@ -172,7 +175,7 @@ class DevAmg {
     X.prow.upload(pr);
     X.pcol.upload(P.col);
     X.pval.upload(castVec(P.val));
-    X.lanesP = lanesFor(P.nFine ? (double)P.col.size() / P.nFine : 1.0);
+    X.lanesP = lanesFor(P.nFine ? (double)P.col.size() / P.nFine : 1.0, P.nFine);
     // Transpose for gather restriction; within a coarse row, fine rows ascend.
     std::vector<int> tr((std::size_t)P.nCoarse + 1, 0);
     for (int c : P.col) ++tr[(std::size_t)c + 1];
@@ -190,7 +193,7 @@ class DevAmg {
     X.trow.upload(tr);
     X.tcol.upload(tc);
     X.tval.upload(castVec(tv));
-    X.lanesT = lanesFor(P.nCoarse ? (double)tc.size() / P.nCoarse : 1.0);
+    X.lanesT = lanesFor(P.nCoarse ? (double)tc.size() / P.nCoarse : 1.0, P.nCoarse);
   }
 
   void restrictTo(hipStream_t s, const Xfer& X, const TV* f, TV* c) {
@@ -208,7 +211,7 @@ class DevAmg {
 
   // Generic smoother on an operator (row, col, val, diag, l1) from x = 0.
   template <class TM>
-  void smooth(hipStream_t s, int n, int lanes, const int* row, const int* col, const TM* val, const TV* diag,
+  void smooth(hipStream_t s, int n, int lanes, double meanRow, const int* row, const int* col, const TM* val, const TV* diag,
               const TV* l1, const DevColoring<TV>* color, double lambda, int jacobiSweeps, int mcgsSweeps,
               const TV* b, TV* x, TV* tmp, TV* r) {
     const int grid = blocksFor(n);
@@ -236,8 +239,10 @@ class DevAmg {
     } else if (opt_.kind == "mcgs") {
       fill<TV>(s, n, x, TV(0));
       for (int sw = 0; sw < mcgsSweeps; ++sw) {
-        if (opt_.mcgsOrder == "forward" || opt_.mcgsOrder == "symmetric") colorPass(s, lanes, row, col, val, diag, b, x, *color, false);
-        if (opt_.mcgsOrder == "backward" || opt_.mcgsOrder == "symmetric") colorPass(s, lanes, row, col, val, diag, b, x, *color, true);
+        if (opt_.mcgsOrder == "forward" || opt_.mcgsOrder == "symmetric")
+          colorPass(s, meanRow, row, col, val, diag, b, x, *color, false);
+        if (opt_.mcgsOrder == "backward" || opt_.mcgsOrder == "symmetric")
+          colorPass(s, meanRow, row, col, val, diag, b, x, *color, true);
       }
     } else {
       fail("unknown AMG smoother " + opt_.kind);
@@ -246,23 +251,23 @@ class DevAmg {
   }
 
   template <class TM>
-  void colorPass(hipStream_t s, int lanes, const int* row, const int* col, const TM* val, const TV* diag, const TV* b,
+  void colorPass(hipStream_t s, double meanRow, const int* row, const int* col, const TM* val, const TV* diag, const TV* b,
                  TV* x, const DevColoring<TV>& C, bool reverse) {
     for (int q = 0; q < C.ncolors; ++q) {
       const int c = reverse ? C.ncolors - 1 - q : q;
-      const int begin = C.off[(std::size_t)c];
-      mcgsColor<TM, TV>(s, lanes, begin, C.off[(std::size_t)c + 1] - begin, C.rows.data(), row, col, val, diag, b, x,
+      const int begin = C.off[(std::size_t)c], count = C.off[(std::size_t)c + 1] - begin;
+      mcgsColor<TM, TV>(s, lanesFor(meanRow, count), begin, count, C.rows.data(), row, col, val, diag, b, x,
                         opt_.mcgsOmega);
     }
   }
 
   void smoothFine(hipStream_t s, const TV* b, TV* x) {
-    smooth<TF>(s, fine_.n, fine_.lanes, fine_.row, fine_.col, fine_.val, fine_.diag, fine_.l1, &fineColor_,
+    smooth<TF>(s, fine_.n, fine_.lanes, fine_.meanRow, fine_.row, fine_.col, fine_.val, fine_.diag, fine_.l1, &fineColor_,
                fineLambda_, opt_.jacobiFineSweeps, opt_.mcgsFineSweeps, b, x, ftmp_.data(), ftmp_.data());
   }
 
   void smoothLevel(hipStream_t s, Level& A, const TV* b, TV* x) {
-    smooth<TV>(s, A.n, A.lanes, A.row.data(), A.col.data(), A.val.data(), A.diag.data(), A.l1.data(), &A.color,
+    smooth<TV>(s, A.n, A.lanes, A.meanRow, A.row.data(), A.col.data(), A.val.data(), A.diag.data(), A.l1.data(), &A.color,
                A.lambda, opt_.jacobiCoarseSweeps, opt_.mcgsCoarseSweeps, b, x, A.tmp.data(), A.tmp.data());
   }
 

@@ -244,6 +244,36 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
     } else if (n == "pipe") {
       const auto& P = s.setup().pipe;
       copyOut(std::vector<double>{P.nu, P.hpDrop, P.profileScale, P.R, P.L}, host, count);
+    } else if (n == "pipe_geometry") {
+      const auto& P = s.setup().pipe;
+      copyOut(std::vector<double>{P.cx, P.cy, P.R, P.D, P.bulk, P.nu, P.inletVelocity[0], P.inletVelocity[1],
+                                  P.inletVelocity[2], (double)P.inlet, (double)P.wall, (double)P.outlet},
+              host, count);
+    } else if (n == "mesh_sizes") {
+      const auto& M = s.mesh();
+      copyOut(std::vector<int64_t>{M.nPoints(), M.nFaces(), M.nInternal(), M.nCells()}, host, count);
+    } else if (n == "mesh_points") {
+      std::vector<double> v;
+      for (const auto& X : s.mesh().points) v.insert(v.end(), X.begin(), X.end());
+      copyOut(v, host, count);
+    } else if (n == "mesh_tets" || n == "mesh_opp" || n == "mesh_faces" || n == "mesh_owner" ||
+               n == "mesh_neighbour" || n == "mesh_face_patch") {
+      const auto& M = s.mesh();
+      std::vector<int32_t> v;
+      if (n == "mesh_tets")
+        for (const auto& t : M.tets) v.insert(v.end(), t.begin(), t.end());
+      else if (n == "mesh_opp")
+        for (const auto& t : M.oppFace) v.insert(v.end(), t.begin(), t.end());
+      else if (n == "mesh_faces")
+        for (const auto& f : M.faces) v.insert(v.end(), f.begin(), f.end());
+      else if (n == "mesh_owner") v = M.owner;
+      else if (n == "mesh_neighbour") v = M.neighbour;
+      else v = M.facePatch;
+      copyOut(v, host, count);
+    } else if (n == "fixed_slot") {
+      copyOut(s.setup().fixedSlot, host, count);
+    } else if (n == "k" || n == "omega" || n.rfind("turb_", 0) == 0) {
+      copyOut(s.turbField(n), host, count);
     }
     else if (n == "cell_ref") {
       std::vector<int32_t> v;
@@ -321,6 +351,7 @@ int nodals_h8_set(nodals_h8* h, const char* name, const void* host, int64_t coun
     const double* d = static_cast<const double*>(host);
     std::vector<double> v(d, d + count);
     if (n == "u0" || n == "u1" || n == "u2") s.setVelocity(n[1] - '0', v);
+    else if (n == "k" || n == "omega") s.setTurbField(n, v);
     else s.setCellField(n, v);
   });
 }
@@ -659,6 +690,27 @@ int nodals_gnn_evaluate(nodals_gnn* g, double* loss) {
   return guarded([&]() {
     const double l = gnnOf(g).evaluate();
     if (loss) *loss = l;
+  });
+}
+
+int nodals_h8_turb_init(nodals_h8* h, int argc, const char* const* argv) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(argc == 0 || argv != nullptr, "null argv");
+    s.turbInit(nodals::parseTurbArgs(argc, argv));
+  });
+}
+
+int nodals_h8_turb_step(nodals_h8* h, nodals_turb_step_result* res) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(res != nullptr, "null result");
+    const nodals::TurbStepResult r = s.turbStep();
+    res->k_clips = r.kClips;
+    res->omega_clips = r.omegaClips;
+    res->wall_invalid = r.wallInvalid;
+    res->reserved = 0;
+    res->ms = r.ms;
   });
 }
 

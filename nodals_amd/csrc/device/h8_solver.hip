@@ -521,7 +521,8 @@ void H8Solver::upload() {
   fval_.alloc((std::size_t)fineNnz_);
   fdiag_.alloc((std::size_t)nc_);
   fl1_.alloc((std::size_t)nc_);
-  fineLanes_ = lanesFor((double)fineNnz_ / std::max(nc_, 1));
+  fineMeanRow_ = (double)fineNnz_ / std::max(nc_, 1);
+  fineLanes_ = lanesFor(fineMeanRow_, nc_);
   // the refresh does ~1.5 dependent term loads per nonzero: wider groups
   refreshLanes_ = fineNnz_ > 48ll * nc_ ? 32 : 16;
   fineW_.alloc((std::size_t)nc_ * 24);
@@ -557,6 +558,7 @@ void H8Solver::setupAmg() {
     FineView<double, double> v;
     v.n = nc_;
     v.lanes = fineLanes_;
+    v.meanRow = fineMeanRow_;
     v.row = frow_.data();
     v.col = fcol_.data();
     v.val = fval_.data();
@@ -569,6 +571,7 @@ void H8Solver::setupAmg() {
     FineView<float, float> v;
     v.n = nc_;
     v.lanes = fineLanes_;
+    v.meanRow = fineMeanRow_;
     v.row = frow_.data();
     v.col = fcol_.data();
     v.val = fvalf_.data();
@@ -581,6 +584,7 @@ void H8Solver::setupAmg() {
     FineView<bf16, float> v;
     v.n = nc_;
     v.lanes = fineLanes_;
+    v.meanRow = fineMeanRow_;
     v.row = frow_.data();
     v.col = fcol_.data();
     v.val = fvalb_.data();
@@ -742,11 +746,11 @@ void H8Solver::refresh() {
 
 void H8Solver::fineApply(hipStream_t s, const double* x, double* y) {
   if (!(probeOn_ && s == stream_)) {
-    spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y);
+    schurSpmv(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, y);
     return;
   }
-  spmv<double, double, false, false>(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, nullptr, y,
-                                     probeEv_[2 * (std::size_t)probeUsed_], probeEv_[2 * (std::size_t)probeUsed_ + 1]);
+  schurSpmv(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, y,
+            probeEv_[2 * (std::size_t)probeUsed_], probeEv_[2 * (std::size_t)probeUsed_ + 1]);
   if (++probeUsed_ == kProbePairs) probeHarvest();
 }
 
@@ -918,6 +922,19 @@ void H8Solver::attachGnn(GnnModel* g, int warmMode, int recordEvery) {
   sync();
 }
 
+void H8Solver::sstGeometry() {
+  if (sstXY_.size() == (std::size_t)nc_ * 8) return;
+  std::vector<double> xy((std::size_t)nc_ * 8);
+  for (int c = 0; c < nc_; ++c)
+    for (int a = 0; a < 4; ++a) {
+      const auto& X = M_.points[(std::size_t)M_.tets[(std::size_t)c][(std::size_t)a]];
+      xy[(std::size_t)c * 8 + 2 * a] = X[0] - S_.pipe.cx;
+      xy[(std::size_t)c * 8 + 2 * a + 1] = X[1] - S_.pipe.cy;
+    }
+  sstXY_.upload(xy);
+  sstQuad_.upload(keast24());
+}
+
 void H8Solver::sstVolume(int equation, const std::vector<double>& kFree, const std::vector<double>& kFixed,
                          const std::vector<double>& wFree, const std::vector<double>& wFixed, double kFloor,
                          double omegaFloor, int deviatoric, std::vector<double>& mat, std::vector<double>& rhs) {
@@ -926,17 +943,7 @@ void H8Solver::sstVolume(int equation, const std::vector<double>& kFree, const s
   require((int)kFree.size() == nv_ && (int)wFree.size() == nv_ && (int)kFixed.size() == nFix_ &&
               (int)wFixed.size() == nFix_,
           "SST volume: k/omega sizes must be free DOFs and fixed slots");
-  if (sstXY_.size() != (std::size_t)nc_ * 8) {
-    std::vector<double> xy((std::size_t)nc_ * 8);
-    for (int c = 0; c < nc_; ++c)
-      for (int a = 0; a < 4; ++a) {
-        const auto& X = M_.points[(std::size_t)M_.tets[(std::size_t)c][(std::size_t)a]];
-        xy[(std::size_t)c * 8 + 2 * a] = X[0] - S_.pipe.cx;
-        xy[(std::size_t)c * 8 + 2 * a + 1] = X[1] - S_.pipe.cy;
-      }
-    sstXY_.upload(xy);
-    sstQuad_.upload(keast24());
-  }
+  sstGeometry();
   std::vector<double> kd((std::size_t)nv_), wd((std::size_t)nv_);
   for (int i = 0; i < nv_; ++i) {
     kd[(std::size_t)i] = kFree[(std::size_t)perm_[(std::size_t)i]];

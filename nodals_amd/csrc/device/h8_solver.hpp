@@ -17,9 +17,11 @@
 
 #include "../host/amg_setup.hpp"
 #include "../host/h8_setup.hpp"
+#include "../host/turb_setup.hpp"
 #include "amg.hpp"
 #include "gnn.hpp"
 #include "krylov.hpp"
+#include "turb_step.hpp"
 
 namespace nodals {
 
@@ -148,6 +150,20 @@ class H8Solver {
   // mode 1: prediction as the initial guess at its residual-optimal
   // multiple; mode 2: as an added first Krylov direction.
   void attachGnn(GnnModel* g, int warmMode, int recordEvery);
+  // k-omega SST transport step on the current velocity (masked-P1 state
+  // space; scalar DOFs are mesh entities, vertices first).  turbInit sets
+  // up the step and the inlet state (k, omega at their inlet values on every
+  // vertex, zero on face bubbles); turbStep advances k and then omega once.
+  void turbInit(const TurbParams& P);
+  TurbStepResult turbStep();
+  bool hasTurb() const { return (bool)turb_; }
+  // "k", "omega" (nS), "turb_val" (vertex operator values, relaxed),
+  // "turb_diag", "turb_rhs" (vertex rows) of the last equation assembled,
+  // topology arrays "turb_row", "turb_col", "turb_color", "turb_cell_ent",
+  // "turb_sample_cell", "turb_sample_l", "turb_heights", "turb_inlet_state"
+  // (kIn, omegaIn).
+  std::vector<double> turbField(const std::string& name);
+  void setTurbField(const std::string& name, const std::vector<double>& v);
   double* devScratchA() { return tmpA_.data(); }
   double* devScratchB() { return tmpB_.data(); }
   void sync();
@@ -183,6 +199,7 @@ class H8Solver {
 
   std::vector<int> perm_, pinv_;  // device numbering: perm_[new] = reference row
   int nc_ = 0, nv_ = 0, nFix_ = 0, fineLanes_ = 16, refreshLanes_ = 32;
+  double fineMeanRow_ = 8.0;
   long long momNnz_ = 0, fineNnz_ = 0;
   std::vector<int> colorOff_, colorSplit_;  // colour c: [off, split) long rows, [split, off+1) short rows
   // Momentum rows with more than kLongRow nonzeros (vertex rows) get
@@ -212,6 +229,10 @@ class H8Solver {
   double setupMs_ = 0;
   double chebLambda_ = 1.0;
   DBuf<double> sstXY_, sstQuad_;
+  void sstGeometry();
+  // One k (equation 0) or omega (1) update of the transport step.
+  int turbEquation(int equation);
+  std::unique_ptr<TurbState> turb_;
   GnnModel* gnn_ = nullptr;
   bool gnnWarm_ = false, gnnAsGuess_ = false;
   int gnnEvery_ = 0;

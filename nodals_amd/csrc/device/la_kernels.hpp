@@ -190,6 +190,25 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int* __restrict__ row
   }
 }
 
+// y = A x of the P0 pressure Schur operator, the Krylov operator of the
+// pressure solve (the body of k_spmv<LANES, double, double, false, false>
+// under its own name, so profiles and launch probes see it apart from the
+// AMG levels' products).
+template <int LANES>
+__global__ void __launch_bounds__(256) k_schur_spmv(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                    const double* __restrict__ val, const double* __restrict__ x,
+                                                    double* __restrict__ y) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int r = gid / LANES;
+  const int lane = gid % LANES;
+  if (r >= n) return;
+  double s = 0;
+  const int e = row[r + 1];
+  for (int k = row[r] + lane; k < e; k += LANES) s += val[k] * x[col[k]];
+  s = groupSum(s, LANES);
+  if (lane == 0) y[r] = s;
+}
+
 // Single-sweep Jacobi smoothing fused into the V-cycle residuals.
 // Pre-smoothing from zero and its residual in one pass:
 //   t = w b / d,   r = b - A t
@@ -303,16 +322,17 @@ __global__ void k_diag_gather(int n, const int* __restrict__ diagPos, const T* _
 }
 
 // ---------------------------------------------------------------- launch helpers
-// Lanes per CSR row by mean row length.  Measured on gfx950 (64^3 P1
-// Laplacian, 15 nnz/row; 40k-cell P0 Schur operator, 65 nnz/row): eight
-// lanes per row beat 16 and 32 for both, so rows up to 128 nonzeros share a
-// wavefront eight ways and loop; longer rows get wider groups.
-inline int lanesFor(double meanRow) {
-  if (meanRow <= 6) return 4;
-  if (meanRow <= 128) return 8;
-  if (meanRow <= 256) return 16;
-  if (meanRow <= 512) return 32;
-  return 64;
+// Lanes per CSR row from the mean row length and the number of rows.  Eight
+// lanes per row for rows up to 128 nonzeros (measured on gfx950: 64^3 P1
+// Laplacian, 15 nnz/row, and the 40k-cell P0 Schur operator, 65 nnz/row,
+// both beat 16 and 32 lanes), wider groups for longer rows.  When the rows
+// are too few to give the chip 2^17 threads (two waves per SIMD), the group
+// widens further while every lane keeps at least two nonzeros: a 1,073-row
+// coarse level with 88 nnz/row runs in 2.7 us at 32 lanes and 4.7 us at 8.
+inline int lanesFor(double meanRow, long long rows) {
+  int L = meanRow <= 6 ? 4 : meanRow <= 128 ? 8 : meanRow <= 256 ? 16 : meanRow <= 512 ? 32 : 64;
+  while (L < 64 && rows * L < (1LL << 17) && 4.0 * L <= meanRow) L *= 2;
+  return L;
 }
 
 template <class TM, class TV, bool RESID, bool ACC>
@@ -337,6 +357,26 @@ void spmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const
     default: NODALS_SPMV_LAUNCH(64) break;
   }
 #undef NODALS_SPMV_LAUNCH
+  NODALS_LAUNCHED();
+}
+
+inline void schurSpmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const double* val,
+                      const double* x, double* y, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
+  if (n <= 0) return;
+  const int grid = blocksFor((long long)n * lanes);
+#define NODALS_SCHUR_LAUNCH(LN)                                                                                  \
+  if (start)                                                                                                    \
+    hipExtLaunchKernelGGL((k_schur_spmv<LN>), dim3(grid), dim3(256), 0, s, start, stop, 0, n, row, col, val, x, y); \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_schur_spmv<LN>), grid, 256, 0, s, n, row, col, val, x, y);
+  switch (lanes) {
+    case 4: NODALS_SCHUR_LAUNCH(4) break;
+    case 8: NODALS_SCHUR_LAUNCH(8) break;
+    case 16: NODALS_SCHUR_LAUNCH(16) break;
+    case 32: NODALS_SCHUR_LAUNCH(32) break;
+    default: NODALS_SCHUR_LAUNCH(64) break;
+  }
+#undef NODALS_SCHUR_LAUNCH
   NODALS_LAUNCHED();
 }
 

@@ -83,12 +83,17 @@ __device__ inline void p1bf3Basis(const double l[4], double phi[8], double gr[8]
 // invJ (row-major dxi_j/dx_d), det, the (x, y) of the four vertices relative
 // to the pipe axis; fields: velocity (xyz interleaved, free and fixed) and
 // k, omega (free and fixed).  Outputs: mat[c][64], rhs[c][8].
-__global__ void __launch_bounds__(256) k_sst_volume(
+// Transport-step form: with sref (scalar DOF of every cell basis) k and
+// omega are read as kf[sref], wf[sref]; nuBase is added to the diffusion
+// coefficient (the molecular part, which Keast's rule integrates exactly
+// for this basis) and conv[c][64], when given, is added to the matrix.
+static __global__ void __launch_bounds__(256) k_sst_volume(
     int nc, const int* __restrict__ cref, const double* __restrict__ cinvJ, const double* __restrict__ cdet,
     const double* __restrict__ cxy, const double* __restrict__ u, const double* __restrict__ ufix,
     const double* __restrict__ kf, const double* __restrict__ kfix, const double* __restrict__ wf,
     const double* __restrict__ wfix, const double* __restrict__ quad, double radius, double nu, int equation,
-    double kFloor, double omFloor, int deviatoric, double* __restrict__ mat, double* __restrict__ rhs) {
+    double kFloor, double omFloor, int deviatoric, double* __restrict__ mat, double* __restrict__ rhs,
+    const int* __restrict__ sref = nullptr, double nuBase = 0.0, const double* __restrict__ conv = nullptr) {
   __shared__ double PH[4][kSstQ][8], GP[4][kSstQ][8][3], DW[4][kSstQ], RW[4][kSstQ], SW[4][kSstQ];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
@@ -104,8 +109,15 @@ __global__ void __launch_bounds__(256) k_sst_volume(
     for (int a = 0; a < 8; ++a) {
       const int r = cref[(long long)c * 8 + a];
       const double* U = r >= 0 ? u + 3ll * r : ufix + 3ll * (-r - 1);
-      const double Ka = r >= 0 ? kf[r] : kfix[-r - 1];
-      const double Wa = r >= 0 ? wf[r] : wfix[-r - 1];
+      double Ka, Wa;
+      if (sref) {
+        const int g = sref[(long long)c * 8 + a];
+        Ka = kf[g];
+        Wa = wf[g];
+      } else {
+        Ka = r >= 0 ? kf[r] : kfix[-r - 1];
+        Wa = r >= 0 ? wf[r] : wfix[-r - 1];
+      }
       const double pa = phi[a];
       PH[wv][q][a] = pa;
       kv += Ka * pa;
@@ -129,7 +141,7 @@ __global__ void __launch_bounds__(256) k_sst_volume(
                                     deviatoric != 0);
     const double wt = cdet[c] * Q[4];
     const bool isK = equation == 0;
-    DW[wv][q] = wt * (isK ? s.sigmaK : s.sigmaW) * s.nut;
+    DW[wv][q] = wt * ((isK ? s.sigmaK : s.sigmaW) * s.nut + nuBase);
     RW[wv][q] = wt * (isK ? s.reactK : s.reactOmega);
     SW[wv][q] = wt * (isK ? s.pk : s.srcOmega);
   }
@@ -143,7 +155,7 @@ __global__ void __launch_bounds__(256) k_sst_volume(
                         GP[wv][q][a][2] * GP[wv][q][b][2];
       z += DW[wv][q] * gd + RW[wv][q] * PH[wv][q][a] * PH[wv][q][b];
     }
-    mat[(long long)c * 64 + lane] = z;
+    mat[(long long)c * 64 + lane] = conv ? z + conv[(long long)c * 64 + lane] : z;
   }
   if (lane < 8) {
     double z = 0.0;

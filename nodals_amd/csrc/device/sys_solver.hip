@@ -82,7 +82,8 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
   NODALS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   red_.init(32);
   n_ = n;
-  lanes_ = lanesFor((double)nnz / n);
+  meanRow_ = (double)nnz / n;
+  lanes_ = lanesFor(meanRow_, n);
   std::vector<int> r32(A_.row.begin(), A_.row.end());
   row_.upload(r32);
   col_.upload(A_.col);
@@ -109,6 +110,7 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
     FineView<double, double> v;
     v.n = n;
     v.lanes = lanes_;
+    v.meanRow = meanRow_;
     v.row = row_.data();
     v.col = col_.data();
     v.val = val_.data();
@@ -129,6 +131,8 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
       FineView<float, float> v;
       v.n = n;
       v.lanes = lanes_;
+      v.meanRow = meanRow_;
+    v.meanRow = meanRow_;
       v.row = row_.data();
       v.col = col_.data();
       v.val = valf_.data();
@@ -144,6 +148,8 @@ SysSolver::SysSolver(int n, const std::int64_t* row, const std::int32_t* col, co
       FineView<bf16, float> v;
       v.n = n;
       v.lanes = lanes_;
+      v.meanRow = meanRow_;
+    v.meanRow = meanRow_;
       v.row = row_.data();
       v.col = col_.data();
       v.val = valb_.data();

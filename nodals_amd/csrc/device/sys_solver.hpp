@@ -51,6 +51,7 @@ class SysSolver {
   CsrD A_;
   AmgHierarchy H_;
   int n_ = 0, lanes_ = 8;
+  double meanRow_ = 8.0;
   double setupHostMs_ = 0;
   hipStream_t stream_ = nullptr;
   dev::Reducer red_;

@@ -1,0 +1,281 @@
+// k-omega SST transport step of an H8 context (see turb_step.hpp).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "h8_kernels.hpp"
+#include "h8_solver.hpp"
+#include "turb_kernels.hpp"
+
+namespace nodals {
+
+using namespace dev;
+
+TurbParams parseTurbArgs(int argc, const char* const* argv) {
+  TurbParams P;
+  auto num = [](const std::string& flag, const char* v) {
+    char* end = nullptr;
+    const double x = std::strtod(v, &end);
+    require(end && *end == '\0' && std::isfinite(x), "invalid value for " + flag + ": " + v);
+    return x;
+  };
+  for (int i = 0; i < argc; ++i) {
+    const std::string f = argv[i];
+    require(f.rfind("--", 0) == 0 && i + 1 < argc, "expected --flag value, got " + f);
+    const char* v = argv[++i];
+    if (f == "--sst-intensity") P.intensity = num(f, v);
+    else if (f == "--sst-length-ratio") P.lengthRatio = num(f, v);
+    else if (f == "--sst-alpha-k") P.alphaK = num(f, v);
+    else if (f == "--sst-alpha-omega") P.alphaOmega = num(f, v);
+    else if (f == "--sst-k-fgs-sweeps") P.kSweeps = (int)num(f, v);
+    else if (f == "--sst-omega-fgs-sweeps") P.omegaSweeps = (int)num(f, v);
+    else if (f == "--sst-gs-omega") P.gsOmega = num(f, v);
+    else if (f == "--sst-k-floor") P.kFloor = num(f, v);
+    else if (f == "--sst-omega-floor") P.omegaFloor = num(f, v);
+    else if (f == "--sst-deviatoric-production") P.deviatoric = num(f, v) != 0.0;
+    else if (f == "--sst-omega-wall-mode") {
+      const std::string m = v;
+      require(m == "of_auto" || m == "none", "--sst-omega-wall-mode must be of_auto or none");
+      P.wall = m == "of_auto";
+    } else if (f == "--sst-wall-penalty-gamma") P.wallGamma = num(f, v);
+    else if (f.size() == 20 && f.rfind("--sst-profile-fit-y", 0) == 0 && f[19] >= '1' && f[19] <= '5')
+      P.heights[(std::size_t)(f[19] - '1')] = num(f, v);
+    else if (f == "--sst-wall-sample-scale") P.sampleScale = num(f, v);
+    else if (f == "--sst-of-kappa") P.ofKappa = num(f, v);
+    else if (f == "--sst-of-E") P.ofE = num(f, v);
+    else if (f == "--sst-of-beta1") P.ofBeta1 = num(f, v);
+    else if (f == "--sst-of-re-blend") P.ofReBlend = num(f, v);
+    else if (f == "--sst-of-production-limit") P.ofProdLimit = num(f, v);
+    else fail("unknown k-omega option " + f);
+  }
+  require(P.gsOmega > 0 && P.gsOmega < 2, "--sst-gs-omega must be in (0, 2)");
+  require(P.wallGamma > 0, "--sst-wall-penalty-gamma must be positive");
+  require(P.intensity > 0 && P.lengthRatio > 0, "--sst-intensity and --sst-length-ratio must be positive");
+  const bool anyY = P.heights[0] > 0 || P.heights[1] > 0 || P.heights[2] > 0 || P.heights[3] > 0 || P.heights[4] > 0;
+  if (anyY)
+    for (int m = 0; m < 5; ++m)
+      require(P.heights[(std::size_t)m] > 0 && (m == 0 || P.heights[(std::size_t)m] > P.heights[(std::size_t)m - 1]),
+              "the five --sst-profile-fit-y heights must all be given, positive and increasing");
+  return P;
+}
+
+namespace {
+
+Tri12 tri12() {
+  Tri12 Q;
+  for (int q = 0; q < 12; ++q)
+    for (int j = 0; j < 4; ++j) Q.q[q][j] = kTri12[q][j];
+  return Q;
+}
+
+}  // namespace
+
+void H8Solver::turbInit(const TurbParams& P) {
+  auto st = std::make_unique<TurbState>();
+  st->P = P;
+  st->T = buildTurbTopo(M_, S_, P);
+  if (!hasDevice()) {  // host-only context: topology only
+    turb_ = std::move(st);
+    return;
+  }
+  sstGeometry();
+  const TurbTopo& T = st->T;
+  st->sref.upload(T.cellEnt);
+  st->row.upload(T.row);
+  st->col.upload(T.col);
+  st->diagPos.upload(T.diagPos);
+  st->entOff.upload(T.entOff);
+  st->termOff.upload(T.termOff);
+  st->term.upload(T.term);
+  st->rhsOff.upload(T.rhsOff);
+  st->rhsTerm.upload(T.rhsTerm);
+  st->crows.upload(T.coloring.rows);
+  st->colorOff = T.coloring.offset;
+  std::vector<int> fc, fl;
+  std::vector<double> fg;
+  for (const TurbFace& f : T.inlet) {
+    fc.push_back(f.cell);
+    fl.push_back(f.lf);
+    fg.insert(fg.end(), {f.n[0], f.n[1], f.n[2], f.area});
+  }
+  if (fc.empty()) {
+    fc.push_back(0);
+    fl.push_back(0);
+    fg.assign(4, 0.0);
+  }
+  st->fcell.upload(fc);
+  st->flf.upload(fl);
+  st->fgeom.upload(fg);
+  std::vector<double> wg, sl, sp;
+  std::vector<int> sc;
+  for (const TurbFace& f : T.wall) wg.insert(wg.end(), {f.n[0], f.n[1], f.n[2], f.area});
+  for (const TurbSample& s : T.samples) {
+    sc.push_back(s.cell);
+    sl.insert(sl.end(), s.l, s.l + 4);
+    sp.insert(sp.end(), s.phi, s.phi + 8);
+  }
+  if (sc.empty()) {
+    wg.assign(4, 0.0);
+    sc.push_back(0);
+    sl.assign(4, 0.0);
+    sp.assign(8, 0.0);
+  }
+  st->wgeom.upload(wg);
+  st->scell.upload(sc);
+  st->sl.upload(sl);
+  st->sphi.upload(sp);
+  st->coef.alloc((std::size_t)std::max(T.nWallQ, 1));
+  st->target.alloc((std::size_t)std::max(T.nWallQ, 1));
+  st->conv.alloc((std::size_t)nc_ * 64);
+  const std::size_t nb = (std::size_t)nc_ + T.nInlet + T.nWallQ;
+  st->blk.alloc(nb * 64);
+  st->brhs.alloc(nb * 8);
+  st->val.alloc(T.col.size());
+  st->diag.alloc((std::size_t)T.nvert);
+  st->rhs.alloc((std::size_t)T.nvert);
+  std::vector<double> k0((std::size_t)T.nS, 0.0), w0((std::size_t)T.nS, 0.0);
+  for (int i = 0; i < T.nvert; ++i) {
+    k0[(std::size_t)i] = T.kIn;
+    w0[(std::size_t)i] = T.omegaIn;
+  }
+  st->k.upload(k0);
+  st->w.upload(w0);
+  st->counters.upload(std::vector<int>(4, 0));
+  for (hipEvent_t& e : st->ev) NODALS_HIP(hipEventCreate(&e));
+  turb_ = std::move(st);
+}
+
+int H8Solver::turbEquation(int eq) {
+  TurbState& st = *turb_;
+  const TurbTopo& T = st.T;
+  const TurbParams& P = st.P;
+  hipStream_t s = stream_;
+  const TurbConsts K{S_.pipe.R, S_.pipe.nu, P.kFloor, P.omegaFloor, P.deviatoric ? 1 : 0, eq};
+  double* x = eq == 0 ? st.k.data() : st.w.data();
+  // element blocks: volume terms with the molecular diffusion, plus convection
+  hipLaunchKernelGGL(k_sst_volume, (nc_ + 3) / 4, 256, 0, s, nc_, cref_.data(), cinvJ_.data(), cdet_.data(),
+                     sstXY_.data(), u_.data(), fixedVal_.data(), st.k.data(), st.k.data(), st.w.data(), st.w.data(),
+                     sstQuad_.data(), S_.pipe.R, S_.pipe.nu, eq, P.kFloor, P.omegaFloor, P.deviatoric ? 1 : 0,
+                     st.blk.data(), st.brhs.data(), st.sref.data(), S_.pipe.nu, st.conv.data());
+  NODALS_LAUNCHED();
+  if (T.nInlet > 0) {
+    hipLaunchKernelGGL(k_turb_inlet, (T.nInlet + 3) / 4, 256, 0, s, T.nInlet, st.fcell.data(), st.flf.data(),
+                       st.fgeom.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(), u_.data(),
+                       fixedVal_.data(), st.k.data(), st.w.data(), K, tri12(), S_.pipe.inletVelocity[0],
+                       S_.pipe.inletVelocity[1], S_.pipe.inletVelocity[2], eq == 0 ? T.kIn : T.omegaIn,
+                       st.blk.data() + (std::size_t)nc_ * 64, st.brhs.data() + (std::size_t)nc_ * 8);
+    NODALS_LAUNCHED();
+  }
+  int nb = nc_ + T.nInlet;
+  if (eq == 1 && T.nWallQ > 0) {
+    WallConsts W{P.wallGamma, P.ofKappa, P.ofE, P.ofBeta1, P.ofReBlend, P.ofProdLimit, {0, 0, 0, 0, 0}};
+    for (int m = 0; m < 5; ++m) W.y[m] = T.heights[(std::size_t)m];
+    hipLaunchKernelGGL(k_turb_wall_coef, blocksFor(T.nWallQ), 256, 0, s, T.nWallQ, st.wgeom.data(), st.scell.data(),
+                       st.sl.data(), st.sphi.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(),
+                       u_.data(), fixedVal_.data(), st.k.data(), st.w.data(), K, W, tri12(), st.coef.data(),
+                       st.target.data(), st.counters.data() + 2);
+    NODALS_LAUNCHED();
+    hipLaunchKernelGGL(k_turb_wall_block, (T.nWallQ + 3) / 4, 256, 0, s, T.nWallQ, st.sphi.data(), st.coef.data(),
+                       st.target.data(), st.blk.data() + (std::size_t)nb * 64, st.brhs.data() + (std::size_t)nb * 8);
+    NODALS_LAUNCHED();
+    nb += T.nWallQ;
+  }
+  const double alpha = eq == 0 ? P.alphaK : P.alphaOmega;
+  hipLaunchKernelGGL(k_turb_gather<8>, blocksFor((long long)T.nvert * 8), 256, 0, s, T.nvert, st.row.data(),
+                     st.diagPos.data(), st.entOff.data(), st.termOff.data(), st.term.data(), st.rhsOff.data(),
+                     st.rhsTerm.data(), nb * 64, nb * 8, st.blk.data(), st.brhs.data(), x, alpha, st.val.data(),
+                     st.diag.data(), st.rhs.data(), st.counters.data() + 3);
+  NODALS_LAUNCHED();
+  const double meanRow = (double)T.col.size() / std::max(T.nvert, 1);
+  const int sweeps = eq == 0 ? P.kSweeps : P.omegaSweeps;
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int c = 0; c < T.coloring.ncolors; ++c) {
+      const int b = st.colorOff[(std::size_t)c], n = st.colorOff[(std::size_t)c + 1] - b;
+      mcgsColor<double, double>(s, lanesFor(meanRow, n), b, n, st.crows.data(), st.row.data(), st.col.data(),
+                                st.val.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
+    }
+  hipLaunchKernelGGL(k_turb_floor, blocksFor(T.nvert), 256, 0, s, T.nvert, eq == 0 ? P.kFloor : P.omegaFloor, x,
+                     st.counters.data() + eq);
+  NODALS_LAUNCHED();
+  return nb;
+}
+
+TurbStepResult H8Solver::turbStep() {
+  requireDevice();
+  require((bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
+  TurbState& st = *turb_;
+  hipStream_t s = stream_;
+  hipLaunchKernelGGL(k_fill<int>, 1, 256, 0, s, 4ll, st.counters.data(), 0);
+  NODALS_LAUNCHED();
+  NODALS_HIP(hipEventRecord(st.ev[0], s));
+  const int waves = std::min(nc_, 8192);
+  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, s, nc_, cref_.data(), cinvJ_.data(), cdet_.data(), u_.data(),
+                     fixedVal_.data(), centT_.data(), st.conv.data());
+  NODALS_LAUNCHED();
+  turbEquation(0);
+  turbEquation(1);
+  NODALS_HIP(hipEventRecord(st.ev[1], s));
+  sync();
+  ++st.steps;
+  std::vector<int> cnt = st.counters.host();
+  TurbStepResult R;
+  R.kClips = cnt[0];
+  R.omegaClips = cnt[1];
+  R.wallInvalid = cnt[2];
+  float ms = 0;
+  NODALS_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]));
+  R.ms = ms;
+  require(cnt[3] == 0, "k-omega step: " + std::to_string(cnt[3]) + " relaxed rows with a non-positive or "
+                       "non-finite diagonal", kErrNonFinite);
+  return R;
+}
+
+std::vector<double> H8Solver::turbField(const std::string& n) {
+  require((bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
+  TurbState& st = *turb_;
+  const TurbTopo& T = st.T;
+  const bool devField = n == "k" || n == "omega" || n == "turb_val" || n == "turb_diag" || n == "turb_rhs";
+  if (devField) requireDevice();
+  if (hasDevice()) sync();
+  auto toD = [](const std::vector<int>& v) { return std::vector<double>(v.begin(), v.end()); };
+  if (n == "k") return st.k.host();
+  if (n == "omega") return st.w.host();
+  if (n == "turb_val") return st.val.host();
+  if (n == "turb_diag") return st.diag.host();
+  if (n == "turb_rhs") return st.rhs.host();
+  if (n == "turb_row") return toD(T.row);
+  if (n == "turb_col") return toD(T.col);
+  if (n == "turb_color") return toD(T.coloring.color);
+  if (n == "turb_cell_ent") return toD(T.cellEnt);
+  if (n == "turb_sample_cell") {
+    std::vector<double> v;
+    for (const TurbSample& s : T.samples) v.push_back(s.cell);
+    return v;
+  }
+  if (n == "turb_sample_l") {
+    std::vector<double> v;
+    for (const TurbSample& s : T.samples) v.insert(v.end(), s.l, s.l + 4);
+    return v;
+  }
+  if (n == "turb_heights") return std::vector<double>(T.heights.begin(), T.heights.end());
+  if (n == "turb_inlet_state") return {T.kIn, T.omegaIn, (double)T.nInlet, (double)T.wall.size(), (double)T.nS,
+                                       (double)T.nvert, (double)T.coloring.ncolors};
+  fail("unknown k-omega field " + n);
+}
+
+void H8Solver::setTurbField(const std::string& n, const std::vector<double>& v) {
+  require((bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
+  TurbState& st = *turb_;
+  requireDevice();
+  require(n == "k" || n == "omega", "settable k-omega fields: k, omega");
+  require((int)v.size() == st.T.nS, n + " must have one value per scalar DOF (vertices, then faces)");
+  for (int i = st.T.nvert; i < st.T.nS; ++i)
+    require(v[(std::size_t)i] == 0.0, n + ": face-bubble coefficients must be zero in the masked-P1 state space");
+  sync();
+  (n == "k" ? st.k : st.w).upload(v);
+}
+
+}  // namespace nodals

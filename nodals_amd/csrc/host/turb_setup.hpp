@@ -1,0 +1,94 @@
+// Host setup of the k-omega SST transport step on the P1+BF3 tetrahedral
+// space in the masked-P1 state space (vertex coefficients active, face-bubble
+// coefficients held at zero): the scalar DOF numbering, the vertex-row
+// operator pattern with its greedy colouring, the atomic-free gather maps of
+// the assembled rows, the inlet face plans and the wall-normal sample points
+// of the omega wall closure.
+//
+// Reference: turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/
+// modules/coupling/g2_step_k.inc, g2_step_omega.inc (step order),
+// modules/coupling/g2_state.inc:12-27 (inlet k/omega, q1 mask),
+// modules/transport/transport_preflight.inc + transport_finalize.inc
+// (assembly and row-L1 relaxation), scalar_linear_solver.inc:33-67 (fixed
+// coloured sweeps), tet_inlet_flux.inc (inlet flux), puretet_of_auto_point3.cuh
+// (omega wall penalty) and modules/wall/tet_wall_match_cache.inc:3-67 (sample
+// location by adjacency walk).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "h8_setup.hpp"
+#include "mesh.hpp"
+
+namespace nodals {
+
+struct TurbParams {
+  double intensity = 0.05, lengthRatio = 0.07;  // inlet k = 1.5 (I U)^2, omega from l = ratio * D
+  double alphaK = 0.3, alphaOmega = 0.3;        // row-L1 under-relaxation
+  int kSweeps = 20, omegaSweeps = 20;           // fixed coloured GS sweeps per equation
+  double gsOmega = 0.7;                         // sweep relaxation
+  double kFloor = 1e-8, omegaFloor = 1e-3;
+  bool deviatoric = false;                      // dev(2S):grad(U) production
+  bool wall = true;                             // omega wall penalty at the third sample
+  double wallGamma = 1e6;
+  // Wall-normal sample heights; zeros select the reference rule
+  // d_i * D / 0.004 * sampleScale (turbulence/sst_gpu_v2/nodals.py:289-292).
+  std::array<double, 5> heights{{0, 0, 0, 0, 0}};
+  double sampleScale = 0.5;
+  double ofKappa = 0.41, ofE = 9.8, ofBeta1 = 0.075, ofReBlend = 11.0, ofProdLimit = 10.0;
+};
+
+// A boundary face seen from its tetrahedron: local face lf is the face
+// opposite local vertex lf; n is the outward unit normal.
+struct TurbFace {
+  std::int32_t cell = -1, lf = -1;
+  double n[3] = {0, 0, 0};
+  double area = 0;
+};
+
+// One wall-normal sample: containing cell and its P1+BF3 basis values there.
+struct TurbSample {
+  std::int32_t cell = -1;
+  double l[4] = {0, 0, 0, 0};
+  double phi[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct TurbTopo {
+  int nS = 0, nvert = 0, nc = 0;
+  int nInlet = 0, nWallQ = 0;  // inlet faces, wall quadrature points (12 per wall face)
+  std::vector<std::int32_t> cellEnt;  // [nc][8] scalar DOF (mesh entity) of each basis
+  std::vector<TurbFace> inlet, wall;
+  std::vector<TurbSample> samples;    // [wall face][12][5]
+  // Operator of the vertex rows over vertex columns (sorted), in which the
+  // coloured sweeps run (bubble coefficients are zero, so bubble columns
+  // drop out of the sweeps but not of the row-L1 norms).
+  std::vector<std::int32_t> row, col, diagPos;
+  Coloring coloring;
+  // Gather maps.  Blocks: cells [0, nc), inlet faces [nc, nc + nInlet) (on
+  // the face's cell), wall quadrature points [nc + nInlet, nc + nInlet +
+  // nWallQ) (on the cell of their third sample).  Row i owns entries
+  // [entOff[i], entOff[i+1]): first its CSR columns in order, then its
+  // bubble columns ascending; entry e sums block values term[termOff[e] ..
+  // termOff[e+1]) (block*64 + 8a + b, ascending).  Its right-hand side sums
+  // rhsTerm[rhsOff[i] .. rhsOff[i+1]) (block*8 + a).
+  std::vector<std::int32_t> entOff, termOff, term, rhsOff, rhsTerm;
+  std::vector<std::int32_t> blockCell;  // cell of every block
+  double kIn = 0, omegaIn = 0;
+  std::array<double, 5> heights{{0, 0, 0, 0, 0}};  // sample heights in use
+};
+
+// Builds the topology for the H8 pipe (inlet and wall patches of S.pipe).
+// Throws nodals::Error if a wall sample lies outside the mesh.
+TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P);
+
+// Local vertices of face lf (the face opposite vertex lf) in the reference
+// order (modules/foundation/hxt4a_foundation.inc:37-39).
+extern const int kTetFace[4][3];
+// Twelve-point degree-6 triangle rule (Dunavant), barycentric coordinates on
+// the three face vertices and weights summing to one, as tabulated by the
+// reference (hxt5b_boundary.cuh:42-60).
+extern const double kTri12[12][4];
+
+}  // namespace nodals

@@ -77,9 +77,6 @@ def volume_terms(ref_sst, cell_ref, cell_invJ, cell_det, cell_xy, U_free, U_fixe
                  radius, nu, equation, k_floor=1e-8, omega_floor=1e-3, deviatoric=False):
     """Element matrices (nc,8,8) and vectors (nc,8).  U_free (3,nv),
     U_fixed (nfixed,3); cell_xy (nc,4,2) vertex coordinates about the axis."""
-    nc = cell_ref.shape[0]
-    Lq, Wq = keast24()
-    phi, gr = basis(Lq)                                   # (24,8), (24,8,3)
     ref = cell_ref
     free = ref >= 0
     slot = np.where(free, 0, -ref - 1)
@@ -87,6 +84,17 @@ def volume_terms(ref_sst, cell_ref, cell_invJ, cell_det, cell_xy, U_free, U_fixe
     Uc = np.where(free[..., None], U_free.T[idx], U_fixed[slot])      # (nc,8,3)
     Kc = np.where(free, k_free[idx], k_fixed[slot])
     Wc = np.where(free, w_free[idx], w_fixed[slot])
+    return volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius, nu, equation, k_floor,
+                              omega_floor, deviatoric)
+
+
+def volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius, nu, equation, k_floor=1e-8,
+                       omega_floor=1e-3, deviatoric=False):
+    """volume_terms from per-cell basis coefficients: Uc (nc,8,3), Kc and Wc
+    (nc,8)."""
+    nc = Uc.shape[0]
+    Lq, Wq = keast24()
+    phi, gr = basis(Lq)                                   # (24,8), (24,8,3)
     gp = np.einsum("qaj,cjd->cqad", gr, cell_invJ)                    # (nc,24,8,3)
     gu = np.einsum("cai,cqad->cqid", Uc, gp).reshape(nc, 24, 9)       # du_i/dx_d row-major
     kv = Kc @ phi.T                                                   # (nc,24)
