@@ -50,9 +50,10 @@ typedef struct {
   double setup_ms;
   /* Extensions (no reference counterpart): total pressure Krylov
    * iterations, learned warm starts tried, dropped (fallback), samples
-   * recorded. */
+   * recorded; device time of the k-omega steps of a coupled run. */
   int64_t pressure_its;
   int32_t warm_starts, fallbacks, recorded, reserved2;
+  double turb_ms;
 } nodals_h8_run_result;
 
 /* Last error message of the calling thread ("" if none). */
@@ -211,6 +212,14 @@ typedef struct nodals_turb_step_result {
 } nodals_turb_step_result;
 int nodals_h8_turb_init(nodals_h8* h, int argc, const char* const* argv);
 int nodals_h8_turb_step(nodals_h8* h, nodals_turb_step_result* res);
+/* Coupled k-omega SST mode of nodals_h8_run (on = 1; needs
+ * nodals_h8_turb_init): the momentum assembly adds the eddy-viscosity
+ * diffusion (nu_t of the SST local state at each Keast point, the isotropic
+ * part of sst_mom_tet, turbulence/sst_gpu_v2/source/
+ * puretet_profile5_eq5_hxt5b/hxt5d_sst_g2.cuh:1220) and every SIMPLE
+ * iteration ends with one k-omega step (the G2 order: momentum, pressure,
+ * turbulence, sst_application.inc:2818-2978). */
+int nodals_h8_turb_couple(nodals_h8* h, int32_t on);
 
 /* ---------------------------------------------------------------- GNN ---
  * Learned pressure-correction warm start (no reference counterpart; the

@@ -36,7 +36,7 @@ class RunResult(C.Structure):
                 ("momentum_ms", C.c_double), ("continuity_ms", C.c_double), ("pressure_ms", C.c_double),
                 ("pupdate_ms", C.c_double), ("setup_ms", C.c_double), ("pressure_its", C.c_int64),
                 ("warm_starts", C.c_int32), ("fallbacks", C.c_int32), ("recorded", C.c_int32),
-                ("reserved2", C.c_int32)]
+                ("reserved2", C.c_int32), ("turb_ms", C.c_double)]
 
     def as_dict(self) -> Dict[str, float]:
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
@@ -58,7 +58,7 @@ EXPORTS = [
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
     "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
-    "nodals_h8_turb_init", "nodals_h8_turb_step",
+    "nodals_h8_turb_init", "nodals_h8_turb_step", "nodals_h8_turb_couple",
 ]
 
 
@@ -112,6 +112,7 @@ def lib() -> C.CDLL:
     L.nodals_h8_sst_volume.argtypes = [vp, C.c_int32, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int32, vp, vp]
     L.nodals_h8_turb_init.argtypes = [vp, C.c_int, vp]
     L.nodals_h8_turb_step.argtypes = [vp, vp]
+    L.nodals_h8_turb_couple.argtypes = [vp, C.c_int32]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -255,6 +256,11 @@ class H8:
         st = self.get("turb_inlet_state", 7)
         self.turb = {"k_in": st[0], "omega_in": st[1], "inlet_faces": int(st[2]), "wall_faces": int(st[3]),
                      "ns": int(st[4]), "nvert": int(st[5]), "colors": int(st[6])}
+
+    def turb_couple(self, on: bool = True) -> None:
+        """Coupled SST mode of run(): eddy viscosity in the momentum
+        diffusion and one k-omega step after every pressure update."""
+        _check(lib().nodals_h8_turb_couple(self._h, 1 if on else 0))
 
     def turb_step(self) -> Dict[str, float]:
         r = TurbStepResult()

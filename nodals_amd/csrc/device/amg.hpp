@@ -133,20 +133,21 @@ class DevAmg {
     graph_.run(s, graphKey({b, z, fine_.val, fine_.diag}), [&]() { apply(s, b, z); });
   }
 
+  // z = M^{-1} b for FP64 vectors b, z, replayed from one captured graph per
+  // (b, z) pair: with the fused Jacobi cycle the FP64 <-> TV roundings happen
+  // inside its first and last kernels (identical values to separate casts);
+  // otherwise through the fixed TV buffers.  Returns false (nothing done)
+  // when the smoother is not the fused Jacobi cycle.
+  bool applyWide(hipStream_t s, const double* b, double* z) {
+    if (!fusedJacobi()) return false;
+    wide_.run(s, graphKey({b, z, fine_.val, fine_.diag}), [&]() { fusedCycle<double, double>(s, b, z); });
+    return true;
+  }
+
   // z = M^{-1} b on the finest level.
   void apply(hipStream_t s, const TV* b, TV* z) {
     if (fusedJacobi()) {
-      // pre-smoothing + residual, coarse correction, residual + post-
-      // smoothing: two fine passes instead of five kernels
-      const TV* den = opt_.kind == "l1jacobi" ? fine_.l1 : fine_.diag;
-      jacobiPreResid<TF, TV>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den, opt_.jacobiOmega,
-                             ftmp_.data(), fr_.data());
-      Level& C = L_[0];
-      restrictTo(s, P_[0], fr_.data(), C.b.data());
-      vcycle(s, 0, C.b.data(), C.x.data());
-      prolongAdd(s, P_[0], C.x.data(), ftmp_.data());
-      residJacobiPost<TF, TV>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den, opt_.jacobiOmega,
-                              ftmp_.data(), z);
+      fusedCycle<TV, TV>(s, b, z);
       return;
     }
     smoothFine(s, b, z);
@@ -161,6 +162,21 @@ class DevAmg {
   }
 
  private:
+  template <class TB, class TZ>
+  // Pre-smoothing + residual, coarse correction, residual + post-smoothing:
+  // two fine passes instead of five kernels.
+  void fusedCycle(hipStream_t s, const TB* b, TZ* z) {
+    const TV* den = opt_.kind == "l1jacobi" ? fine_.l1 : fine_.diag;
+    jacobiPreResid<TF, TV, TB>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den, opt_.jacobiOmega,
+                               ftmp_.data(), fr_.data());
+    Level& C = L_[0];
+    restrictTo(s, P_[0], fr_.data(), C.b.data());
+    vcycle(s, 0, C.b.data(), C.x.data());
+    prolongAdd(s, P_[0], C.x.data(), ftmp_.data());
+    residJacobiPost<TF, TV, TB, TZ>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den,
+                                    opt_.jacobiOmega, ftmp_.data(), z);
+  }
+
   template <class TS>
   static std::vector<TV> castVec(const std::vector<TS>& v) {
     std::vector<TV> o(v.size());
@@ -318,6 +334,7 @@ class DevAmg {
   int termN_ = 0;
   DBuf<TV> fr_, ftmp_, fcorr_;
   GraphCache graph_;
+  GraphSet wide_;
 };
 
 }  // namespace dev

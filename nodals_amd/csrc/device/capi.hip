@@ -187,6 +187,7 @@ int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res) {
     res->fallbacks = r.fallbacks;
     res->recorded = r.recorded;
     res->reserved2 = 0;
+    res->turb_ms = r.turbMs;
   });
 }
 
@@ -712,6 +713,10 @@ int nodals_h8_turb_step(nodals_h8* h, nodals_turb_step_result* res) {
     res->reserved = 0;
     res->ms = r.ms;
   });
+}
+
+int nodals_h8_turb_couple(nodals_h8* h, int32_t on) {
+  return guarded([&]() { solver(h).turbCouple(on != 0); });
 }
 
 int nodals_h8_attach_gnn(nodals_h8* h, nodals_gnn* g, int32_t warm_start, int32_t record_every) {

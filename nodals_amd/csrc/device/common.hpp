@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -138,6 +140,26 @@ class GraphCache {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   std::string key_;
+};
+
+// Several captured sequences keyed by their arguments (a V-cycle replayed
+// for each (input, output) vector pair of a Krylov basis); at most kMax are
+// kept, all dropped when a new key would exceed that.
+class GraphSet {
+ public:
+  template <class F>
+  void run(hipStream_t s, const std::string& key, F&& body) {
+    auto it = set_.find(key);
+    if (it == set_.end()) {
+      if (set_.size() >= kMax) set_.clear();
+      it = set_.emplace(key, std::make_unique<GraphCache>()).first;
+    }
+    it->second->run(s, key, std::forward<F>(body));
+  }
+
+ private:
+  static constexpr std::size_t kMax = 96;
+  std::map<std::string, std::unique_ptr<GraphCache>> set_;
 };
 
 inline std::string graphKey(std::initializer_list<const void*> ptrs, double scalar = 0.0) {

@@ -76,6 +76,7 @@ struct H8RunResult {
   double setupMs = 0;
   long long pressureIts = 0;  // total Krylov iterations of the run
   int warmStarts = 0, fallbacks = 0, recorded = 0;
+  double turbMs = 0;          // k-omega steps of a coupled run
 };
 
 class H8Solver {
@@ -157,6 +158,10 @@ class H8Solver {
   void turbInit(const TurbParams& P);
   TurbStepResult turbStep();
   bool hasTurb() const { return (bool)turb_; }
+  // Coupled mode: the momentum assembly adds the eddy-viscosity diffusion
+  // of the current k, omega (SST nu_t at every Keast point) and run()
+  // advances k and omega once after every pressure update.
+  void turbCouple(bool on);
   // "k", "omega" (nS), "turb_val" (vertex operator values, relaxed),
   // "turb_diag", "turb_rhs" (vertex rows) of the last equation assembled,
   // topology arrays "turb_row", "turb_col", "turb_color", "turb_cell_ent",
@@ -232,7 +237,11 @@ class H8Solver {
   void sstGeometry();
   // One k (equation 0) or omega (1) update of the transport step.
   int turbEquation(int equation);
+  void turbAdvance();       // convection + k + omega, queued on the stream
+  void eddyDiffusion();     // adds the nu_t diffusion blocks to cbuf_
+  int turbCounters(TurbStepResult* R);  // reads the step's counters (syncs)
   std::unique_ptr<TurbState> turb_;
+  bool turbCouple_ = false;
   GnnModel* gnn_ = nullptr;
   bool gnnWarm_ = false, gnnAsGuess_ = false;
   int gnnEvery_ = 0;

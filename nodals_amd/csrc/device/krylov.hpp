@@ -151,55 +151,65 @@ static __global__ void __launch_bounds__(256) k_gs_update_scale(long long n, int
 // to host memory, the scaling 1/h_{j+1,j} to device memory.
 // One 256-thread block: it first finishes the second pass's reduction
 // (h2[q] = sum of nparts block partials, q <= j+1, one wavefront per output
-// in turn, fixed order), then thread 0 does the Givens step.
+// in turn, fixed order) while other lanes stage h1, the previous rotations
+// and g_j in LDS; thread 0 then does the Givens step on LDS operands (a
+// serial chain of global-memory round trips otherwise), and the column is
+// written back by j+1 lanes at once.
 static __global__ void __launch_bounds__(256) k_arnoldi_givens(int j, int nparts, const double* __restrict__ part,
                                                                 double* __restrict__ st, double* __restrict__ host) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ double h2s[34], hcol[34], css[32], sns[32], gj;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int q = wv; q <= j + 1; q += 4) {
     double v = 0.0;
     for (int i = lane; i < nparts; i += 64) v += part[(long long)q * nparts + i];
     v = groupSum(v, 64);
-    if (lane == 0) st[fg::kH2 + q] = v;
+    if (lane == 0) {
+      st[fg::kH2 + q] = v;
+      h2s[q] = v;
+    }
+  }
+  if (t <= j) hcol[t] = st[fg::kH1 + t];
+  if (t < j) {
+    css[t] = st[fg::kCs + t];
+    sns[t] = st[fg::kSn + t];
+  }
+  if (t == 255) gj = st[fg::kG + j];
+  __syncthreads();
+  if (t == 0) {
+    double hh = 0.0;
+    for (int i = 0; i <= j; ++i) {
+      hcol[i] = hcol[i] + h2s[i];
+      hh += h2s[i] * h2s[i];
+    }
+    const double d2 = h2s[j + 1] - hh;
+    const double hn = sqrt(d2 > 0.0 ? d2 : 0.0);
+    hcol[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+      const double a = hcol[i], b = hcol[i + 1];
+      hcol[i] = css[i] * a + sns[i] * b;
+      hcol[i + 1] = -sns[i] * a + css[i] * b;
+    }
+    const double a = hcol[j], b = hcol[j + 1];
+    const double d = hypot(a, b);
+    const bool bad = !(d > 0.0) || !isfinite(d) || !isfinite(hn);
+    const double c = bad ? 0.0 : a / d, sgn = bad ? 0.0 : b / d;
+    st[fg::kCs + j] = c;
+    st[fg::kSn + j] = sgn;
+    hcol[j] = d;
+    hcol[j + 1] = 0.0;
+    const double g1 = -sgn * gj, g0 = c * gj;
+    st[fg::kG + j + 1] = g1;
+    st[fg::kG + j] = g0;
+    st[fg::kScal] = hn > 0.0 ? 1.0 / hn : 0.0;
+    host[fg::hG + j] = g0;
+    host[fg::hG + j + 1] = g1;
+    host[fg::hZero + j] = hn == 0.0 ? 1.0 : 0.0;
+    host[fg::hBad + j] = bad ? 1.0 : 0.0;
+    host[fg::hEst + j] = fabs(g1);
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double* H = st + fg::kH;
-  double* cs = st + fg::kCs;
-  double* sn = st + fg::kSn;
-  double* g = st + fg::kG;
-  const double* h1 = st + fg::kH1;
-  const double* h2 = st + fg::kH2;
-  double hh = 0.0;
-  for (int i = 0; i <= j; ++i) {
-    H[i * 32 + j] = h1[i] + h2[i];
-    hh += h2[i] * h2[i];
-  }
-  const double d2 = h2[j + 1] - hh;
-  const double hn = sqrt(d2 > 0.0 ? d2 : 0.0);
-  H[(j + 1) * 32 + j] = hn;
-  for (int i = 0; i < j; ++i) {
-    const double a = H[i * 32 + j], b = H[(i + 1) * 32 + j];
-    H[i * 32 + j] = cs[i] * a + sn[i] * b;
-    H[(i + 1) * 32 + j] = -sn[i] * a + cs[i] * b;
-  }
-  const double a = H[j * 32 + j], b = H[(j + 1) * 32 + j];
-  const double d = hypot(a, b);
-  const bool bad = !(d > 0.0) || !isfinite(d) || !isfinite(hn);
-  const double c = bad ? 0.0 : a / d, sgn = bad ? 0.0 : b / d;
-  cs[j] = c;
-  sn[j] = sgn;
-  H[j * 32 + j] = d;
-  H[(j + 1) * 32 + j] = 0.0;
-  const double gj = g[j];
-  g[j + 1] = -sgn * gj;
-  g[j] = c * gj;
-  st[fg::kScal] = hn > 0.0 ? 1.0 / hn : 0.0;
-  for (int i = 0; i <= j; ++i) host[fg::hH + i * 32 + j] = H[i * 32 + j];
-  host[fg::hG + j] = g[j];
-  host[fg::hG + j + 1] = g[j + 1];
-  host[fg::hZero + j] = hn == 0.0 ? 1.0 : 0.0;
-  host[fg::hBad + j] = bad ? 1.0 : 0.0;
-  host[fg::hEst + j] = fabs(g[j + 1]);
+  if (t <= j + 1) st[fg::kH + t * 32 + j] = hcol[t];
+  if (t <= j) host[fg::hH + t * 32 + j] = hcol[t];
 }
 
 inline KrylovResult pcg(hipStream_t s, Reducer& R, int n, const DevOp& A, const DevOp& M, const double* rhs, double rtol,

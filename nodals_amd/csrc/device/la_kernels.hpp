@@ -214,10 +214,13 @@ __global__ void __launch_bounds__(256) k_schur_spmv(int n, const int* __restrict
 //   t = w b / d,   r = b - A t
 // (t_j of every neighbour is formed from b_j and d_j as the separate
 // smoother would, so the values equal the unfused sequence).
-template <int LANES, class TM, class TV>
+// b may be stored wider than TV (an FP64 Krylov vector entering an FP32
+// hierarchy): it is rounded to TV on load, exactly as a separate cast pass
+// would round it.
+template <int LANES, class TM, class TV, class TB = TV>
 __global__ void __launch_bounds__(256) k_jacobi_pre_resid(int n, const int* __restrict__ row,
                                                           const int* __restrict__ col, const TM* __restrict__ val,
-                                                          const TV* __restrict__ b, const TV* __restrict__ d,
+                                                          const TB* __restrict__ b, const TV* __restrict__ d,
                                                           double omega, TV* __restrict__ t, TV* __restrict__ r) {
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int i = gid / LANES;
@@ -227,23 +230,25 @@ __global__ void __launch_bounds__(256) k_jacobi_pre_resid(int n, const int* __re
   const int e = row[i + 1];
   for (int k = row[i] + lane; k < e; k += LANES) {
     const int j = col[k];
-    const TV tj = (TV)omega * b[j] / d[j];
+    const TV tj = (TV)omega * (TV)b[j] / d[j];
     s += ld<TV>(val, k) * tj;
   }
   s = groupSum(s, LANES);
   if (lane == 0) {
-    t[i] = (TV)omega * b[i] / d[i];
-    r[i] = b[i] - s;
+    const TV bi = (TV)b[i];
+    t[i] = (TV)omega * bi / d[i];
+    r[i] = bi - s;
   }
 }
 // Residual and post-smoothing correction in one pass:
 //   z = t + w (b - A t) / d      (z and t distinct)
-template <int LANES, class TM, class TV>
+// (b as above; z may be stored wider than TV and receives the TV result.)
+template <int LANES, class TM, class TV, class TB = TV, class TZ = TV>
 __global__ void __launch_bounds__(256) k_resid_jacobi_post(int n, const int* __restrict__ row,
                                                            const int* __restrict__ col, const TM* __restrict__ val,
-                                                           const TV* __restrict__ b, const TV* __restrict__ d,
+                                                           const TB* __restrict__ b, const TV* __restrict__ d,
                                                            double omega, const TV* __restrict__ t,
-                                                           TV* __restrict__ z) {
+                                                           TZ* __restrict__ z) {
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int i = gid / LANES;
   const int lane = gid % LANES;
@@ -253,8 +258,8 @@ __global__ void __launch_bounds__(256) k_resid_jacobi_post(int n, const int* __r
   for (int k = row[i] + lane; k < e; k += LANES) s += ld<TV>(val, k) * t[col[k]];
   s = groupSum(s, LANES);
   if (lane == 0) {
-    const TV r = b[i] - s;
-    z[i] = t[i] + (TV)omega * r / d[i];
+    const TV r = (TV)b[i] - s;
+    z[i] = (TZ)(t[i] + (TV)omega * r / d[i]);
   }
 }
 
@@ -380,13 +385,13 @@ inline void schurSpmv(hipStream_t s, int lanes, int n, const int* row, const int
   NODALS_LAUNCHED();
 }
 
-template <class TM, class TV>
-void jacobiPreResid(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TV* b,
+template <class TM, class TV, class TB = TV>
+void jacobiPreResid(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TB* b,
                     const TV* d, double omega, TV* t, TV* r) {
   if (n <= 0) return;
   const int grid = blocksFor((long long)n * lanes);
 #define NODALS_JPR(LN) \
-  hipLaunchKernelGGL((k_jacobi_pre_resid<LN, TM, TV>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, r)
+  hipLaunchKernelGGL((k_jacobi_pre_resid<LN, TM, TV, TB>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, r)
   switch (lanes) {
     case 4: NODALS_JPR(4); break;
     case 8: NODALS_JPR(8); break;
@@ -398,13 +403,13 @@ void jacobiPreResid(hipStream_t s, int lanes, int n, const int* row, const int* 
   NODALS_LAUNCHED();
 }
 
-template <class TM, class TV>
-void residJacobiPost(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TV* b,
-                     const TV* d, double omega, const TV* t, TV* z) {
+template <class TM, class TV, class TB = TV, class TZ = TV>
+void residJacobiPost(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TB* b,
+                     const TV* d, double omega, const TV* t, TZ* z) {
   if (n <= 0) return;
   const int grid = blocksFor((long long)n * lanes);
 #define NODALS_RJP(LN) \
-  hipLaunchKernelGGL((k_resid_jacobi_post<LN, TM, TV>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, z)
+  hipLaunchKernelGGL((k_resid_jacobi_post<LN, TM, TV, TB, TZ>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, z)
   switch (lanes) {
     case 4: NODALS_RJP(4); break;
     case 8: NODALS_RJP(8); break;

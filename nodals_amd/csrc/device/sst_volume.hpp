@@ -86,14 +86,17 @@ __device__ inline void p1bf3Basis(const double l[4], double phi[8], double gr[8]
 // Transport-step form: with sref (scalar DOF of every cell basis) k and
 // omega are read as kf[sref], wf[sref]; nuBase is added to the diffusion
 // coefficient (the molecular part, which Keast's rule integrates exactly
-// for this basis) and conv[c][64], when given, is added to the matrix.
+// for this basis) and conv[c][64], when given, is added to the matrix (conv
+// may be mat itself: every lane reads and writes only its own entry).
+// equation 2 is the momentum form: the eddy-viscosity diffusion block
+// sum_q w_q det nu_t grad phi_a . grad phi_b only (no reaction, no rhs).
 static __global__ void __launch_bounds__(256) k_sst_volume(
     int nc, const int* __restrict__ cref, const double* __restrict__ cinvJ, const double* __restrict__ cdet,
     const double* __restrict__ cxy, const double* __restrict__ u, const double* __restrict__ ufix,
     const double* __restrict__ kf, const double* __restrict__ kfix, const double* __restrict__ wf,
     const double* __restrict__ wfix, const double* __restrict__ quad, double radius, double nu, int equation,
-    double kFloor, double omFloor, int deviatoric, double* __restrict__ mat, double* __restrict__ rhs,
-    const int* __restrict__ sref = nullptr, double nuBase = 0.0, const double* __restrict__ conv = nullptr) {
+    double kFloor, double omFloor, int deviatoric, double* mat, double* __restrict__ rhs,
+    const int* __restrict__ sref = nullptr, double nuBase = 0.0, const double* conv = nullptr) {
   __shared__ double PH[4][kSstQ][8], GP[4][kSstQ][8][3], DW[4][kSstQ], RW[4][kSstQ], SW[4][kSstQ];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
@@ -141,9 +144,15 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
                                     deviatoric != 0);
     const double wt = cdet[c] * Q[4];
     const bool isK = equation == 0;
-    DW[wv][q] = wt * ((isK ? s.sigmaK : s.sigmaW) * s.nut + nuBase);
-    RW[wv][q] = wt * (isK ? s.reactK : s.reactOmega);
-    SW[wv][q] = wt * (isK ? s.pk : s.srcOmega);
+    if (equation == 2) {
+      DW[wv][q] = wt * s.nut;
+      RW[wv][q] = 0.0;
+      SW[wv][q] = 0.0;
+    } else {
+      DW[wv][q] = wt * ((isK ? s.sigmaK : s.sigmaW) * s.nut + nuBase);
+      RW[wv][q] = wt * (isK ? s.reactK : s.reactOmega);
+      SW[wv][q] = wt * (isK ? s.pk : s.srcOmega);
+    }
   }
   __syncthreads();
   if (!live) return;
@@ -157,7 +166,7 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
     }
     mat[(long long)c * 64 + lane] = conv ? z + conv[(long long)c * 64 + lane] : z;
   }
-  if (lane < 8) {
+  if (rhs && lane < 8) {
     double z = 0.0;
     for (int q = 0; q < kSstQ; ++q) z += SW[wv][q] * PH[wv][q][lane];
     rhs[(long long)c * 8 + lane] = z;

@@ -189,48 +189,81 @@ int H8Solver::turbEquation(int eq) {
                      st.rhsTerm.data(), nb * 64, nb * 8, st.blk.data(), st.brhs.data(), x, alpha, st.val.data(),
                      st.diag.data(), st.rhs.data(), st.counters.data() + 3);
   NODALS_LAUNCHED();
+  // the fixed sweeps (sweeps x colours launches on fixed buffers) replay
+  // from one captured graph per equation
   const double meanRow = (double)T.col.size() / std::max(T.nvert, 1);
   const int sweeps = eq == 0 ? P.kSweeps : P.omegaSweeps;
-  for (int sw = 0; sw < sweeps; ++sw)
-    for (int c = 0; c < T.coloring.ncolors; ++c) {
-      const int b = st.colorOff[(std::size_t)c], n = st.colorOff[(std::size_t)c + 1] - b;
-      mcgsColor<double, double>(s, lanesFor(meanRow, n), b, n, st.crows.data(), st.row.data(), st.col.data(),
-                                st.val.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
-    }
+  st.sweepGraph[eq].run(s, graphKey({x, st.val.data(), st.rhs.data()}, sweeps * 4.0 + P.gsOmega), [&]() {
+    for (int sw = 0; sw < sweeps; ++sw)
+      for (int c = 0; c < T.coloring.ncolors; ++c) {
+        const int b = st.colorOff[(std::size_t)c], n = st.colorOff[(std::size_t)c + 1] - b;
+        mcgsColor<double, double>(s, lanesFor(meanRow, n), b, n, st.crows.data(), st.row.data(), st.col.data(),
+                                  st.val.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
+      }
+  });
   hipLaunchKernelGGL(k_turb_floor, blocksFor(T.nvert), 256, 0, s, T.nvert, eq == 0 ? P.kFloor : P.omegaFloor, x,
                      st.counters.data() + eq);
   NODALS_LAUNCHED();
   return nb;
 }
 
-TurbStepResult H8Solver::turbStep() {
-  requireDevice();
-  require((bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
+void H8Solver::turbAdvance() {
   TurbState& st = *turb_;
   hipStream_t s = stream_;
   hipLaunchKernelGGL(k_fill<int>, 1, 256, 0, s, 4ll, st.counters.data(), 0);
   NODALS_LAUNCHED();
-  NODALS_HIP(hipEventRecord(st.ev[0], s));
   const int waves = std::min(nc_, 8192);
-  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, s, nc_, cref_.data(), cinvJ_.data(), cdet_.data(), u_.data(),
-                     fixedVal_.data(), centT_.data(), st.conv.data());
+  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, s, nc_, cref_.data(), cinvJ_.data(),
+                     cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), st.conv.data());
   NODALS_LAUNCHED();
   turbEquation(0);
   turbEquation(1);
-  NODALS_HIP(hipEventRecord(st.ev[1], s));
-  sync();
   ++st.steps;
-  std::vector<int> cnt = st.counters.host();
+}
+
+int H8Solver::turbCounters(TurbStepResult* R) {
+  std::vector<int> cnt = turb_->counters.host();
+  R->kClips = cnt[0];
+  R->omegaClips = cnt[1];
+  R->wallInvalid = cnt[2];
+  require(cnt[3] == 0, "k-omega step: " + std::to_string(cnt[3]) + " relaxed rows with a non-positive or "
+                       "non-finite diagonal", kErrNonFinite);
+  return cnt[3];
+}
+
+TurbStepResult H8Solver::turbStep() {
+  requireDevice();
+  require((bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
+  TurbState& st = *turb_;
+  NODALS_HIP(hipEventRecord(st.ev[0], stream_));
+  turbAdvance();
+  NODALS_HIP(hipEventRecord(st.ev[1], stream_));
+  sync();
   TurbStepResult R;
-  R.kClips = cnt[0];
-  R.omegaClips = cnt[1];
-  R.wallInvalid = cnt[2];
   float ms = 0;
   NODALS_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]));
   R.ms = ms;
-  require(cnt[3] == 0, "k-omega step: " + std::to_string(cnt[3]) + " relaxed rows with a non-positive or "
-                       "non-finite diagonal", kErrNonFinite);
+  turbCounters(&R);
   return R;
+}
+
+void H8Solver::turbCouple(bool on) {
+  requireDevice();
+  require(!on || (bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
+  turbCouple_ = on;
+}
+
+// Eddy-viscosity diffusion of the momentum equation added to the cell
+// blocks of the convection (cbuf_), which the momentum gather then
+// assembles like the convection itself (Dirichlet columns to the rhs).
+void H8Solver::eddyDiffusion() {
+  TurbState& st = *turb_;
+  const TurbParams& P = st.P;
+  hipLaunchKernelGGL(k_sst_volume, (nc_ + 3) / 4, 256, 0, stream_, nc_, cref_.data(), cinvJ_.data(), cdet_.data(),
+                     sstXY_.data(), u_.data(), fixedVal_.data(), st.k.data(), st.k.data(), st.w.data(), st.w.data(),
+                     sstQuad_.data(), S_.pipe.R, S_.pipe.nu, 2, P.kFloor, P.omegaFloor, P.deviatoric ? 1 : 0,
+                     cbuf_.data(), nullptr, st.sref.data(), 0.0, cbuf_.data());
+  NODALS_LAUNCHED();
 }
 
 std::vector<double> H8Solver::turbField(const std::string& n) {

@@ -43,6 +43,7 @@ struct TurbState {
   DBuf<int> fcell, flf, scell, counters;
   DBuf<double> fgeom, wgeom, sl, sphi, coef, target, conv, blk, brhs, val, diag, rhs, k, w;
   std::vector<int> colorOff;
+  GraphCache sweepGraph[2];  // k, omega
   hipEvent_t ev[2] = {nullptr, nullptr};
   int steps = 0;
   ~TurbState() {

@@ -195,18 +195,22 @@ def cell_values(case: H8Case, U: np.ndarray) -> np.ndarray:
     return out
 
 
-def assemble_central(case: H8Case, U: np.ndarray, T_cent: np.ndarray):
+def assemble_central(case: H8Case, U: np.ndarray, T_cent: np.ndarray, extra: Optional[np.ndarray] = None):
     """Diffusion CSR + Picard central convection and its Dirichlet RHS.
 
     host_assemble_central_g4 (gpu/serial_cuda/g4_host.hpp:317-320) /
     h4_assemble_convection_only_kernel (h4_kernels.cuh:10-56):
     ur[m][j] = sum_d u_d(m) invJ[j][d], C_ab = det sum_{m,j} ur[m][j] T[a][m][b][j];
     free columns add into the row slot, fixed columns move to the RHS.
+    extra (nc,8,8), when given, is added to every cell's block first (the
+    eddy-viscosity diffusion of the coupled SST mode).
     Returns aval (nnz,), conv (3,nv), C (nc,8,8).
     """
     cf = cell_values(case, U)                                     # (nc,8,3)
     ur = np.einsum("cmd,cjd->cmj", cf, case.cell_invJ)            # (nc,8,3)
     C = np.einsum("cmj,ambj->cab", ur, T_cent) * case.cell_det[:, None, None]
+    if extra is not None:
+        C = C + extra
     aval = case.mom_diff.copy()
     conv = np.zeros((3, case.nv))
     ref = case.cell_ref
@@ -631,11 +635,13 @@ def pressure_drop_fit(case: H8Case, p: np.ndarray) -> float:
 
 def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work="fgs1", p_solver="pcg",
            p_rtol=0.5, p_atol=1e-12, p_max=20, smoother: Optional[Smoother] = None, run_mode="fixed10",
-           max_outer=10, simple_tol=1e-6, restart=30, U0=None, p0=None, callback=None):
+           max_outer=10, simple_tol=1e-6, restart=30, U0=None, p0=None, callback=None, extra_blocks=None):
     """H8 SIMPLE loop (gpu/serial_cuda/h8_main_base.cu:2046-2119): assemble,
     B^T p + RHS, fixed momentum work, continuity, fine refresh, pressure
     Krylov + AMG, p += alpha_p dp; converge mode audits the momentum residual
-    once the continuity target is met."""
+    once the continuity target is met.  extra_blocks(U), when given, adds
+    cell blocks to the momentum assembly; callback(it, U, p) runs after the
+    pressure update (the coupled SST mode advances k and omega there)."""
     smoother = smoother or Smoother()
     _, T_cent = tensors()
     U = np.zeros((3, case.nv)) if U0 is None else U0.copy()
@@ -645,7 +651,7 @@ def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work=
     hist = []
     converged = False
     for it in range(1, max_outer + 1):
-        aval, conv, _ = assemble_central(case, U, T_cent)
+        aval, conv, _ = assemble_central(case, U, T_cent, None if extra_blocks is None else extra_blocks(U))
         delta, diag, rau = finalize_relax(case, aval, alpha_u)
         bt = bt_apply(case, p)
         b = momentum_rhs(case, conv, bt, delta, U)

@@ -107,7 +107,9 @@ def volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius
                                deviatoric=deviatoric)
     out = out.reshape(nc, 24, 14)
     wt = cell_det[:, None] * Wq[None, :]
-    if equation == 0:
+    if equation == 2:  # momentum form: eddy-viscosity diffusion only
+        D, R, S = wt * out[..., 0], np.zeros_like(wt), np.zeros_like(wt)
+    elif equation == 0:
         D, R, S = wt * out[..., 3] * out[..., 0], wt * out[..., 10], wt * out[..., 8]
     else:
         D, R, S = wt * out[..., 4] * out[..., 0], wt * out[..., 11], wt * out[..., 9]

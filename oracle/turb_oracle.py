@@ -362,6 +362,17 @@ class Step:
             self.w = xn
         return int(bad.sum())
 
+    def eddy_blocks(self, U: np.ndarray) -> np.ndarray:
+        """(nc,8,8) eddy-viscosity diffusion of the momentum equation at the
+        current k, omega: sum_q w_q det nu_t grad phi_a . grad phi_b (the
+        isotropic part of sst_mom_tet, hxt5d_sst_g2.cuh:1220)."""
+        P, case = self.P, self.case
+        Uc = H.cell_values(case, U)
+        blocks, _, _ = S.volume_terms_cells(self.ref, Uc, self.k[self.ent], self.w[self.ent], case.cell_invJ,
+                                            case.cell_det, self.cell_xy, self.R, self.nu, 2, P.k_floor,
+                                            P.omega_floor, P.deviatoric)
+        return blocks
+
     def step(self, U: np.ndarray):
         """One k then omega update at velocity U (3, nv free, reference order)."""
         ck = self.equation(0, U)
