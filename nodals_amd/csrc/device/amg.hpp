@@ -70,6 +70,7 @@ class DevAmg {
   };
   struct Xfer {
     int nFine = 0, nCoarse = 0, lanesP = 8, lanesT = 8;
+    bool aggregation = false;  // exactly one entry per fine row (row i holds entry i)
     DBuf<int> prow, pcol, trow, tcol;
     DBuf<TV> pval, tval;
   };
@@ -172,7 +173,14 @@ class DevAmg {
     Level& C = L_[0];
     restrictTo(s, P_[0], fr_.data(), C.b.data());
     vcycle(s, 0, C.b.data(), C.x.data());
-    prolongAdd(s, P_[0], C.x.data(), ftmp_.data());
+    const Xfer& X = P_[0];
+    if (X.aggregation) {
+      residJacobiPostAgg<TF, TV, TB, TZ>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den,
+                                         opt_.jacobiOmega, ftmp_.data(), X.pcol.data(), X.pval.data(), C.x.data(),
+                                         z);
+      return;
+    }
+    prolongAdd(s, X, C.x.data(), ftmp_.data());
     residJacobiPost<TF, TV, TB, TZ>(s, fine_.lanes, fine_.n, fine_.row, fine_.col, fine_.val, b, den,
                                     opt_.jacobiOmega, ftmp_.data(), z);
   }
@@ -192,6 +200,9 @@ class DevAmg {
     X.pcol.upload(P.col);
     X.pval.upload(castVec(P.val));
     X.lanesP = lanesFor(P.nFine ? (double)P.col.size() / P.nFine : 1.0, P.nFine);
+    X.aggregation = (std::size_t)P.nFine == P.col.size();
+    for (int i = 0; X.aggregation && i < P.nFine; ++i)
+      X.aggregation = P.row[(std::size_t)i] == i && P.row[(std::size_t)i + 1] == i + 1;
     // Transpose for gather restriction; within a coarse row, fine rows ascend.
     std::vector<int> tr((std::size_t)P.nCoarse + 1, 0);
     for (int c : P.col) ++tr[(std::size_t)c + 1];
@@ -299,9 +310,15 @@ class DevAmg {
       jacobiPreResid<TV, TV>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), b, den, opt_.jacobiOmega,
                              A.tmp.data(), A.r.data());
       Level& C = L_[(std::size_t)l + 1];
-      restrictTo(s, P_[(std::size_t)l + 1], A.r.data(), C.b.data());
+      const Xfer& X = P_[(std::size_t)l + 1];
+      restrictTo(s, X, A.r.data(), C.b.data());
       vcycle(s, l + 1, C.b.data(), C.x.data());
-      prolongAdd(s, P_[(std::size_t)l + 1], C.x.data(), A.tmp.data());
+      if (X.aggregation) {
+        residJacobiPostAgg<TV, TV>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), b, den,
+                                   opt_.jacobiOmega, A.tmp.data(), X.pcol.data(), X.pval.data(), C.x.data(), x);
+        return;
+      }
+      prolongAdd(s, X, C.x.data(), A.tmp.data());
       residJacobiPost<TV, TV>(s, A.lanes, A.n, A.row.data(), A.col.data(), A.val.data(), b, den, opt_.jacobiOmega,
                               A.tmp.data(), x);
       return;

@@ -263,6 +263,39 @@ __global__ void __launch_bounds__(256) k_resid_jacobi_post(int n, const int* __r
   }
 }
 
+// Product rounded on its own (never fused into a following add).
+__device__ __forceinline__ float mulRn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mulRn(double a, double b) { return __dmul_rn(a, b); }
+
+// The same with the coarse correction of an aggregation transfer (one
+// entry per fine row: pcol[j], pval[j]) folded in: t'_j = t_j + pval_j
+// xc[pcol_j] is formed on the fly for every row the pass reads, with the
+// product rounded before the add, which is the value the separate
+// prolongation kernel would have stored (its row sum is that single
+// product).
+template <int LANES, class TM, class TV, class TB = TV, class TZ = TV>
+__global__ void __launch_bounds__(256) k_resid_jacobi_post_agg(
+    int n, const int* __restrict__ row, const int* __restrict__ col, const TM* __restrict__ val,
+    const TB* __restrict__ b, const TV* __restrict__ d, double omega, const TV* __restrict__ t,
+    const int* __restrict__ pcol, const TV* __restrict__ pval, const TV* __restrict__ xc, TZ* __restrict__ z) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / LANES;
+  const int lane = gid % LANES;
+  if (i >= n) return;
+  TV s = 0;
+  const int e = row[i + 1];
+  for (int k = row[i] + lane; k < e; k += LANES) {
+    const int j = col[k];
+    s += ld<TV>(val, k) * (t[j] + mulRn(pval[j], xc[pcol[j]]));
+  }
+  s = groupSum(s, LANES);
+  if (lane == 0) {
+    const TV ti = t[i] + mulRn(pval[i], xc[pcol[i]]);
+    const TV r = (TV)b[i] - s;
+    z[i] = (TZ)(ti + (TV)omega * r / d[i]);
+  }
+}
+
 // l1[r] = sum_k |a_rk|
 template <int LANES, class TM, class TV>
 __global__ void __launch_bounds__(256) k_row_l1(int n, const int* __restrict__ row, const TM* __restrict__ val,
@@ -400,6 +433,26 @@ void jacobiPreResid(hipStream_t s, int lanes, int n, const int* row, const int* 
     default: NODALS_JPR(64); break;
   }
 #undef NODALS_JPR
+  NODALS_LAUNCHED();
+}
+
+template <class TM, class TV, class TB = TV, class TZ = TV>
+void residJacobiPostAgg(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TB* b,
+                        const TV* d, double omega, const TV* t, const int* pcol, const TV* pval, const TV* xc,
+                        TZ* z) {
+  if (n <= 0) return;
+  const int grid = blocksFor((long long)n * lanes);
+#define NODALS_RJPA(LN)                                                                                        \
+  hipLaunchKernelGGL((k_resid_jacobi_post_agg<LN, TM, TV, TB, TZ>), grid, 256, 0, s, n, row, col, val, b, d, omega, \
+                     t, pcol, pval, xc, z)
+  switch (lanes) {
+    case 4: NODALS_RJPA(4); break;
+    case 8: NODALS_RJPA(8); break;
+    case 16: NODALS_RJPA(16); break;
+    case 32: NODALS_RJPA(32); break;
+    default: NODALS_RJPA(64); break;
+  }
+#undef NODALS_RJPA
   NODALS_LAUNCHED();
 }
 

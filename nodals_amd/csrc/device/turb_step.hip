@@ -184,10 +184,20 @@ int H8Solver::turbEquation(int eq) {
     nb += T.nWallQ;
   }
   const double alpha = eq == 0 ? P.alphaK : P.alphaOmega;
-  hipLaunchKernelGGL(k_turb_gather<8>, blocksFor((long long)T.nvert * 8), 256, 0, s, T.nvert, st.row.data(),
-                     st.diagPos.data(), st.entOff.data(), st.termOff.data(), st.term.data(), st.rhsOff.data(),
-                     st.rhsTerm.data(), nb * 64, nb * 8, st.blk.data(), st.brhs.data(), x, alpha, st.val.data(),
-                     st.diag.data(), st.rhs.data(), st.counters.data() + 3);
+  // lanes per row from the entries per row (vertex + bubble columns, about
+  // 50 on tetrahedral meshes): each lane's serial chain of indirect loads
+  // stays short
+  const double meanEnt = (double)T.entOff[(std::size_t)T.nvert] / std::max(T.nvert, 1);
+  const int gl = meanEnt > 48 ? 32 : meanEnt > 12 ? 16 : 8;
+#define NODALS_TURB_GATHER(L)                                                                                      \
+  hipLaunchKernelGGL(k_turb_gather<L>, blocksFor((long long)T.nvert * L), 256, 0, s, T.nvert, st.row.data(),        \
+                     st.diagPos.data(), st.entOff.data(), st.termOff.data(), st.term.data(), st.rhsOff.data(),      \
+                     st.rhsTerm.data(), nb * 64, nb * 8, st.blk.data(), st.brhs.data(), x, alpha, st.val.data(),   \
+                     st.diag.data(), st.rhs.data(), st.counters.data() + 3)
+  if (gl == 32) NODALS_TURB_GATHER(32);
+  else if (gl == 16) NODALS_TURB_GATHER(16);
+  else NODALS_TURB_GATHER(8);
+#undef NODALS_TURB_GATHER
   NODALS_LAUNCHED();
   // the fixed sweeps (sweeps x colours launches on fixed buffers) replay
   // from one captured graph per equation
