@@ -271,6 +271,8 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
       else if (n == "mesh_neighbour") v = M.neighbour;
       else v = M.facePatch;
       copyOut(v, host, count);
+    } else if (n == "wall_distance") {
+      copyOut(s.setup().wallDistance, host, count);
     } else if (n == "fixed_slot") {
       copyOut(s.setup().fixedSlot, host, count);
     } else if (n == "k" || n == "omega" || n.rfind("turb_", 0) == 0) {

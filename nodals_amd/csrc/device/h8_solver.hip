@@ -120,6 +120,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--inlet") o.inlet = next();
     else if (a == "--outlet") o.outlet = next();
     else if (a == "--lid") o.lid = next();
+    else if (a == "--duct") o.duct = inum(next()) != 0;
     else if (a == "--re") o.re = num(next());
     else if (a == "--bulk") o.bulk = num(next());
     else if (a == "--mixing-length") o.mixingLength = inum(next()) != 0;
@@ -253,6 +254,7 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   prm.inlet = opt_.inlet;
   prm.outlet = opt_.outlet;
   prm.lid = opt_.lid;
+  prm.duct = opt_.duct;
   prm.re = opt_.re;
   prm.bulk = opt_.bulk;
   prm.alphaU = opt_.alphaU;
@@ -1173,7 +1175,7 @@ H8RunResult H8Solver::run() {
   R.avgPressureIts = finalIt ? (double)sumP / finalIt : 0.0;
   R.pressureIts = sumP;
   const std::vector<double> p = getCellField("p");
-  R.pressureDrop = pressureDropFit(S_, p);
+  R.pressureDrop = S_.duct ? pressureDropPatches(S_, p) : pressureDropFit(S_, p);
   R.pass = o.runMode == "fixed10" ? (finalIt == 10 && pressureAll && finiteAll) : converged;
   (void)lastAudit;
   return R;

@@ -28,6 +28,7 @@ namespace nodals {
 struct H8Options {
   std::string mesh, tag = "h8", wall = "patch_0_0", inlet = "patch_2_0", outlet = "patch_1_0", vtuOut;
   std::string lid;  // extension: lid-driven cavity on this patch (no inlet/outlet, pressure pinned in cell 0)
+  bool duct = false;  // extension: general duct (uniform inflow, hydraulic diameter, patch pressure drop)
   double re = 20, bulk = 1, simpleTol = 1e-6;
   double alphaU = .5, alphaP = .5, rauScale = 1.0;
   double momRtol = 1e-6, momAtol = 1e-12, momDrop = .1, momOmega = 1.0;

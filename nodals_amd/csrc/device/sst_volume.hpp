@@ -96,7 +96,8 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
     const double* __restrict__ kf, const double* __restrict__ kfix, const double* __restrict__ wf,
     const double* __restrict__ wfix, const double* __restrict__ quad, double radius, double nu, int equation,
     double kFloor, double omFloor, int deviatoric, double* mat, double* __restrict__ rhs,
-    const int* __restrict__ sref = nullptr, double nuBase = 0.0, const double* conv = nullptr) {
+    const int* __restrict__ sref = nullptr, double nuBase = 0.0, const double* conv = nullptr,
+    const double* __restrict__ vdist = nullptr) {
   __shared__ double PH[4][kSstQ][8], GP[4][kSstQ][8][3], DW[4][kSstQ], RW[4][kSstQ], SW[4][kSstQ];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
@@ -135,13 +136,20 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
         gu[6 + d] += U[2] * gd;
       }
     }
-    double xq = 0, yq = 0;
-    for (int a = 0; a < 4; ++a) {
-      xq += l[a] * cxy[(long long)c * 8 + 2 * a];
-      yq += l[a] * cxy[(long long)c * 8 + 2 * a + 1];
+    double dq;
+    if (vdist) {  // P1 interpolation of the vertex wall-distance field
+      double dd = 0;
+      for (int a = 0; a < 4; ++a) dd += l[a] * vdist[sref[(long long)c * 8 + a]];
+      dq = sst::floorDistance(fmax(0.0, dd), radius);
+    } else {
+      double xq = 0, yq = 0;
+      for (int a = 0; a < 4; ++a) {
+        xq += l[a] * cxy[(long long)c * 8 + 2 * a];
+        yq += l[a] * cxy[(long long)c * 8 + 2 * a + 1];
+      }
+      dq = sst::pipeDistance(xq, yq, radius);
     }
-    const sst::Local s = sst::local(sst::pipeDistance(xq, yq, radius), nu, gu, kv, wvv, gk, gw, kFloor, omFloor,
-                                    deviatoric != 0);
+    const sst::Local s = sst::local(dq, nu, gu, kv, wvv, gk, gw, kFloor, omFloor, deviatoric != 0);
     const double wt = cdet[c] * Q[4];
     const bool isK = equation == 0;
     if (equation == 2) {

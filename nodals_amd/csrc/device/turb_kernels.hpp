@@ -76,12 +76,24 @@ __device__ inline void turbStateAt(int c, const double l[4], const int* __restri
   }
 }
 
-__device__ inline void pointXY(int c, const double l[4], const double* __restrict__ cxy, double& x, double& y) {
-  x = y = 0.0;
+// Wall distance at barycentric point l of cell c: the P1 interpolation of
+// the vertex distance field when one is installed (floored at zero, the
+// reference's stage20_distance, modules/geometry/wall_distance_device.cuh:
+// 4-10), else the analytic distance of the straight pipe about its axis.
+__device__ inline double pointDistance(int c, const double l[4], const int* __restrict__ sref,
+                                       const double* __restrict__ vdist, const double* __restrict__ cxy,
+                                       double radius) {
+  if (vdist) {
+    double d = 0.0;
+    for (int a = 0; a < 4; ++a) d += l[a] * vdist[sref[(long long)c * 8 + a]];
+    return sst::floorDistance(fmax(0.0, d), radius);
+  }
+  double x = 0.0, y = 0.0;
   for (int a = 0; a < 4; ++a) {
     x += l[a] * cxy[(long long)c * 8 + 2 * a];
     y += l[a] * cxy[(long long)c * 8 + 2 * a + 1];
   }
+  return sst::pipeDistance(x, y, radius);
 }
 
 // Inlet flux of one face (one wavefront per face, four per block):
@@ -94,7 +106,7 @@ static __global__ void __launch_bounds__(256) k_turb_inlet(
     const int* __restrict__ cref, const int* __restrict__ sref, const double* __restrict__ cinvJ,
     const double* __restrict__ cxy, const double* __restrict__ u, const double* __restrict__ ufix,
     const double* __restrict__ kf, const double* __restrict__ wf, TurbConsts K, Tri12 Q, double inX, double inY,
-    double inZ, double phiIn, double* __restrict__ blk, double* __restrict__ brhs) {
+    double inZ, double phiIn, double* __restrict__ blk, double* __restrict__ brhs, const double* __restrict__ vdist) {
   __shared__ double PH[4][12][8], DN[4][12][8], NUE[4][12], WQ[4][12];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int f = blockIdx.x * 4 + wv;
@@ -106,10 +118,8 @@ static __global__ void __launch_bounds__(256) k_turb_inlet(
     for (int j = 0; j < 3; ++j) l[kTurbFace[lf][j]] = Q.q[lane][j];
     double phi[8], gp[8][3], gu[9], us[3], kv, wvv, gk[3], gw[3];
     turbStateAt(c, l, cref, sref, cinvJ, u, ufix, kf, wf, phi, gp, gu, us, kv, wvv, gk, gw);
-    double x, y;
-    pointXY(c, l, cxy, x, y);
-    const sst::Local s = sst::local(sst::pipeDistance(x, y, K.radius), K.nu, gu, kv, wvv, gk, gw, K.kFloor,
-                                    K.omFloor, K.deviatoric != 0);
+    const sst::Local s = sst::local(pointDistance(c, l, sref, vdist, cxy, K.radius), K.nu, gu, kv, wvv, gk, gw,
+                                    K.kFloor, K.omFloor, K.deviatoric != 0);
     NUE[wv][lane] = K.nu + (K.equation == 0 ? s.sigmaK : s.sigmaW) * s.nut;
     WQ[wv][lane] = Q.q[lane][3] * G[3];
     for (int a = 0; a < 8; ++a) {
@@ -152,7 +162,8 @@ static __global__ void __launch_bounds__(256) k_turb_wall_coef(
     const double* __restrict__ sphi, const int* __restrict__ cref, const int* __restrict__ sref,
     const double* __restrict__ cinvJ, const double* __restrict__ cxy, const double* __restrict__ u,
     const double* __restrict__ ufix, const double* __restrict__ kf, const double* __restrict__ wf, TurbConsts K,
-    WallConsts W, Tri12 Q, double* __restrict__ coef, double* __restrict__ target, int* __restrict__ bad) {
+    WallConsts W, Tri12 Q, double* __restrict__ coef, double* __restrict__ target, int* __restrict__ bad,
+    const double* __restrict__ vdist) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nq) return;
   const double* G = wgeom + 4ll * (w / 12);
@@ -176,10 +187,8 @@ static __global__ void __launch_bounds__(256) k_turb_wall_coef(
   const double l[4] = {sl[s3 * 4], sl[s3 * 4 + 1], sl[s3 * 4 + 2], sl[s3 * 4 + 3]};
   double phi[8], gp[8][3], gu[9], us[3], kv, wvv, gk[3], gw[3];
   turbStateAt(c, l, cref, sref, cinvJ, u, ufix, kf, wf, phi, gp, gu, us, kv, wvv, gk, gw);
-  double x, y;
-  pointXY(c, l, cxy, x, y);
-  const sst::Local ss =
-      sst::local(sst::pipeDistance(x, y, K.radius), K.nu, gu, kv, wvv, gk, gw, K.kFloor, K.omFloor, K.deviatoric != 0);
+  const sst::Local ss = sst::local(pointDistance(c, l, sref, vdist, cxy, K.radius), K.nu, gu, kv, wvv, gk, gw,
+                                   K.kFloor, K.omFloor, K.deviatoric != 0);
   const double normal = us[0] * G[0] + us[1] * G[1] + us[2] * G[2];
   const double Ut = sqrt(fmax(0.0, us[0] * us[0] + us[1] * us[1] + us[2] * us[2] - normal * normal));
   const sst::OmegaWall oa = sst::omegaWall(W.y[2], K.nu, Ut, P.uTau, kv, wvv, ss.pk, K.kFloor, K.omFloor, W.kappa,

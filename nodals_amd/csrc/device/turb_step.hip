@@ -144,6 +144,7 @@ void H8Solver::turbInit(const TurbParams& P) {
   st->k.upload(k0);
   st->w.upload(w0);
   st->counters.upload(std::vector<int>(4, 0));
+  if (!S_.wallDistance.empty()) st->vdist.upload(S_.wallDistance);
   for (hipEvent_t& e : st->ev) NODALS_HIP(hipEventCreate(&e));
   turb_ = std::move(st);
 }
@@ -159,14 +160,14 @@ int H8Solver::turbEquation(int eq) {
   hipLaunchKernelGGL(k_sst_volume, (nc_ + 3) / 4, 256, 0, s, nc_, cref_.data(), cinvJ_.data(), cdet_.data(),
                      sstXY_.data(), u_.data(), fixedVal_.data(), st.k.data(), st.k.data(), st.w.data(), st.w.data(),
                      sstQuad_.data(), S_.pipe.R, S_.pipe.nu, eq, P.kFloor, P.omegaFloor, P.deviatoric ? 1 : 0,
-                     st.blk.data(), st.brhs.data(), st.sref.data(), S_.pipe.nu, st.conv.data());
+                     st.blk.data(), st.brhs.data(), st.sref.data(), S_.pipe.nu, st.conv.data(), st.vdistPtr());
   NODALS_LAUNCHED();
   if (T.nInlet > 0) {
     hipLaunchKernelGGL(k_turb_inlet, (T.nInlet + 3) / 4, 256, 0, s, T.nInlet, st.fcell.data(), st.flf.data(),
                        st.fgeom.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(), u_.data(),
                        fixedVal_.data(), st.k.data(), st.w.data(), K, tri12(), S_.pipe.inletVelocity[0],
                        S_.pipe.inletVelocity[1], S_.pipe.inletVelocity[2], eq == 0 ? T.kIn : T.omegaIn,
-                       st.blk.data() + (std::size_t)nc_ * 64, st.brhs.data() + (std::size_t)nc_ * 8);
+                       st.blk.data() + (std::size_t)nc_ * 64, st.brhs.data() + (std::size_t)nc_ * 8, st.vdistPtr());
     NODALS_LAUNCHED();
   }
   int nb = nc_ + T.nInlet;
@@ -176,7 +177,7 @@ int H8Solver::turbEquation(int eq) {
     hipLaunchKernelGGL(k_turb_wall_coef, blocksFor(T.nWallQ), 256, 0, s, T.nWallQ, st.wgeom.data(), st.scell.data(),
                        st.sl.data(), st.sphi.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(),
                        u_.data(), fixedVal_.data(), st.k.data(), st.w.data(), K, W, tri12(), st.coef.data(),
-                       st.target.data(), st.counters.data() + 2);
+                       st.target.data(), st.counters.data() + 2, st.vdistPtr());
     NODALS_LAUNCHED();
     hipLaunchKernelGGL(k_turb_wall_block, (T.nWallQ + 3) / 4, 256, 0, s, T.nWallQ, st.sphi.data(), st.coef.data(),
                        st.target.data(), st.blk.data() + (std::size_t)nb * 64, st.brhs.data() + (std::size_t)nb * 8);
@@ -272,7 +273,7 @@ void H8Solver::eddyDiffusion() {
   hipLaunchKernelGGL(k_sst_volume, (nc_ + 3) / 4, 256, 0, stream_, nc_, cref_.data(), cinvJ_.data(), cdet_.data(),
                      sstXY_.data(), u_.data(), fixedVal_.data(), st.k.data(), st.k.data(), st.w.data(), st.w.data(),
                      sstQuad_.data(), S_.pipe.R, S_.pipe.nu, 2, P.kFloor, P.omegaFloor, P.deviatoric ? 1 : 0,
-                     cbuf_.data(), nullptr, st.sref.data(), 0.0, cbuf_.data());
+                     cbuf_.data(), nullptr, st.sref.data(), 0.0, cbuf_.data(), st.vdistPtr());
   NODALS_LAUNCHED();
 }
 

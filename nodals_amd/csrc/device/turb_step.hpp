@@ -42,6 +42,8 @@ struct TurbState {
   DBuf<int> sref, row, col, diagPos, entOff, termOff, term, rhsOff, rhsTerm, crows;
   DBuf<int> fcell, flf, scell, counters;
   DBuf<double> fgeom, wgeom, sl, sphi, coef, target, conv, blk, brhs, val, diag, rhs, k, w;
+  DBuf<double> vdist;  // vertex wall distances (duct mode), empty for the straight pipe
+  const double* vdistPtr() const { return vdist.size() ? vdist.data() : nullptr; }
   std::vector<int> colorOff;
   GraphCache sweepGraph[2];  // k, omega
   hipEvent_t ev[2] = {nullptr, nullptr};

@@ -133,6 +133,56 @@ PipeInfo makeCavity(const TetMesh& M, const H8Params& prm) {
   return P;
 }
 
+// General duct: hydraulic diameter of the inlet, uniform inflow along the
+// inward inlet normal.
+PipeInfo makeDuct(const TetMesh& M, const H8Params& prm) {
+  PipeInfo P;
+  P.wall = M.patchIndex(prm.wall);
+  P.inlet = M.patchIndex(prm.inlet);
+  P.outlet = M.patchIndex(prm.outlet);
+  P.re = prm.re;
+  P.bulk = prm.bulk;
+  require(P.wall >= 0 && P.inlet >= 0 && P.outlet >= 0,
+          "duct patch not found (wall=" + prm.wall + " inlet=" + prm.inlet + " outlet=" + prm.outlet + ")");
+  require(prm.re > 0 && prm.bulk > 0, "invalid duct flow parameters");
+  const Patch& pin = M.patches[(std::size_t)P.inlet];
+  V3 sfSum = {0, 0, 0}, cen = {0, 0, 0};
+  for (int f = pin.start; f < pin.start + pin.count; ++f) {
+    const auto& F = M.faces[(std::size_t)f];
+    const V3 X[3] = {M.points[(std::size_t)F[0]], M.points[(std::size_t)F[1]], M.points[(std::size_t)F[2]]};
+    const double a = 0.5 * norm(cross(sub(X[1], X[0]), sub(X[2], X[0])));
+    P.inletArea += a;
+    for (int d = 0; d < 3; ++d) cen[(std::size_t)d] += a * (X[0][(std::size_t)d] + X[1][(std::size_t)d] + X[2][(std::size_t)d]) / 3.0;
+    const V3 sf = outwardAreaVector(M, f);
+    for (int d = 0; d < 3; ++d) sfSum[(std::size_t)d] += sf[(std::size_t)d];
+  }
+  P.inletProjectedArea = norm(sfSum);
+  require(P.inletArea > 0.0 && P.inletProjectedArea > 0.0, "degenerate duct inlet");
+  for (int d = 0; d < 3; ++d) {
+    P.inletNormal[d] = sfSum[(std::size_t)d] / P.inletProjectedArea;
+    P.inletVelocity[d] = -prm.bulk * P.inletNormal[d];
+  }
+  P.cx = cen[0] / P.inletArea;
+  P.cy = cen[1] / P.inletArea;
+  P.D = 2.0 * std::sqrt(P.inletArea / 3.14159265358979323846);
+  P.R = 0.5 * P.D;
+  P.nu = prm.bulk * P.D / prm.re;
+  P.circleArea = P.inletArea;
+  P.profileScale = 1.0;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (const auto& x : M.points)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], x[(std::size_t)d]);
+      hi[d] = std::max(hi[d], x[(std::size_t)d]);
+    }
+  P.zIn = lo[2];
+  P.zOut = hi[2];
+  P.L = hi[2] - lo[2];
+  P.hpGradient = 0.0;
+  P.hpDrop = 0.0;  // no closed form
+  return P;
+}
+
 PipeInfo makePipe(const TetMesh& M, const H8Params& prm) {
   PipeInfo P;
   P.wall = M.patchIndex(prm.wall);
@@ -358,7 +408,8 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
   const int nth = hostThreads(prm.threads);
   H8Setup S;
   S.cavity = !prm.lid.empty();
-  S.pipe = S.cavity ? makeCavity(M, prm) : makePipe(M, prm);
+  S.duct = !S.cavity && prm.duct;
+  S.pipe = S.cavity ? makeCavity(M, prm) : S.duct ? makeDuct(M, prm) : makePipe(M, prm);
   S.pinCell = S.cavity ? 0 : -1;
   const int np = M.nPoints(), nf = M.nFaces(), ni = M.nInternal(), nc = M.nCells();
   S.nc = nc;
@@ -417,13 +468,31 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
       S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)np + f]][0] = (20.0 / 9.0) * (exact - vm);
     }
   }
-  for (int v = 0; v < np && !S.cavity; ++v)
+  if (S.duct) {
+    // uniform inflow: every inlet vertex off the wall carries the inlet
+    // velocity; each inlet face bubble makes the face average exact
+    for (int v = 0; v < np; ++v)
+      if (onInlet[(std::size_t)v] && !onWall[(std::size_t)v])
+        for (int d = 0; d < 3; ++d) S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)v]][(std::size_t)d] = S.pipe.inletVelocity[d];
+    const Patch& pin = M.patches[(std::size_t)S.pipe.inlet];
+    for (int f = pin.start; f < pin.start + pin.count; ++f) {
+      const auto& F = M.faces[(std::size_t)f];
+      for (int d = 0; d < 3; ++d) {
+        double vm = 0;
+        for (int j = 0; j < 3; ++j)
+          vm += S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)F[(std::size_t)j]]][(std::size_t)d] / 3.0;
+        S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)np + f]][(std::size_t)d] =
+            (20.0 / 9.0) * (S.pipe.inletVelocity[d] - vm);
+      }
+    }
+  }
+  for (int v = 0; v < np && !S.cavity && !S.duct; ++v)
     if (onInlet[(std::size_t)v] && !onWall[(std::size_t)v]) {
       const int s = S.fixedSlot[(std::size_t)v];
       S.fixedValue[(std::size_t)s][2] =
           S.pipe.profileScale * idealUz(S.pipe, M.points[(std::size_t)v][0], M.points[(std::size_t)v][1]);
     }
-  if (!S.cavity) {
+  if (!S.cavity && !S.duct) {
     const Patch& pin = M.patches[(std::size_t)S.pipe.inlet];
     for (int f = pin.start; f < pin.start + pin.count; ++f) {
       const auto& F = M.faces[(std::size_t)f];
@@ -626,7 +695,137 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
   S.initRau = hostFinalizeRelax(S, a0, prm.alphaU, &S.initDelta);
   if (prm.rauScale != 1.0)
     for (double& q : S.initRau) q *= prm.rauScale;
+  if (S.duct) {
+    for (int pass = 0; pass < 2; ++pass) {
+      const Patch& pt = M.patches[(std::size_t)(pass ? S.pipe.outlet : S.pipe.inlet)];
+      for (int f = pt.start; f < pt.start + pt.count; ++f) {
+        const auto& F = M.faces[(std::size_t)f];
+        const V3 X[3] = {M.points[(std::size_t)F[0]], M.points[(std::size_t)F[1]], M.points[(std::size_t)F[2]]};
+        (pass ? S.outletCells : S.inletCells).push_back(M.owner[(std::size_t)f]);
+        (pass ? S.outletAreas : S.inletAreas).push_back(0.5 * norm(cross(sub(X[1], X[0]), sub(X[2], X[0]))));
+      }
+    }
+    S.wallDistance = wallDistanceField(M, S.pipe.wall, nth);
+  }
   return S;
+}
+
+double pressureDropPatches(const H8Setup& S, const std::vector<double>& p) {
+  if (p.size() != (std::size_t)S.nc || S.inletCells.empty() || S.outletCells.empty())
+    return std::numeric_limits<double>::quiet_NaN();
+  auto mean = [&](const std::vector<std::int32_t>& c, const std::vector<double>& a) {
+    long double sa = 0, sp = 0;
+    for (std::size_t i = 0; i < c.size(); ++i) {
+      sa += a[i];
+      sp += a[i] * p[(std::size_t)c[i]];
+    }
+    return (double)(sp / sa);
+  };
+  return mean(S.inletCells, S.inletAreas) - mean(S.outletCells, S.outletAreas);
+}
+
+namespace {
+
+// Distance from point p to triangle (a, b, c): the closest point by the
+// Voronoi-region tests of Ericson, Real-Time Collision Detection, 5.1.5.
+double pointTriangleDistance(const V3& p, const V3& a, const V3& b, const V3& c) {
+  const V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  auto dot3 = [](const V3& u, const V3& v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  auto at = [&](double v, double w) {
+    const V3 q = {a[0] + v * ab[0] + w * ac[0], a[1] + v * ab[1] + w * ac[1], a[2] + v * ab[2] + w * ac[2]};
+    return norm(sub(p, q));
+  };
+  const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0 && d2 <= 0) return norm(ap);
+  const V3 bp = sub(p, b);
+  const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0 && d4 <= d3) return norm(bp);
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) return at(d1 / (d1 - d3), 0.0);
+  const V3 cp = sub(p, c);
+  const double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0 && d5 <= d6) return norm(cp);
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) return at(0.0, d2 / (d2 - d6));
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return at(1.0 - w, w);
+  }
+  const double den = 1.0 / (va + vb + vc);
+  return at(vb * den, vc * den);
+}
+
+}  // namespace
+
+std::vector<double> wallDistanceField(const TetMesh& M, int patch, int threads) {
+  require(patch >= 0 && patch < (int)M.patches.size(), "wall patch not found for the distance field");
+  const Patch& pt = M.patches[(std::size_t)patch];
+  require(pt.count > 0, "empty wall patch");
+  // uniform grid over the wall triangles' bounding boxes
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  double meanEdge = 0;
+  for (int f = pt.start; f < pt.start + pt.count; ++f)
+    for (int j = 0; j < 3; ++j) {
+      const V3& x = M.points[(std::size_t)M.faces[(std::size_t)f][(std::size_t)j]];
+      const V3& y = M.points[(std::size_t)M.faces[(std::size_t)f][(std::size_t)((j + 1) % 3)]];
+      meanEdge += norm(sub(x, y));
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], x[(std::size_t)d]);
+        hi[d] = std::max(hi[d], x[(std::size_t)d]);
+      }
+    }
+  meanEdge /= 3.0 * pt.count;
+  const double h = std::max(2.0 * meanEdge, 1e-12);
+  int dim[3];
+  for (int d = 0; d < 3; ++d) dim[d] = std::max(1, std::min(512, (int)std::ceil((hi[d] - lo[d]) / h) + 1));
+  auto cellOf = [&](double x, int d) {
+    return std::max(0, std::min(dim[d] - 1, (int)std::floor((x - lo[d]) / h)));
+  };
+  const std::size_t ncell = (std::size_t)dim[0] * dim[1] * dim[2];
+  std::vector<std::vector<int>> bins(ncell);
+  for (int f = pt.start; f < pt.start + pt.count; ++f) {
+    int b0[3] = {1 << 30, 1 << 30, 1 << 30}, b1[3] = {-1, -1, -1};
+    for (int j = 0; j < 3; ++j) {
+      const V3& x = M.points[(std::size_t)M.faces[(std::size_t)f][(std::size_t)j]];
+      for (int d = 0; d < 3; ++d) {
+        b0[d] = std::min(b0[d], cellOf(x[(std::size_t)d], d));
+        b1[d] = std::max(b1[d], cellOf(x[(std::size_t)d], d));
+      }
+    }
+    for (int i = b0[0]; i <= b1[0]; ++i)
+      for (int j = b0[1]; j <= b1[1]; ++j)
+        for (int k = b0[2]; k <= b1[2]; ++k) bins[((std::size_t)i * dim[1] + j) * dim[2] + k].push_back(f);
+  }
+  const int np = M.nPoints();
+  std::vector<double> dist((std::size_t)np, 0.0);
+  parallelFor(np, hostThreads(threads), 256, [&](int vb, int ve) {
+    std::vector<int> seen;
+    for (int v = vb; v < ve; ++v) {
+      const V3& p = M.points[(std::size_t)v];
+      const int c[3] = {cellOf(p[0], 0), cellOf(p[1], 1), cellOf(p[2], 2)};
+      double best = 1e300;
+      // rings of grid cells; every triangle within distance r*h of p lies in
+      // ring <= r + 1, so the search stops once best <= r * h
+      for (int r = 0; r < 1024; ++r) {
+        for (int i = c[0] - r; i <= c[0] + r; ++i)
+          for (int j = c[1] - r; j <= c[1] + r; ++j)
+            for (int k = c[2] - r; k <= c[2] + r; ++k) {
+              if (std::max(std::abs(i - c[0]), std::max(std::abs(j - c[1]), std::abs(k - c[2]))) != r) continue;
+              if (i < 0 || j < 0 || k < 0 || i >= dim[0] || j >= dim[1] || k >= dim[2]) continue;
+              for (int f : bins[((std::size_t)i * dim[1] + j) * dim[2] + k]) {
+                const auto& F = M.faces[(std::size_t)f];
+                best = std::min(best, pointTriangleDistance(p, M.points[(std::size_t)F[0]], M.points[(std::size_t)F[1]],
+                                                            M.points[(std::size_t)F[2]]));
+              }
+            }
+        if (best <= r * h) break;
+        if (r > std::max(dim[0], std::max(dim[1], dim[2]))) break;
+      }
+      dist[(std::size_t)v] = best;
+    }
+  });
+  return dist;
 }
 
 void hostAssembleCentral(const H8Setup& S, const std::array<std::vector<double>, 3>& U,

@@ -79,6 +79,12 @@ struct H8Params {
   // face is Dirichlet, the lid patch moves at (bulk, 0, 0), nu = bulk L / re
   // with L the largest box extent, and the pressure is pinned in cell 0.
   std::string lid;
+  // General duct instead of the straight z-aligned pipe: a uniform inlet
+  // velocity bulk along the inward inlet normal, D the hydraulic diameter of
+  // the inlet (2 sqrt(A/pi)), the pressure drop as the difference of the
+  // face-area-weighted pressures next to the inlet and the outlet, and a
+  // geometric wall-distance field on the vertices (for the SST terms).
+  bool duct = false;
   double re = 20.0, bulk = 1.0;
   double alphaU = 0.5;
   double rauScale = 1.0;
@@ -88,7 +94,13 @@ struct H8Params {
 struct H8Setup {
   PipeInfo pipe;
   bool cavity = false;
+  bool duct = false;
   int pinCell = -1;  // pressure row pinned (cavity), -1 for none
+  // duct mode: cells next to the inlet / outlet with their face areas, and
+  // the distance of every vertex to the nearest wall face
+  std::vector<std::int32_t> inletCells, outletCells;
+  std::vector<double> inletAreas, outletAreas;
+  std::vector<double> wallDistance;
   int nc = 0, nv = 0, nFixed = 0;
   int nPoints = 0, nFaces = 0;
   int freeVertices = 0, freeFaces = 0, fixedVertices = 0, fixedFaces = 0;
@@ -137,6 +149,12 @@ FineTopo buildFineTopo(const H8Setup& S);
 // Linear least-squares fit of p over cell-centroid z, times -(zmax-zmin)
 // (g4_host.hpp:424).
 double pressureDropFit(const H8Setup& S, const std::vector<double>& p);
+// Duct mode: area-weighted mean pressure of the inlet cells minus that of
+// the outlet cells.
+double pressureDropPatches(const H8Setup& S, const std::vector<double>& p);
+// Distance of every mesh vertex to the nearest face of the given patch
+// (exact point-triangle distance; uniform-grid search).
+std::vector<double> wallDistanceField(const TetMesh& M, int patch, int threads);
 
 int hostThreads(int requested);
 

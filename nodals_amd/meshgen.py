@@ -350,6 +350,40 @@ def write_pipe_case(directory: os.PathLike, core: int = 8, ring: int = 5, nslab:
     return write_polymesh(directory, mesh)
 
 
+def write_helix_case(directory: os.PathLike, core: int = 12, ring: int = 9, nslab: int = 60,
+                     diameter: float = 1.0, coil_radius_d: float = 5.0, pitch_d: float = 2.0,
+                     turns: float = 2.0) -> Path:
+    """Helical pipe polyMesh with the H8 patch names (run it with --duct 1).
+
+    ``patch_0_0`` wall, ``patch_2_0`` inlet (the section at the start of the
+    helix), ``patch_1_0`` outlet (the section at its end).  The defaults are
+    the spiral configuration's section (576 quads: a 12x12 core and 4 x
+    (12 x 9) ring) swept through 60 slabs, 6 tets per hex: 207,360 tets.
+    """
+    pts, tets = helical_pipe(core, ring, nslab, diameter, coil_radius_d, pitch_d, turns)
+    Rc = coil_radius_d * diameter
+    c = pitch_d * diameter / (2.0 * math.pi)
+    T = 2.0 * math.pi * turns
+    tol = 1e-8 * diameter
+
+    def plane(t):
+        cen = np.array([Rc * math.cos(t), Rc * math.sin(t), c * t])
+        tan = np.array([-Rc * math.sin(t), Rc * math.cos(t), c])
+        return cen, tan / np.linalg.norm(tan)
+
+    c0, t0 = plane(0.0)
+    c1, t1 = plane(T)
+
+    def classify(fc, n):
+        out = np.zeros(fc.shape[0], dtype=np.int64)
+        out[np.abs((fc - c0) @ t0) < tol] = 1
+        out[np.abs((fc - c1) @ t1) < tol] = 2
+        return out
+
+    mesh = tets_to_polymesh(pts, tets, classify, ["patch_0_0", "patch_2_0", "patch_1_0"])
+    return write_polymesh(directory, mesh)
+
+
 def write_cavity_case(directory: os.PathLike, n: int = 16) -> Path:
     """Lid-driven cavity polyMesh: ``lid`` at z=1 and ``walls`` elsewhere."""
     pts, tets = kuhn_cube(n)

@@ -89,9 +89,12 @@ def volume_terms(ref_sst, cell_ref, cell_invJ, cell_det, cell_xy, U_free, U_fixe
 
 
 def volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius, nu, equation, k_floor=1e-8,
-                       omega_floor=1e-3, deviatoric=False):
+                       omega_floor=1e-3, deviatoric=False, cell_vdist=None):
     """volume_terms from per-cell basis coefficients: Uc (nc,8,3), Kc and Wc
-    (nc,8)."""
+    (nc,8).  cell_vdist (nc,4), when given, is the wall distance at the four
+    vertices: the distance at a point is its P1 interpolation floored at zero
+    (modules/geometry/wall_distance_device.cuh:4-10) instead of the pipe's
+    analytic distance."""
     nc = Uc.shape[0]
     Lq, Wq = keast24()
     phi, gr = basis(Lq)                                   # (24,8), (24,8,3)
@@ -101,10 +104,16 @@ def volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius
     wv = Wc @ phi.T
     gk = np.einsum("ca,cqad->cqd", Kc, gp)
     gw = np.einsum("ca,cqad->cqd", Wc, gp)
-    xy = np.einsum("qa,cak->cqk", Lq, cell_xy)                        # (nc,24,2)
-    out, _ = ref_sst.sst_local(gu.reshape(-1, 9), kv.ravel(), wv.ravel(), gk.reshape(-1, 3), gw.reshape(-1, 3), nu,
-                               radius, pipe_xy=xy.reshape(-1, 2), k_floor=k_floor, omega_floor=omega_floor,
-                               deviatoric=deviatoric)
+    if cell_vdist is not None:
+        dq = np.maximum(0.0, np.einsum("qa,ca->cq", Lq, cell_vdist))
+        out, _ = ref_sst.sst_local(gu.reshape(-1, 9), kv.ravel(), wv.ravel(), gk.reshape(-1, 3), gw.reshape(-1, 3),
+                                   nu, radius, dist=dq.ravel(), k_floor=k_floor, omega_floor=omega_floor,
+                                   deviatoric=deviatoric)
+    else:
+        xy = np.einsum("qa,cak->cqk", Lq, cell_xy)                    # (nc,24,2)
+        out, _ = ref_sst.sst_local(gu.reshape(-1, 9), kv.ravel(), wv.ravel(), gk.reshape(-1, 3), gw.reshape(-1, 3),
+                                   nu, radius, pipe_xy=xy.reshape(-1, 2), k_floor=k_floor, omega_floor=omega_floor,
+                                   deviatoric=deviatoric)
     out = out.reshape(nc, 24, 14)
     wt = cell_det[:, None] * Wq[None, :]
     if equation == 2:  # momentum form: eddy-viscosity diffusion only

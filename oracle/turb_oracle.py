@@ -168,9 +168,12 @@ def locate(m: Mesh, adj: np.ndarray, start: int, p):
 class Step:
     """The transport step on one context's data."""
 
-    def __init__(self, m: Mesh, case: H.H8Case, pipe: np.ndarray, P: Params, ref_sst):
-        """pipe: cx cy R D bulk nu inletVelocity(3) inletPatch wallPatch outletPatch."""
+    def __init__(self, m: Mesh, case: H.H8Case, pipe: np.ndarray, P: Params, ref_sst, wall_distance=None):
+        """pipe: cx cy R D bulk nu inletVelocity(3) inletPatch wallPatch outletPatch.
+        wall_distance (np,): vertex distance field of a duct (None: the
+        straight pipe's analytic distance about (cx, cy))."""
         self.m, self.case, self.P, self.ref = m, case, P, ref_sst
+        self.vdist = None if wall_distance is None else np.asarray(wall_distance, dtype=float)
         self.cx, self.cy, self.R, self.D, self.bulk, self.nu = pipe[:6]
         self.inlet_velocity = pipe[6:9]
         inlet_patch, wall_patch = int(pipe[9]), int(pipe[10])
@@ -219,9 +222,14 @@ class Step:
         gu = np.einsum("nai,nad->nid", Uc, gp).reshape(-1, 9)
         kv, wv = (Kc * phi).sum(1), (Wc * phi).sum(1)
         gk, gw = np.einsum("na,nad->nd", Kc, gp), np.einsum("na,nad->nd", Wc, gp)
-        xy = np.einsum("na,nak->nk", L, self.cell_xy[cells])
-        out, _ = self.ref.sst_local(gu, kv, wv, gk, gw, self.nu, self.R, pipe_xy=xy, k_floor=self.P.k_floor,
-                                    omega_floor=self.P.omega_floor, deviatoric=self.P.deviatoric)
+        if self.vdist is not None:
+            d = np.maximum(0.0, np.einsum("na,na->n", L, self.vdist[self.m.tets[cells]]))
+            out, _ = self.ref.sst_local(gu, kv, wv, gk, gw, self.nu, self.R, dist=d, k_floor=self.P.k_floor,
+                                        omega_floor=self.P.omega_floor, deviatoric=self.P.deviatoric)
+        else:
+            xy = np.einsum("na,nak->nk", L, self.cell_xy[cells])
+            out, _ = self.ref.sst_local(gu, kv, wv, gk, gw, self.nu, self.R, pipe_xy=xy, k_floor=self.P.k_floor,
+                                        omega_floor=self.P.omega_floor, deviatoric=self.P.deviatoric)
         us = np.einsum("nai,na->ni", Uc, phi)
         return out, phi, gp, us, kv, wv
 
@@ -235,7 +243,7 @@ class Step:
         conv = np.einsum("cmj,ambj->cab", ur, self.T_cent) * case.cell_det[:, None, None]
         Kc, Wc = self.k[self.ent], self.w[self.ent]
         vol, vrhs, _ = S.volume_terms_cells(self.ref, Uc, Kc, Wc, case.cell_invJ, case.cell_det, self.cell_xy, self.R,
-                                            self.nu, eq, P.k_floor, P.omega_floor, P.deviatoric)
+                                            self.nu, eq, P.k_floor, P.omega_floor, P.deviatoric, self._cell_vdist())
         blocks = [self.nu * self.diff + conv + vol]
         brhs = [vrhs]
         bcell = [np.arange(nc)]
@@ -362,6 +370,9 @@ class Step:
             self.w = xn
         return int(bad.sum())
 
+    def _cell_vdist(self):
+        return None if self.vdist is None else self.vdist[self.m.tets]
+
     def eddy_blocks(self, U: np.ndarray) -> np.ndarray:
         """(nc,8,8) eddy-viscosity diffusion of the momentum equation at the
         current k, omega: sum_q w_q det nu_t grad phi_a . grad phi_b (the
@@ -370,7 +381,7 @@ class Step:
         Uc = H.cell_values(case, U)
         blocks, _, _ = S.volume_terms_cells(self.ref, Uc, self.k[self.ent], self.w[self.ent], case.cell_invJ,
                                             case.cell_det, self.cell_xy, self.R, self.nu, 2, P.k_floor,
-                                            P.omega_floor, P.deviatoric)
+                                            P.omega_floor, P.deviatoric, self._cell_vdist())
         return blocks
 
     def step(self, U: np.ndarray):

@@ -22,6 +22,7 @@ def mesh_dir(tmp_path_factory):
     return {
         "small": str(meshgen.write_pipe_case(d / "small", core=4, ring=2, nslab=6)),
         "medium": str(meshgen.write_pipe_case(d / "medium", core=6, ring=3, nslab=12)),
+        "helix": str(meshgen.write_helix_case(d / "helix", core=4, ring=2, nslab=48)),
     }
 
 
