@@ -259,6 +259,43 @@ def sst_coupled_leg(mesh: str) -> dict:
     return out
 
 
+def helix_leg() -> dict:
+    """Config 5's geometry: the 207,360-tet helix (576-quad section x 60 slabs,
+    coil radius 5D, pitch 2D, two turns) in duct mode, FP32 AMG with
+    multicolour Gauss-Seidel smoothing (the spiral case's smoother; the
+    section cells are about 20 times longer than wide, which Jacobi-smoothed
+    aggregation does not handle).  Laminar Re = 20 to 1e-6, and coupled SST at
+    Re = 200 to 1e-5 (alpha 0.7 / 0.3, pressure to 1e-2, 20 + 20 k-omega
+    sweeps)."""
+    from nodals_amd import meshgen
+    from nodals_amd._lib import H8
+
+    d = ROOT / "_meshes" / "helix_207k"
+    if not (d / "faces").exists():
+        meshgen.write_helix_case(d)
+    base = ["--mesh", str(d), "--duct", "1", "--run-mode", "converge", "--max-outer", "6000", "--amg-smoother", "mcgs",
+            "--pressure-solver", "fgmres", "--p-max", "400", "--amg-precision", "fp32", "--verbose", "0"]
+    out = {}
+    for key, extra, sst in (("laminar_re20", ["--re", "20", "--simple-tol", "1e-6", "--alpha-u", "0.5", "--alpha-p", "0.5",
+                                              "--p-rtol", "0.5"], False),
+                            ("sst_re200", ["--re", "200", "--simple-tol", "1e-5", "--alpha-u", "0.7", "--alpha-p", "0.3",
+                                           "--p-rtol", "1e-2"], True)):
+        h = H8(base + extra)
+        if sst:
+            h.turb_init([])
+            h.turb_couple(True)
+        t0 = time.perf_counter()
+        r = h.run()
+        dt = time.perf_counter() - t0
+        out[key] = {"s": dt, "outer": r["outer"], "converged": r["converged"], "pressure_its": r["pressure_its"],
+                    "ms_per_simple": 1e3 * dt / max(r["outer"], 1), "turb_s": r["turb_ms"] / 1e3,
+                    "pressure_drop": r["pressure_drop"], "cells": h.info["cells"]}
+        print(f"[bench] helix 207k {key}: {dt:.2f} s, outer={r['outer']} converged={r['converged']} "
+              f"pressure its={r['pressure_its']}", file=sys.stderr, flush=True)
+        h.close()
+    return out
+
+
 def learned_leg(mesh: str, steps: int, epochs: int) -> dict:
     """Learned warm start + FP32 AMG on the same pipe: samples recorded from
     the solver's own converged runs at Re 10, 15, 30, 45 (every 4th SIMPLE
@@ -315,6 +352,7 @@ def main() -> int:
     ap.add_argument("--no-reference-legs", action="store_true")
     ap.add_argument("--no-turb", action="store_true")
     ap.add_argument("--no-sst-coupled", action="store_true")
+    ap.add_argument("--no-helix", action="store_true")
     ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--kernel-profile", action="store_true",
                     help="only the fine SpMV and the copy calibration kernel (for counter runs)")
@@ -389,6 +427,7 @@ def main() -> int:
     dp64 = fp64["runs"][-1]["pressure_drop"]
     turb = None if a.no_turb else turb_leg(h, 20)
     coupled = None if a.no_sst_coupled else sst_coupled_leg(mesh)
+    helix = None if a.no_helix else helix_leg()
     poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
     learned = None if a.no_learned else learned_leg(mesh, a.steps, a.learned_epochs)
     cavity = None if a.no_cavity else cavity_leg(1)
@@ -435,6 +474,7 @@ def main() -> int:
             "cavity_16_fp64": cavity,
             "k_omega_step_40k": turb,
             "sst_coupled_pipe_40k_re200": coupled,
+            "helix_207k_fp32_amg": helix,
             "reference_published_benchmarks": refl,
             "parallelism": "single GPU",
         },
