@@ -1045,8 +1045,22 @@ H8RunResult H8Solver::run() {
         o.amgPrecision.c_str(), o.pressureSolver.c_str(), setupMs_);
     std::fflush(stdout);
   }
-  hipEvent_t ev[7];
-  for (auto& e : ev) NODALS_HIP(hipEventCreate(&e));
+  // Stage events, double-buffered: an iteration's events are read at the
+  // next iteration (after its continuity norm, a host synchronisation, has
+  // passed them), so the stream is not drained at every iteration's end.
+  hipEvent_t evs[2][7];
+  for (auto& b : evs)
+    for (auto& e : b) NODALS_HIP(hipEventCreate(&e));
+  auto harvest = [&](hipEvent_t* e) {
+    float ms[5];
+    for (int k = 0; k < 5; ++k) NODALS_HIP(hipEventElapsedTime(&ms[k], e[k], e[k + 1]));
+    R.assemblyMs += ms[0];
+    R.momentumMs += ms[1];
+    R.continuityMs += ms[2];
+    R.pressureMs += ms[3];
+    R.pupdateMs += ms[4];
+  };
+  bool pending = false;  // the other buffer holds an unread iteration
   bool converged = false, pressureAll = true, finiteAll = true;
   double cont0 = -1.0, contRel = 1.0;
   long long sumP = 0;
@@ -1059,6 +1073,7 @@ H8RunResult H8Solver::run() {
   const auto wall0 = std::chrono::steady_clock::now();
   try {
     for (int it = 1; it <= o.maxOuter; ++it) {
+      hipEvent_t* ev = evs[it & 1];
       NODALS_HIP(hipEventRecord(ev[0], stream_));
       assemble(o.alphaU, o.rauScale);
       NODALS_HIP(hipEventRecord(ev[1], stream_));
@@ -1123,25 +1138,21 @@ H8RunResult H8Solver::run() {
       NODALS_HIP(hipEventRecord(ev[4], stream_));
       pressureUpdate(o.alphaP);
       NODALS_HIP(hipEventRecord(ev[5], stream_));
+      if (pending) harvest(evs[(it & 1) ^ 1]);  // complete: the continuity norm above waited for it
+      pending = false;
       if (turbCouple_) {
         turbAdvance();
         NODALS_HIP(hipEventRecord(ev[6], stream_));
-      }
-      NODALS_HIP(hipEventSynchronize(turbCouple_ ? ev[6] : ev[5]));
-      float ms[5];
-      for (int k = 0; k < 5; ++k) NODALS_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
-      if (turbCouple_) {
+        NODALS_HIP(hipEventSynchronize(ev[6]));
+        harvest(ev);
         float tm = 0;
         NODALS_HIP(hipEventElapsedTime(&tm, ev[5], ev[6]));
         R.turbMs += tm;
         TurbStepResult tr;
         turbCounters(&tr);
+      } else {
+        pending = true;
       }
-      R.assemblyMs += ms[0];
-      R.momentumMs += ms[1];
-      R.continuityMs += ms[2];
-      R.pressureMs += ms[3];
-      R.pupdateMs += ms[4];
       finiteAll = finiteAll && std::isfinite(contRel) && std::isfinite(pr.rel);
       bool allMet = false;
       if (audited) {
@@ -1163,11 +1174,14 @@ H8RunResult H8Solver::run() {
       if (converged) break;
     }
   } catch (...) {
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& b : evs)
+      for (auto& e : b) (void)hipEventDestroy(e);
     throw;
   }
   sync();
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (pending) harvest(evs[finalIt & 1]);
+  for (auto& b : evs)
+    for (auto& e : b) (void)hipEventDestroy(e);
   R.loopMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
   R.outer = finalIt;
   R.converged = converged ? 1 : 0;

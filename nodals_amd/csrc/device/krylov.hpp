@@ -470,7 +470,17 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     for (int i = 0; i < k; ++i) c.c[i] = y[(std::size_t)i];
     hipLaunchKernelGGL(k_multi_axpy, nb, 256, 0, s, N, k, Z, c, W.x.data(), 1.0);
     NODALS_LAUNCHED();
-    // true FP64 residual
+    // A cycle whose residual estimate met the target ends the solve: the
+    // estimate is |b - A x| up to FP64 rounding (w = A z_j and its
+    // orthogonalisation are FP64 whatever the preconditioner's precision),
+    // so no residual pass is spent on it.
+    if (done && k > 0 && hs[fg::hEst + k - 1] <= target) {
+      out.its = its;
+      out.rel = hs[fg::hEst + k - 1] / r0;
+      out.ok = true;
+      return out;
+    }
+    // true FP64 residual for the restart
     A(s, W.x.data(), W.r.data());
     hipLaunchKernelGGL(k_xpby<double>, nb, 256, 0, s, N, rhs, -1.0, W.r.data());
     NODALS_LAUNCHED();

@@ -565,8 +565,11 @@ def richardson(A, M, b, rtol, atol, maxit, omega=1.0):
 
 
 def fgmres(A, M, b, rtol, atol, maxit, restart=30):
-    """Right-preconditioned flexible GMRES(m) with the true residual
-    recomputed at every restart (textbook FGMRES, Saad 1993)."""
+    """Right-preconditioned flexible GMRES(m) (textbook FGMRES, Saad 1993).
+    A cycle whose least-squares residual reaches the target ends the solve:
+    that residual equals |b - A x| up to FP64 rounding, because w = A z_j
+    and its orthogonalisation are FP64 whatever M's precision.  Otherwise
+    the true residual is recomputed for the restart."""
     n = b.size
     x = np.zeros(n)
     r = b.copy()
@@ -584,6 +587,7 @@ def fgmres(A, M, b, rtol, atol, maxit, restart=30):
         H = np.zeros((restart + 1, restart))
         V[0] = r / beta
         k = 0
+        est = None
         for j in range(restart):
             if its >= maxit:
                 break
@@ -601,12 +605,16 @@ def fgmres(A, M, b, rtol, atol, maxit, restart=30):
             y, *_ = np.linalg.lstsq(H[:k + 1, :k], e1, rcond=None)
             res = np.linalg.norm(e1 - H[:k + 1, :k] @ y)
             if res <= target or H[j + 1, j] == 0.0:
+                est = res
                 break
             V[j + 1] = w / H[j + 1, j]
         e1 = np.zeros(k + 1)
         e1[0] = beta
         y, *_ = np.linalg.lstsq(H[:k + 1, :k], e1, rcond=None)
         x = x + y @ Z[:k]
+        if est is not None and est <= target:
+            out.update(its=its, rel=est / r0, ok=True)
+            return x, out
         r = b - A @ x
         beta = float(np.linalg.norm(r))
         out.update(its=its, rel=beta / r0)
