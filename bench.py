@@ -381,43 +381,49 @@ def main() -> int:
         return 0
 
     results = {}
-    ctxs = {}
+    # One solver context alive at a time.  A context runs the fine Schur
+    # refresh on a second stream beside the momentum solve; streams share a
+    # small pool of hardware queues, and with another context alive the two
+    # streams of the new one can share a queue and lose the overlap.
     for prec in ("fp64", "fp32"):
         h = H8(solver_args(mesh, prec))
-        ctxs[prec] = h
-        for _ in range(a.warmup):
+        try:
+            for _ in range(a.warmup):
+                reset(h)
+                h.run()
+            times, runs = [], []
+            for _ in range(a.steps):
+                reset(h)
+                t0 = time.perf_counter()
+                r = h.run()
+                times.append(time.perf_counter() - t0)
+                runs.append(r)
+                print(f"[bench] {prec} step: {times[-1]:.3f} s, outer={r['outer']} converged={r['converged']} "
+                      f"avgP={r['avg_pressure_its']:.2f} dp={r['pressure_drop']:.6f}", file=sys.stderr, flush=True)
+            results[prec] = {"times": times, "runs": runs}
+            if prec != "fp32":
+                continue
+            info = dict(h.info)
+            # Roofline kernel: the FP64 fine Schur SpMV (the Krylov operator).
+            # Its launches inside one more full solve of the same workload are
+            # bracketed by HIP events on the solver stream (kept out of the
+            # timed steps, whose wall clock the extra events would perturb).
             reset(h)
+            h.probe_start()
             h.run()
-        times, runs = [], []
-        for _ in range(a.steps):
-            reset(h)
-            t0 = time.perf_counter()
-            r = h.run()
-            times.append(time.perf_counter() - t0)
-            runs.append(r)
-            print(f"[bench] {prec} step: {times[-1]:.3f} s, outer={r['outer']} converged={r['converged']} "
-                  f"avgP={r['avg_pressure_its']:.2f} dp={r['pressure_drop']:.6f}", file=sys.stderr, flush=True)
-        results[prec] = {"times": times, "runs": runs}
-
-    import numpy as np
-    h = ctxs["fp32"]
-    # Roofline kernel: the FP64 fine Schur SpMV (the Krylov operator).  Its
-    # launches inside one more full solve of the same workload are bracketed
-    # by HIP events on the solver stream (kept out of the timed steps, whose
-    # wall clock the extra events would perturb).
-    reset(h)
-    h.probe_start()
-    h.run()
-    launches, probe_ms = h.probe_stop()
-    ms = probe_ms / max(launches, 1)
-    _, by = h.time_kernel("fine_spmv", 1)
-    achieved = by / (ms * 1e-3) / 1e9
-    warm_ms, _ = h.time_kernel("fine_spmv", a.spmv_reps)
-    fine_lanes = int(h.get("fine_lanes", 1)[0])
-    extra = {}
-    for k in ("refresh", "mom_gs", "assemble", "continuity"):
-        kms, kby = h.time_kernel(k, 100)
-        extra[k] = {"ms": kms, "GB_s": kby / (kms * 1e-3) / 1e9}
+            launches, probe_ms = h.probe_stop()
+            ms = probe_ms / max(launches, 1)
+            _, by = h.time_kernel("fine_spmv", 1)
+            achieved = by / (ms * 1e-3) / 1e9
+            warm_ms, _ = h.time_kernel("fine_spmv", a.spmv_reps)
+            fine_lanes = int(h.get("fine_lanes", 1)[0])
+            extra = {}
+            for k in ("refresh", "mom_gs", "assemble", "continuity"):
+                kms, kby = h.time_kernel(k, 100)
+                extra[k] = {"ms": kms, "GB_s": kby / (kms * 1e-3) / 1e9}
+            turb = None if a.no_turb else turb_leg(h, 20)
+        finally:
+            h.close()
 
     fp32 = results["fp32"]
     fp64 = results["fp64"]
@@ -425,7 +431,6 @@ def main() -> int:
     outer = int(fp32["runs"][-1]["outer"])
     dp32 = fp32["runs"][-1]["pressure_drop"]
     dp64 = fp64["runs"][-1]["pressure_drop"]
-    turb = None if a.no_turb else turb_leg(h, 20)
     coupled = None if a.no_sst_coupled else sst_coupled_leg(mesh)
     helix = None if a.no_helix else helix_leg()
     poisson = None if a.no_poisson else poisson_leg(64, max(1, a.steps))
@@ -456,7 +461,7 @@ def main() -> int:
             "workload": ("laminar straight pipe D=1 L=10D Re=20, O-grid 224 quads x 30 slabs, 40,320 tets; "
                          "SIMPLE from rest to residual 1e-6 (FGS momentum, SA-AMG FGMRES(30) pressure at the reference's "
                          "default p-rtol 0.5)"),
-            "cells": ctxs["fp32"].info["cells"], "free_velocity_dofs": ctxs["fp32"].info["nv"],
+            "cells": info["cells"], "free_velocity_dofs": info["nv"],
             "fp32_amg_s": value, "fp64_s": float(np.mean(fp64["times"])),
             "speedup_fp32_amg_vs_fp64": float(np.mean(fp64["times"])) / value,
             "outer_iterations_fp32_amg": outer, "outer_iterations_fp64": int(fp64["runs"][-1]["outer"]),
@@ -489,8 +494,6 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    for h in ctxs.values():
-        h.close()
     return 0
 
 

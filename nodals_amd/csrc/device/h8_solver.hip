@@ -311,6 +311,15 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   }
   NODALS_HIP(hipSetDevice(0));
   NODALS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // second stream (default priority, like the first) for the fine Schur
+  // refresh, which depends on the assembly's rAU but not on the momentum
+  // solve, so the two overlap inside a SIMPLE iteration.  Streams share a
+  // small pool of hardware queues (GPU_MAX_HW_QUEUES, 4 by default): with a
+  // second context alive in the process, this context's two streams can land
+  // on one queue and then run one after the other (same results, no overlap).
+  NODALS_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+  NODALS_HIP(hipEventCreateWithFlags(&evAsm_, hipEventDisableTiming));
+  NODALS_HIP(hipEventCreateWithFlags(&evRef_, hipEventDisableTiming));
   red_.init(32);
   upload();
   setupAmg();
@@ -324,6 +333,9 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
 
 H8Solver::~H8Solver() {
   for (hipEvent_t e : probeEv_) (void)hipEventDestroy(e);
+  if (evAsm_) (void)hipEventDestroy(evAsm_);
+  if (evRef_) (void)hipEventDestroy(evRef_);
+  if (stream2_) (void)hipStreamDestroy(stream2_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -713,14 +725,18 @@ double H8Solver::continuity() {
 
 void H8Solver::refresh() {
   requireDevice();
-  hipLaunchKernelGGL(k_fine_weights, blocksFor((long long)nc_ * 8), 256, 0, stream_, nc_, cref_.data(), bcoef_.data(),
+  refreshOn(stream_);
+}
+
+void H8Solver::refreshOn(hipStream_t s) {
+  hipLaunchKernelGGL(k_fine_weights, blocksFor((long long)nc_ * 8), 256, 0, s, nc_, cref_.data(), bcoef_.data(),
                      rau_.data(), fineW_.data());
   NODALS_LAUNCHED();
   const int grid = blocksFor((long long)nc_ * refreshLanes_);
   const bool f32 = opt_.amgPrecision == "fp32", b16 = opt_.amgPrecision == "bf16";
   const bool l1 = opt_.amgSmoother == "l1jacobi";
 #define NODALS_REFRESH(L, F, B, LL)                                                                                  \
-  hipLaunchKernelGGL((k_fine_refresh<L, F, B, LL>), grid, 256, 0, stream_, nc_, frow_.data(), fcol_.data(),         \
+  hipLaunchKernelGGL((k_fine_refresh<L, F, B, LL>), grid, 256, 0, s, nc_, frow_.data(), fcol_.data(),               \
                      fdiagPos_.data(), nzOff_.data(), nzTerm_.data(), bcoef_.data(), fineW_.data(), S_.pinCell,     \
                      fval_.data(), fvalf_.data(), fvalb_.data(), fdiag_.data(), fdiagf_.data(), fl1_.data(),        \
                      fl1f_.data())
@@ -1077,6 +1093,12 @@ H8RunResult H8Solver::run() {
       NODALS_HIP(hipEventRecord(ev[0], stream_));
       assemble(o.alphaU, o.rauScale);
       NODALS_HIP(hipEventRecord(ev[1], stream_));
+      // the fine Schur refresh needs rAU only: it runs on the second stream
+      // beside the momentum solve and the continuity residual
+      NODALS_HIP(hipEventRecord(evAsm_, stream_));
+      NODALS_HIP(hipStreamWaitEvent(stream2_, evAsm_, 0));
+      refreshOn(stream2_);
+      NODALS_HIP(hipEventRecord(evRef_, stream2_));
       if (it == 1 && !turbCouple_) {  // the setup snapshot is of the laminar operator
         const double rr = rauSnapshotRel();
         if (verbose)
@@ -1097,7 +1119,7 @@ H8RunResult H8Solver::run() {
       if (it == 1) cont0 = cn;
       contRel = cn / std::max(cont0, 1e-300);
       NODALS_HIP(hipEventRecord(ev[3], stream_));
-      refresh();
+      NODALS_HIP(hipStreamWaitEvent(stream_, evRef_, 0));
       // learned warm start and sample recording
       int slot = -1;
       const double* guess = nullptr;

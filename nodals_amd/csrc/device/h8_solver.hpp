@@ -200,6 +200,9 @@ class H8Solver {
   FineTopo F_;
   AmgHierarchy H_;
   hipStream_t stream_ = nullptr;
+  hipStream_t stream2_ = nullptr;  // the refresh, overlapped with the momentum solve
+  hipEvent_t evAsm_ = nullptr, evRef_ = nullptr;
+  void refreshOn(hipStream_t s);
   dev::Reducer red_;
   dev::KrylovWork kw_;
 
