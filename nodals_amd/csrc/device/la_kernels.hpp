@@ -22,6 +22,41 @@ __device__ __forceinline__ T groupSum(T v, int lanes) {
   return v;
 }
 
+// A lane's share of a CSR row product: s = sum of val(k) * gather(col[k])
+// over k = k0, k0 + LANES, ... < e, accumulated in exactly that order (the
+// order of the one-at-a-time loop, so results are bit-identical to it).  Each
+// step first issues U column loads, U value loads and U gathers and only then
+// accumulates, so a lane keeps U independent memory chains in flight instead
+// of one (measured on gfx950: the 40k-cell P0 Schur product, 65 nnz/row,
+// 8.7 -> 6.9 us; the 64^3 P1 Laplacian, 15 nnz/row, 17.2 -> 12.2 us at four
+// lanes).  Slots past the row end are predicated off and never touch s.
+// Entries in column `skip` are left out (-1: none).
+template <int LANES>
+constexpr int rowUnroll() {
+  return LANES == 4 ? 4 : 2;
+}
+template <int LANES, class TS, class Val, class Gather>
+__device__ __forceinline__ TS rowDot(int k0, int e, const int* __restrict__ col, int skip, Val val, Gather gather) {
+  constexpr int U = rowUnroll<LANES>();
+  TS s = 0;
+  for (int k = k0; k < e; k += U * LANES) {
+    int c[U];
+    TS a[U], v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kk = k + u * LANES;
+      c[u] = kk < e ? col[kk] : -1;
+      a[u] = kk < e ? val(kk) : TS(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = c[u] >= 0 ? gather(c[u]) : TS(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c[u] >= 0 && c[u] != skip) s += a[u] * v[u];
+  }
+  return s;
+}
+
 // ---------------------------------------------------------------- BLAS-1
 template <class T>
 __global__ void k_fill(long long n, T* x, T a) {
@@ -179,9 +214,8 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int* __restrict__ row
   const int r = gid / LANES;
   const int lane = gid % LANES;
   if (r >= n) return;
-  TV s = 0;
-  const int e = row[r + 1];
-  for (int k = row[r] + lane; k < e; k += LANES) s += ld<TV>(val, k) * x[col[k]];
+  TV s = rowDot<LANES, TV>(
+      row[r] + lane, row[r + 1], col, -1, [&](int k) { return ld<TV>(val, k); }, [&](int j) { return x[j]; });
   s = groupSum(s, LANES);
   if (lane == 0) {
     if (RESID) y[r] = b[r] - s;
@@ -202,9 +236,8 @@ __global__ void __launch_bounds__(256) k_schur_spmv(int n, const int* __restrict
   const int r = gid / LANES;
   const int lane = gid % LANES;
   if (r >= n) return;
-  double s = 0;
-  const int e = row[r + 1];
-  for (int k = row[r] + lane; k < e; k += LANES) s += val[k] * x[col[k]];
+  double s = rowDot<LANES, double>(
+      row[r] + lane, row[r + 1], col, -1, [&](int k) { return val[k]; }, [&](int j) { return x[j]; });
   s = groupSum(s, LANES);
   if (lane == 0) y[r] = s;
 }
@@ -226,13 +259,9 @@ __global__ void __launch_bounds__(256) k_jacobi_pre_resid(int n, const int* __re
   const int i = gid / LANES;
   const int lane = gid % LANES;
   if (i >= n) return;
-  TV s = 0;
-  const int e = row[i + 1];
-  for (int k = row[i] + lane; k < e; k += LANES) {
-    const int j = col[k];
-    const TV tj = (TV)omega * (TV)b[j] / d[j];
-    s += ld<TV>(val, k) * tj;
-  }
+  TV s = rowDot<LANES, TV>(
+      row[i] + lane, row[i + 1], col, -1, [&](int k) { return ld<TV>(val, k); },
+      [&](int j) { return (TV)omega * (TV)b[j] / d[j]; });
   s = groupSum(s, LANES);
   if (lane == 0) {
     const TV bi = (TV)b[i];
@@ -253,9 +282,8 @@ __global__ void __launch_bounds__(256) k_resid_jacobi_post(int n, const int* __r
   const int i = gid / LANES;
   const int lane = gid % LANES;
   if (i >= n) return;
-  TV s = 0;
-  const int e = row[i + 1];
-  for (int k = row[i] + lane; k < e; k += LANES) s += ld<TV>(val, k) * t[col[k]];
+  TV s = rowDot<LANES, TV>(
+      row[i] + lane, row[i + 1], col, -1, [&](int k) { return ld<TV>(val, k); }, [&](int j) { return t[j]; });
   s = groupSum(s, LANES);
   if (lane == 0) {
     const TV r = (TV)b[i] - s;
@@ -282,12 +310,9 @@ __global__ void __launch_bounds__(256) k_resid_jacobi_post_agg(
   const int i = gid / LANES;
   const int lane = gid % LANES;
   if (i >= n) return;
-  TV s = 0;
-  const int e = row[i + 1];
-  for (int k = row[i] + lane; k < e; k += LANES) {
-    const int j = col[k];
-    s += ld<TV>(val, k) * (t[j] + mulRn(pval[j], xc[pcol[j]]));
-  }
+  TV s = rowDot<LANES, TV>(
+      row[i] + lane, row[i + 1], col, -1, [&](int k) { return ld<TV>(val, k); },
+      [&](int j) { return t[j] + mulRn(pval[j], xc[pcol[j]]); });
   s = groupSum(s, LANES);
   if (lane == 0) {
     const TV ti = t[i] + mulRn(pval[i], xc[pcol[i]]);
@@ -325,11 +350,8 @@ __global__ void __launch_bounds__(256) k_mcgs_color(int begin, int count, const 
   const int lane = gid % LANES;
   if (q >= count) return;
   const int i = crow[begin + q];
-  TV s = 0;
-  for (int k = row[i] + lane; k < row[i + 1]; k += LANES) {
-    const int j = col[k];
-    if (j != i) s += ld<TV>(val, k) * x[j];
-  }
+  TV s = rowDot<LANES, TV>(
+      row[i] + lane, row[i + 1], col, i, [&](int k) { return ld<TV>(val, k); }, [&](int j) { return x[j]; });
   s = groupSum(s, LANES);
   if (lane == 0) {
     const TV old = x[i];
@@ -360,15 +382,17 @@ __global__ void k_diag_gather(int n, const int* __restrict__ diagPos, const T* _
 }
 
 // ---------------------------------------------------------------- launch helpers
-// Lanes per CSR row from the mean row length and the number of rows.  Eight
-// lanes per row for rows up to 128 nonzeros (measured on gfx950: 64^3 P1
-// Laplacian, 15 nnz/row, and the 40k-cell P0 Schur operator, 65 nnz/row,
-// both beat 16 and 32 lanes), wider groups for longer rows.  When the rows
+// Lanes per CSR row from the mean row length and the number of rows.  Four
+// lanes (four loads in flight each) for rows up to 20 nonzeros, eight (two
+// each) up to 128 (measured on gfx950: the 64^3 P1 Laplacian, 15 nnz/row,
+// runs in 12.2 us at four lanes against 14.3 at eight; the 40k-cell P0 Schur
+// operator, 65 nnz/row, in 6.9 us at eight against 9.7 at four and 7.5 at
+// sixteen), wider groups for longer rows.  When the rows
 // are too few to give the chip 2^17 threads (two waves per SIMD), the group
 // widens further while every lane keeps at least two nonzeros: a 1,073-row
 // coarse level with 88 nnz/row runs in 2.7 us at 32 lanes and 4.7 us at 8.
 inline int lanesFor(double meanRow, long long rows) {
-  int L = meanRow <= 6 ? 4 : meanRow <= 128 ? 8 : meanRow <= 256 ? 16 : meanRow <= 512 ? 32 : 64;
+  int L = meanRow <= 20 ? 4 : meanRow <= 128 ? 8 : meanRow <= 256 ? 16 : meanRow <= 512 ? 32 : 64;
   while (L < 64 && rows * L < (1LL << 17) && 4.0 * L <= meanRow) L *= 2;
   return L;
 }
