@@ -96,6 +96,12 @@ class H8Solver {
   void assemble(double alphaU, double rauScale);
   const char* momentumWork(const std::string& mode, double omega, int outer);
   std::array<double, 3> momentumAudit();
+  // The audit and the continuity norm in two halves: the launch (sums land
+  // in pinned host memory out[], no synchronisation) and the read, valid
+  // once the stream has passed the launch.
+  void momentumAuditLaunch(double* out);
+  static std::array<double, 3> momentumAuditRead(const double* out);
+  void continuityLaunch(double* out);
   double continuity();
   void refresh();
   dev::KrylovResult pressureSolve(const std::string& solver, double rtol, double atol, int maxit);
@@ -202,6 +208,9 @@ class H8Solver {
   hipStream_t stream_ = nullptr;
   hipStream_t stream2_ = nullptr;  // the refresh, overlapped with the momentum solve
   hipEvent_t evAsm_ = nullptr, evRef_ = nullptr;
+  // pinned: [0] the continuity residual's square sum, [1..6] the momentum
+  // audit's sums, read by the SIMPLE loop after its pressure solve
+  double* loopHost_ = nullptr;
   void refreshOn(hipStream_t s);
   dev::Reducer red_;
   dev::KrylovWork kw_;

@@ -622,6 +622,7 @@ class Reducer {
   }
   double* partials() { return part_.data(); }
   double* hostOut() { return host_; }
+  const double* hostOut() const { return host_; }
 
  private:
   DBuf<double> part_;
