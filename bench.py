@@ -233,18 +233,23 @@ def turb_leg(h, steps: int) -> dict:
     return out
 
 
+# The SST case's solver settings (examples/spiral_swept_207k/
+# SPIRAL_SWEPT_ADAPTIVE.case): 20 momentum sweeps at omega 0.7 ([solver.u]
+# m3-fgs), 20 + 20 k-omega sweeps ([sst]), pressure to rtol 1e-3
+# ([solver.p]), alpha_u 0.7 / alpha_p 0.3 ([simple]).
+SST_SOLVER = ["--mom-sweeps", "20", "--mom-omega", "0.7", "--p-rtol", "1e-3", "--alpha-u", "0.7", "--alpha-p", "0.3"]
+
+
 def sst_coupled_leg(mesh: str) -> dict:
     """Coupled SST SIMPLE on the same 40,320-tet pipe at Re = 200: eddy
     viscosity in the momentum and one k-omega step after every pressure
     update, from rest (k, omega at their inlet values) to residual 1e-5,
-    alpha_u 0.7 / alpha_p 0.3 (the spiral case's under-relaxation), FP32
-    AMG, FGMRES to 1e-2."""
+    with the SST case's solver settings, FP32 AMG in FGMRES."""
     from nodals_amd._lib import H8
 
     args = ["--mesh", mesh, "--run-mode", "converge", "--max-outer", "5000", "--simple-tol", "1e-5",
-            "--alpha-u", "0.7", "--alpha-p", "0.3", "--amg-smoother", "jacobi", "--amg-jacobi-omega", "0.7",
-            "--pressure-solver", "fgmres", "--p-rtol", "1e-2", "--p-max", "200", "--amg-precision", "fp32",
-            "--verbose", "0", "--re", "200"]
+            "--amg-smoother", "jacobi", "--amg-jacobi-omega", "0.7", "--pressure-solver", "fgmres", "--p-max", "200",
+            "--amg-precision", "fp32", "--verbose", "0", "--re", "200"] + SST_SOLVER
     h = H8(args)
     h.turb_init([])
     h.turb_couple(True)
@@ -265,8 +270,8 @@ def helix_leg() -> dict:
     multicolour Gauss-Seidel smoothing (the spiral case's smoother; the
     section cells are about 20 times longer than wide, which Jacobi-smoothed
     aggregation does not handle).  Laminar Re = 20 to 1e-6, and coupled SST at
-    Re = 200 to 1e-5 (alpha 0.7 / 0.3, pressure to 1e-2, 20 + 20 k-omega
-    sweeps)."""
+    Re = 200 to 1e-5 with the SST case's solver settings and its pressure
+    hierarchy (CF/PMIS, extended+i interpolation, theta 0.25, MCGS)."""
     from nodals_amd import meshgen
     from nodals_amd._lib import H8
 
@@ -278,8 +283,8 @@ def helix_leg() -> dict:
     out = {}
     for key, extra, sst in (("laminar_re20", ["--re", "20", "--simple-tol", "1e-6", "--alpha-u", "0.5", "--alpha-p", "0.5",
                                               "--p-rtol", "0.5"], False),
-                            ("sst_re200", ["--re", "200", "--simple-tol", "1e-5", "--alpha-u", "0.7", "--alpha-p", "0.3",
-                                           "--p-rtol", "1e-2"], True)):
+                            ("sst_re200", ["--re", "200", "--simple-tol", "1e-5", "--amg-hierarchy", "cf",
+                                           "--cf-interp", "exti"] + SST_SOLVER, True)):
         h = H8(base + extra)
         if sst:
             h.turb_init([])

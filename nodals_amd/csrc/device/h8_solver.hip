@@ -178,6 +178,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--amg-mcgs-coarse-sweeps") o.amgMcgsCoarseSweeps = inum(next());
     else if (a == "--amg-mcgs-order") o.amgMcgsOrder = next();
     else if (a == "--momentum-work") o.momentumWork = next();
+    else if (a == "--mom-sweeps") o.momSweeps = inum(next());
     else if (a == "--run-mode") o.runMode = next();
     else if (a == "--amg-precision") o.amgPrecision = next();
     else if (a == "--pressure-restart") o.pressureRestart = inum(next());
@@ -232,6 +233,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
   require(o.fineCsrRefreshEvery == 1, "H8 requires fine CSR refreshEvery=1");
   require(o.momentumWork == "fgs1" || o.momentumWork == "altgs1" || o.momentumWork == "sgs1",
           "H8 momentum-work must be fgs1, altgs1 or sgs1");
+  require(o.momSweeps >= 1 && o.momSweeps <= 60, "H8 mom-sweeps must be in [1,60]");
   require(o.runMode == "fixed10" || o.runMode == "converge", "H8 run-mode must be fixed10 or converge");
   require(!(o.runMode == "fixed10" && o.maxOuter != 10), "H8 fixed10 requires maxOuter=10");
   require(o.amgPrecision == "fp64" || o.amgPrecision == "fp32" || o.amgPrecision == "bf16",
@@ -652,8 +654,9 @@ void H8Solver::assemble(double alphaU, double rauScale) {
   }
 }
 
-const char* H8Solver::momentumWork(const std::string& mode, double omega, int outer) {
+const char* H8Solver::momentumWork(const std::string& mode, double omega, int outer, int sweeps) {
   requireDevice();
+  require(sweeps >= 1, "momentum work needs at least one sweep");
   auto sweep = [&](bool reverse) {
     const int nco = (int)colorOff_.size() - 1;
     for (int q = 0; q < nco; ++q) {
@@ -667,25 +670,31 @@ const char* H8Solver::momentumWork(const std::string& mode, double omega, int ou
       NODALS_LAUNCHED();
     }
   };
-  // One colour launch per colour; the whole sweep is replayed from a graph.
-  const std::string key = graphKey({u_.data(), aval_.data(), brhs_.data()}, omega);
+  // One colour launch per colour; the unit (a sweep, or a forward and a
+  // backward sweep) repeated `sweeps` times is replayed from one graph.
+  const std::string key = graphKey({u_.data(), aval_.data(), brhs_.data()}, omega) + "x" + std::to_string(sweeps);
+  auto repeat = [&](int graph, const std::string& tag, bool fwd, bool bwd) {
+    sweepGraph_[graph].run(stream_, key + tag, [&]() {
+      for (int k = 0; k < sweeps; ++k) {
+        if (fwd) sweep(false);
+        if (bwd) sweep(true);
+      }
+    });
+  };
   if (mode == "fgs1") {
-    sweepGraph_[0].run(stream_, key + "F", [&]() { sweep(false); });
+    repeat(0, "F", true, false);
     return "FWD";
   }
   if (mode == "altgs1") {
     if (outer & 1) {
-      sweepGraph_[0].run(stream_, key + "F", [&]() { sweep(false); });
+      repeat(0, "F", true, false);
       return "FWD";
     }
-    sweepGraph_[1].run(stream_, key + "B", [&]() { sweep(true); });
+    repeat(1, "B", false, true);
     return "BWD";
   }
   if (mode == "sgs1") {
-    sweepGraph_[0].run(stream_, key + "S", [&]() {
-      sweep(false);
-      sweep(true);
-    });
+    repeat(0, "S", true, true);
     return "FWD+BWD";
   }
   fail("unknown fixed momentum work mode " + mode);
@@ -1128,7 +1137,7 @@ H8RunResult H8Solver::run() {
         momentumAuditLaunch(loopHost_ + 1);
         audited = true;
       }
-      const char* dir = momentumWork(o.momentumWork, o.momOmega, it);
+      const char* dir = momentumWork(o.momentumWork, o.momOmega, it, o.momSweeps);
       NODALS_HIP(hipEventRecord(ev[2], stream_));
       continuityLaunch(loopHost_);
       NODALS_HIP(hipEventRecord(ev[3], stream_));

@@ -57,6 +57,10 @@ struct H8Options {
   // Extensions of this implementation.
   std::string amgPrecision = "fp64";  // fp64 | fp32 | bf16 (finest level bf16, coarse fp32)
   int pressureRestart = 30;           // FGMRES restart length
+  // momentum-work units per SIMPLE iteration (the SST path's fixed M3 sweep
+  // count, NODALS_M3_SWEEPS; with no-slip walls every 3x3 block is a scalar
+  // times the identity, so an M3 forward sweep is this coloured sweep)
+  int momSweeps = 1;
   int verbose = 1;
 };
 
@@ -94,7 +98,7 @@ class H8Solver {
   hipStream_t stream() const { return stream_; }
 
   void assemble(double alphaU, double rauScale);
-  const char* momentumWork(const std::string& mode, double omega, int outer);
+  const char* momentumWork(const std::string& mode, double omega, int outer, int sweeps = 1);
   std::array<double, 3> momentumAudit();
   // The audit and the continuity norm in two halves: the launch (sums land
   // in pinned host memory out[], no synchronisation) and the read, valid

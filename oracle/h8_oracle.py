@@ -643,13 +643,19 @@ def pressure_drop_fit(case: H8Case, p: np.ndarray) -> float:
 
 def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work="fgs1", p_solver="pcg",
            p_rtol=0.5, p_atol=1e-12, p_max=20, smoother: Optional[Smoother] = None, run_mode="fixed10",
-           max_outer=10, simple_tol=1e-6, restart=30, U0=None, p0=None, callback=None, extra_blocks=None):
+           max_outer=10, simple_tol=1e-6, restart=30, U0=None, p0=None, callback=None, extra_blocks=None,
+           mom_sweeps=1):
     """H8 SIMPLE loop (gpu/serial_cuda/h8_main_base.cu:2046-2119): assemble,
     B^T p + RHS, fixed momentum work, continuity, fine refresh, pressure
     Krylov + AMG, p += alpha_p dp; converge mode audits the momentum residual
     once the continuity target is met.  extra_blocks(U), when given, adds
     cell blocks to the momentum assembly; callback(it, U, p) runs after the
-    pressure update (the coupled SST mode advances k and omega there)."""
+    pressure update (the coupled SST mode advances k and omega there).
+    mom_sweeps repeats the momentum work unit that many times per
+    iteration, as the SST path's fixed M3 sweep count
+    (NODALS_M3_SWEEPS, turbulence/sst_gpu_v2/source/
+    puretet_profile5_eq5_hxt5b/modules/app/sst_application.inc:702;
+    its sweeps are m3_fixed_coupled_csr, simple_pressure_predictor.inc:171)."""
     smoother = smoother or Smoother()
     _, T_cent = tensors()
     U = np.zeros((3, case.nv)) if U0 is None else U0.copy()
@@ -667,13 +673,14 @@ def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work=
         audit = None
         if run_mode == "converge" and it > 1 and cont_rel <= simple_tol:
             audit = momentum_audit(A, b, U)
-        if momentum_work == "fgs1":
-            U = gs_sweep(case, A, diag, b, U, mom_omega)
-        elif momentum_work == "sgs1":
-            U = gs_sweep(case, A, diag, b, U, mom_omega)
-            U = gs_sweep(case, A, diag, b, U, mom_omega, reverse=True)
-        elif momentum_work == "altgs1":
-            U = gs_sweep(case, A, diag, b, U, mom_omega, reverse=(it % 2 == 0))
+        for _ in range(mom_sweeps):
+            if momentum_work == "fgs1":
+                U = gs_sweep(case, A, diag, b, U, mom_omega)
+            elif momentum_work == "sgs1":
+                U = gs_sweep(case, A, diag, b, U, mom_omega)
+                U = gs_sweep(case, A, diag, b, U, mom_omega, reverse=True)
+            elif momentum_work == "altgs1":
+                U = gs_sweep(case, A, diag, b, U, mom_omega, reverse=(it % 2 == 0))
         r = continuity(case, U)
         cn = float(np.linalg.norm(r))
         if cont0 is None:

@@ -49,6 +49,10 @@ def test_usage_errors_are_status_codes(mesh_dir):
         H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--supg", "1"], device=False)
     with pytest.raises(NodalsError, match="cannot open"):
         H8(["--mesh", "/nonexistent/polyMesh", "--max-outer", "10"], device=False)
+    for n in ("0", "61"):
+        with pytest.raises(NodalsError, match="mom-sweeps must be in"):
+            H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--mom-sweeps", n], device=False)
+    H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--mom-sweeps", "20"], device=False).close()
 
 
 def test_host_only_handle_rejects_device_calls(mesh_dir):

@@ -221,6 +221,31 @@ def test_simple_fixed10_matches_oracle(mesh_dir):
     h.close()
 
 
+@pytest.mark.parametrize("work,sweeps", [("fgs1", 4), ("sgs1", 2)])
+def test_simple_fixed10_momentum_sweeps_match_oracle(mesh_dir, work, sweeps):
+    """Several momentum work units per SIMPLE iteration at omega 0.7 (the SST
+    path's fixed M3 sweep count and relaxation) against the oracle's loop
+    with the same count."""
+    from nodals_amd._lib import H8
+
+    print(f"\n[gpu] SIMPLE fixed10 small pipe momentum {work} x{sweeps} omega 0.7", flush=True)
+    h = H8(["--mesh", mesh_dir["small"], *BASE, "--amg-smoother", "jacobi", "--pressure-solver", "pcg",
+            "--momentum-work", work, "--mom-sweeps", str(sweeps), "--mom-omega", "0.7", "--verbose", "0"])
+    case = O.case_from_product(h)
+    r = h.run()
+    assert r["pass_"] == 1 and r["outer"] == 10
+    ref = O.simple(case, p_solver="pcg", smoother=O.Smoother(kind="jacobi"), max_outer=10, momentum_work=work,
+                   mom_sweeps=sweeps, mom_omega=0.7)
+    one = O.simple(case, p_solver="pcg", smoother=O.Smoother(kind="jacobi"), max_outer=10, momentum_work=work,
+                   mom_omega=0.7)
+    for d in range(3):
+        got = h.get(f"u{d}", case.nv)
+        assert rel(got, ref["U"][d]) < 1e-9
+        assert rel(got, one["U"][d]) > 1e-6  # the extra sweeps change the trajectory
+    assert rel(h.get("p", case.nc), ref["p"]) < 1e-9
+    h.close()
+
+
 def test_deterministic_bitwise(mesh_dir):
     from nodals_amd._lib import H8
 
