@@ -335,6 +335,10 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
 }
 
 H8Solver::~H8Solver() {
+  // a run that ended in an error can leave work queued that still writes
+  // the pinned loop results and device buffers freed below
+  if (stream2_) (void)hipStreamSynchronize(stream2_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
   for (hipEvent_t e : probeEv_) (void)hipEventDestroy(e);
   if (loopHost_) (void)hipHostFree(loopHost_);
   if (evAsm_) (void)hipEventDestroy(evAsm_);

@@ -350,12 +350,14 @@ __global__ void __launch_bounds__(256) k_mcgs_color(int begin, int count, const 
   const int lane = gid % LANES;
   if (q >= count) return;
   const int i = crow[begin + q];
+  // the row's own b, diagonal and value do not depend on the row product:
+  // their loads are issued with the row's, not after the reduction
+  const TV bi = b[i], di = diag[i], old = x[i];
   TV s = rowDot<LANES, TV>(
       row[i] + lane, row[i + 1], col, i, [&](int k) { return ld<TV>(val, k); }, [&](int j) { return x[j]; });
   s = groupSum(s, LANES);
   if (lane == 0) {
-    const TV old = x[i];
-    const TV gs = (b[i] - s) / diag[i];
+    const TV gs = (bi - s) / di;
     x[i] = old + (TV)omega * (gs - old);
   }
 }
