@@ -50,11 +50,19 @@ struct DevColoring {
   int ncolors = 0;
   std::vector<int> off;
   DBuf<int> rows;
+  bool contiguous = false;  // the operator is numbered in colour order: colour c is rows [off[c], off[c+1])
   void upload(const Coloring& C) {
     ncolors = C.ncolors;
     off = C.offset;
     rows.upload(C.rows);
+    contiguous = false;
   }
+  void uploadContiguous(const Coloring& C) {
+    ncolors = C.ncolors;
+    off = C.offset;
+    contiguous = true;
+  }
+  const int* rowList() const { return contiguous ? nullptr : rows.data(); }
 };
 
 template <class TF, class TV>
@@ -85,30 +93,58 @@ class DevAmg {
     P_.clear();
     L_.resize(nl);
     P_.resize(nl);
+    // An MCGS-smoothed coarse level is stored renumbered in colour order
+    // (the reference's greedy colouring, unchanged): every colour is then
+    // a contiguous row range and a colour pass needs no row list.  Each
+    // row keeps its entries in their order, and the transfers on both
+    // sides are relabelled the same way, so every sum is formed in the
+    // same order as in the original numbering and results are identical.
+    // perm[l][new] = original row; empty = not renumbered.
+    std::vector<std::vector<int>> perm(nl), inv(nl);
     for (std::size_t l = 0; l < nl; ++l) {
       const CsrD& A = H.A[l];
       Level& Lv = L_[l];
       Lv.n = A.n;
       Lv.meanRow = A.n ? (double)A.col.size() / A.n : 1.0;
       Lv.lanes = lanesFor(Lv.meanRow, A.n);
-      std::vector<int> r32(A.row.begin(), A.row.end());
-      Lv.row.upload(r32);
-      Lv.col.upload(A.col);
-      Lv.val.upload(castVec(A.val));
-      Lv.diag.upload(castVec(A.diag));
-      std::vector<double> l1((std::size_t)A.n);
+      if (opt_.kind == "mcgs" && l + 1 < nl) {
+        const Coloring C = greedyColoring(A.n, A.row.data(), A.col.data());
+        perm[l].assign(C.rows.begin(), C.rows.end());
+        inv[l].assign((std::size_t)A.n, 0);
+        for (int i = 0; i < A.n; ++i) inv[l][(std::size_t)perm[l][(std::size_t)i]] = i;
+        Lv.color.uploadContiguous(C);
+      }
+      const std::vector<int>& pm = perm[l];
+      const std::vector<int>& iv = inv[l];
+      auto orig = [&](int i) { return pm.empty() ? i : pm[(std::size_t)i]; };
+      auto relabel = [&](int j) { return iv.empty() ? j : iv[(std::size_t)j]; };
+      std::vector<int> r32((std::size_t)A.n + 1, 0), c32;
+      std::vector<double> v64, d64((std::size_t)A.n), l1((std::size_t)A.n);
+      c32.reserve(A.col.size());
+      v64.reserve(A.col.size());
       for (int i = 0; i < A.n; ++i) {
+        const int o = orig(i);
         double s = 0;
-        for (std::int64_t k = A.row[(std::size_t)i]; k < A.row[(std::size_t)i + 1]; ++k) s += std::abs(A.val[(std::size_t)k]);
+        for (std::int64_t k = A.row[(std::size_t)o]; k < A.row[(std::size_t)o + 1]; ++k) {
+          c32.push_back(relabel(A.col[(std::size_t)k]));
+          v64.push_back(A.val[(std::size_t)k]);
+          s += std::abs(A.val[(std::size_t)k]);
+        }
+        r32[(std::size_t)i + 1] = (int)c32.size();
+        d64[(std::size_t)i] = A.diag[(std::size_t)o];
         l1[(std::size_t)i] = s;
       }
+      Lv.row.upload(r32);
+      Lv.col.upload(c32);
+      Lv.val.upload(castVec(v64));
+      Lv.diag.upload(castVec(d64));
       Lv.l1.upload(castVec(l1));
       for (DBuf<TV>* v : {&Lv.b, &Lv.x, &Lv.r, &Lv.tmp, &Lv.corr}) v->alloc((std::size_t)A.n);
       Lv.x.upload(std::vector<TV>((std::size_t)A.n, TV(1)));
       Lv.lambda = l < H.levelLambda.size() ? H.levelLambda[l] : 1.0;
-      if (opt_.kind == "mcgs" && l + 1 < nl) Lv.color.upload(greedyColoring(A.n, A.row.data(), A.col.data()));
-      uploadTransfer(H.P[l], P_[l]);
     }
+    static const std::vector<int> none;
+    for (std::size_t l = 0; l < nl; ++l) uploadTransfer(H.P[l], P_[l], l > 0 ? inv[l - 1] : none, inv[l]);
     termN_ = H.terminalN;
     tinv_.upload(castVec(H.terminalInv));
     for (DBuf<TV>* v : {&fr_, &ftmp_, &fcorr_}) v->alloc((std::size_t)fineN);
@@ -190,18 +226,41 @@ class DevAmg {
     return o;
   }
 
-  static void uploadTransfer(const Transfer& P, Xfer& X) {
+  // Upload transfer P (fine rows -> coarse columns) and its transpose, with
+  // the fine side relabelled by invF and the coarse side by invC (original
+  // -> stored index; empty = unchanged).  Rows keep their entries' order;
+  // the transpose is formed in the original numbering (within a coarse row,
+  // fine rows ascend) and relabelled afterwards, so its sums keep the
+  // original order too.
+  static void uploadTransfer(const Transfer& P, Xfer& X, const std::vector<int>& invF,
+                             const std::vector<int>& invC) {
     X.nFine = P.nFine;
     X.nCoarse = P.nCoarse;
-    std::vector<int> pr(P.row.begin(), P.row.end());
-    X.prow.upload(pr);
-    X.pcol.upload(P.col);
-    X.pval.upload(castVec(P.val));
-    X.lanesP = lanesFor(P.nFine ? (double)P.col.size() / P.nFine : 1.0, P.nFine);
+    auto relF = [&](int i) { return invF.empty() ? i : invF[(std::size_t)i]; };
+    auto relC = [&](int c) { return invC.empty() ? c : invC[(std::size_t)c]; };
+    std::vector<int> origF((std::size_t)P.nFine), origC((std::size_t)P.nCoarse);
+    for (int i = 0; i < P.nFine; ++i) origF[(std::size_t)relF(i)] = i;
+    for (int c = 0; c < P.nCoarse; ++c) origC[(std::size_t)relC(c)] = c;
     X.aggregation = (std::size_t)P.nFine == P.col.size();
     for (int i = 0; X.aggregation && i < P.nFine; ++i)
       X.aggregation = P.row[(std::size_t)i] == i && P.row[(std::size_t)i + 1] == i + 1;
-    // Transpose for gather restriction; within a coarse row, fine rows ascend.
+    std::vector<int> pr((std::size_t)P.nFine + 1, 0), pc;
+    std::vector<double> pv;
+    pc.reserve(P.col.size());
+    pv.reserve(P.col.size());
+    for (int i = 0; i < P.nFine; ++i) {
+      const int o = origF[(std::size_t)i];
+      for (std::int64_t k = P.row[(std::size_t)o]; k < P.row[(std::size_t)o + 1]; ++k) {
+        pc.push_back(relC(P.col[(std::size_t)k]));
+        pv.push_back(P.val[(std::size_t)k]);
+      }
+      pr[(std::size_t)i + 1] = (int)pc.size();
+    }
+    X.prow.upload(pr);
+    X.pcol.upload(pc);
+    X.pval.upload(castVec(pv));
+    X.lanesP = lanesFor(P.nFine ? (double)P.col.size() / P.nFine : 1.0, P.nFine);
+    // Transpose for gather restriction, in the original numbering.
     std::vector<int> tr((std::size_t)P.nCoarse + 1, 0);
     for (int c : P.col) ++tr[(std::size_t)c + 1];
     for (int c = 0; c < P.nCoarse; ++c) tr[(std::size_t)c + 1] += tr[(std::size_t)c];
@@ -215,9 +274,23 @@ class DevAmg {
         tc[(std::size_t)z] = i;
         tv[(std::size_t)z] = P.val[(std::size_t)k];
       }
-    X.trow.upload(tr);
-    X.tcol.upload(tc);
-    X.tval.upload(castVec(tv));
+    // ... then its rows reordered by the coarse relabelling and its columns
+    // relabelled on the fine side.
+    std::vector<int> tr2((std::size_t)P.nCoarse + 1, 0), tc2;
+    std::vector<double> tv2;
+    tc2.reserve(tc.size());
+    tv2.reserve(tc.size());
+    for (int c = 0; c < P.nCoarse; ++c) {
+      const int o = origC[(std::size_t)c];
+      for (int k = tr[(std::size_t)o]; k < tr[(std::size_t)o + 1]; ++k) {
+        tc2.push_back(relF(tc[(std::size_t)k]));
+        tv2.push_back(tv[(std::size_t)k]);
+      }
+      tr2[(std::size_t)c + 1] = (int)tc2.size();
+    }
+    X.trow.upload(tr2);
+    X.tcol.upload(tc2);
+    X.tval.upload(castVec(tv2));
     X.lanesT = lanesFor(P.nCoarse ? (double)tc.size() / P.nCoarse : 1.0, P.nCoarse);
   }
 
@@ -281,7 +354,7 @@ class DevAmg {
     for (int q = 0; q < C.ncolors; ++q) {
       const int c = reverse ? C.ncolors - 1 - q : q;
       const int begin = C.off[(std::size_t)c], count = C.off[(std::size_t)c + 1] - begin;
-      mcgsColor<TM, TV>(s, lanesFor(meanRow, count), begin, count, C.rows.data(), row, col, val, diag, b, x,
+      mcgsColor<TM, TV>(s, lanesFor(meanRow, count), begin, count, C.rowList(), row, col, val, diag, b, x,
                         opt_.mcgsOmega);
     }
   }

@@ -338,7 +338,9 @@ __global__ void __launch_bounds__(256) k_row_l1(int n, const int* __restrict__ r
   if (lane == 0) l1[r] = s;
 }
 
-// One multicolour Gauss-Seidel colour: rows crow[begin..begin+count).
+// One multicolour Gauss-Seidel colour: rows crow[begin..begin+count), or
+// rows begin..begin+count when crow is null (operator numbered in colour
+// order).
 // x_i <- x_i + omega ((b_i - sum_{j!=i} a_ij x_j)/d_i - x_i)
 template <int LANES, class TM, class TV>
 __global__ void __launch_bounds__(256) k_mcgs_color(int begin, int count, const int* __restrict__ crow,
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(256) k_mcgs_color(int begin, int count, const 
   const int q = gid / LANES;
   const int lane = gid % LANES;
   if (q >= count) return;
-  const int i = crow[begin + q];
+  const int i = crow ? crow[begin + q] : begin + q;  // no list: the colour is a contiguous row range
   // the row's own b, diagonal and value do not depend on the row product:
   // their loads are issued with the row's, not after the reduction
   const TV bi = b[i], di = diag[i], old = x[i];
