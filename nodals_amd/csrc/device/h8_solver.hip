@@ -706,12 +706,12 @@ const char* H8Solver::momentumWork(const std::string& mode, double omega, int ou
 
 void H8Solver::momentumAuditLaunch(double* out) {
   requireDevice();
-  const int nbL = blocksFor((long long)nLong_ * kLongLanes), nbS = blocksFor((long long)nShort_ * kShortLanes);
+  const int nbL = blocksFor((long long)nLong_ * kNormLongLanes), nbS = blocksFor((long long)nShort_ * kNormShortLanes);
   const int grid = nbL + nbS;
   if (normPart_.size() < (std::size_t)grid * 6) normPart_.alloc((std::size_t)grid * 6);
-  hipLaunchKernelGGL((k_mom_norms<kLongLanes, kShortLanes>), grid, 256, 0, stream_, nLong_, longRows_.data(), nShort_,
-                     shortRows_.data(), nbL, mrow_.data(), mcol_.data(), aval_.data(), brhs_.data(), u_.data(),
-                     normPart_.data());
+  hipLaunchKernelGGL((k_mom_norms<kNormLongLanes, kNormShortLanes>), grid, 256, 0, stream_, nLong_, longRows_.data(),
+                     nShort_, shortRows_.data(), nbL, mrow_.data(), mcol_.data(), aval_.data(), brhs_.data(),
+                     u_.data(), normPart_.data());
   NODALS_LAUNCHED();
   hipLaunchKernelGGL(k_reduce_final, 6, 256, 0, stream_, grid, normPart_.data(), out);
   NODALS_LAUNCHED();
