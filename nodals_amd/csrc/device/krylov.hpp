@@ -76,9 +76,17 @@ struct KrylovWork {
 
 // FGMRES cycle start: r0 = sqrt(beta^2), g = (r0, 0, ...), the scaling of the
 // first basis vector (0 when r0 = 0 so nothing non-finite is produced).
-static __global__ void k_fgmres_start(double* __restrict__ st, double* __restrict__ host, int first) {
+// Cycle start.  Finishes the reduction of |r|^2 from its kReduceParts
+// block partials with k_reduce_final's tree (one partial per thread, wave
+// butterflies, the four waves in order), so the value is the one the
+// separate final kernel would store; launched with 256 threads.
+static __global__ void __launch_bounds__(256) k_fgmres_start(const double* __restrict__ part, double* __restrict__ st,
+                                                             double* __restrict__ host, int first) {
+  __shared__ double sh[4];
+  const double r2 = blockSum256(0.0 + part[threadIdx.x], sh);  // 0 + p: k_reduce_final's first add
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double b2 = st[fg::kBeta2];
+  st[fg::kBeta2] = r2;
+  const double b2 = r2;
   const double beta = sqrt(b2 > 0.0 ? b2 : 0.0);
   for (int i = 0; i < 33; ++i) st[fg::kG + i] = 0.0;
   st[fg::kG] = beta;
@@ -89,14 +97,33 @@ static __global__ void k_fgmres_start(double* __restrict__ st, double* __restric
 
 // First Gram-Schmidt update fused with the second pass's dot products:
 //   w -= V h1;  part[q][block] = V_q . w (q <= j), part[j+1][block] = w . w.
-// Reads the basis once for both.
+// Reads the basis once for both.  h1 (k values) is first reduced from the
+// first pass's kReduceParts block partials partIn by every block, with
+// k_reduce_final's tree (one partial per thread, wave butterflies, the four
+// waves in order), so each block holds exactly the values the separate
+// final kernel would store; block 0 stores them in h1.  partIn and part
+// are distinct (other blocks may still read partIn while one writes).
 static __global__ void __launch_bounds__(256) k_gs_update_dots(long long n, int k, const double* __restrict__ V,
-                                                                const double* __restrict__ h1, double* __restrict__ w,
+                                                                const double* __restrict__ partIn,
+                                                                double* __restrict__ h1, double* __restrict__ w,
                                                                 double* __restrict__ part) {
   __shared__ double c[32];
   __shared__ double sh[4][33];
-  if (threadIdx.x < 32) c[threadIdx.x] = (int)threadIdx.x < k ? h1[threadIdx.x] : 0.0;
-  __syncthreads();
+  {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < k; ++q) {
+      const double v = groupSum(0.0 + partIn[(long long)q * kReduceParts + threadIdx.x], 64);
+      if (lane == 0) sh[wv][q] = v;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 32) {
+      const double h = t < k ? sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t] : 0.0;
+      c[t] = h;
+      if (blockIdx.x == 0 && t < k) h1[t] = h;
+    }
+    __syncthreads();
+  }
   double sv[33];
 #pragma unroll
   for (int q = 0; q < 33; ++q) sv[q] = 0.0;
@@ -333,8 +360,8 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
   double* Z = W.Z.data();
   // cycle start from r (first cycle r = rhs, x = 0)
   auto start = [&](const double* r, bool first) {
-    R.dotDev(s, N, r, r, st + fg::kBeta2);
-    hipLaunchKernelGGL(k_fgmres_start, 1, 64, 0, s, st, hs, first ? 1 : 0);
+    R.dotPartials(s, N, r, r);
+    hipLaunchKernelGGL(k_fgmres_start, 1, 256, 0, s, R.partials(), st, hs, first ? 1 : 0);
     NODALS_LAUNCHED();
     hipLaunchKernelGGL(k_scale_dev, nb, 256, 0, s, N, st + fg::kScal, r, V);
     NODALS_LAUNCHED();
@@ -353,10 +380,11 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     A(s, zj, w);
     // classical Gram-Schmidt twice; pass 2's dots share a kernel with pass 1's
     // update, and pass 2's update shares one with the normalisation
-    R.multidotDev(s, N, j + 1, V, w, st + fg::kH1);
-    hipLaunchKernelGGL(k_gs_update_dots, kReduceParts, 256, 0, s, N, j + 1, V, st + fg::kH1, w, R.partials());
+    R.multidotPartials(s, N, j + 1, V, w);
+    hipLaunchKernelGGL(k_gs_update_dots, kReduceParts, 256, 0, s, N, j + 1, V, R.partials(), st + fg::kH1, w,
+                       R.partials2());
     NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_arnoldi_givens, 1, 256, 0, s, j, kReduceParts, R.partials(), st, hs);
+    hipLaunchKernelGGL(k_arnoldi_givens, 1, 256, 0, s, j, kReduceParts, R.partials2(), st, hs);
     NODALS_LAUNCHED();
     hipLaunchKernelGGL(k_gs_update_scale, nb, 256, 0, s, N, j + 1, V, st + fg::kH2, st + fg::kScal, w);
     NODALS_LAUNCHED();

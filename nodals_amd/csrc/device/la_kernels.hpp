@@ -568,6 +568,7 @@ class Reducer {
   void init(int maxVectors) {
     maxVec_ = maxVectors;
     part_.alloc((std::size_t)kReduceParts * maxVectors);
+    part2_.alloc((std::size_t)kReduceParts * maxVectors);
     if (host_) (void)hipHostFree(host_);
     // Coherent pinned memory: the final reduction kernel writes it directly
     // and the host reads it after the stream synchronisation.
@@ -625,11 +626,26 @@ class Reducer {
     NODALS_LAUNCHED();
   }
   double* partials() { return part_.data(); }
+  // a second partials area, for a kernel that reads partials() and writes
+  // its own in the same launch
+  double* partials2() { return part2_.data(); }
+  // Partial sums only (kReduceParts per output, output q at q*kReduceParts),
+  // for a consumer kernel that finishes the reduction itself.
+  template <class TA, class TB>
+  void dotPartials(hipStream_t s, long long n, const TA* x, const TB* y) {
+    hipLaunchKernelGGL((k_dot_part<TA, TB>), kReduceParts, 256, 0, s, n, x, y, part_.data());
+    NODALS_LAUNCHED();
+  }
+  void multidotPartials(hipStream_t s, long long n, int k, const double* V, const double* w) {
+    if (k > maxVec_ || k > 32) fail("multidot width exceeds reducer capacity", kErrInternal);
+    hipLaunchKernelGGL((k_multidot_part<32, false>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
+    NODALS_LAUNCHED();
+  }
   double* hostOut() { return host_; }
   const double* hostOut() const { return host_; }
 
  private:
-  DBuf<double> part_;
+  DBuf<double> part_, part2_;
   double* host_ = nullptr;
   int maxVec_ = 0;
 };
