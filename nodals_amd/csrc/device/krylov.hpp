@@ -155,22 +155,6 @@ static __global__ void __launch_bounds__(256) k_gs_update_dots(long long n, int 
   if (t == k) part[(long long)k * gridDim.x + blockIdx.x] = sh[0][32] + sh[1][32] + sh[2][32] + sh[3][32];
 }
 
-// Second Gram-Schmidt update fused with the normalisation:
-//   w = (w - V h2) * scal.
-static __global__ void __launch_bounds__(256) k_gs_update_scale(long long n, int k, const double* __restrict__ V,
-                                                                 const double* __restrict__ h2,
-                                                                 const double* __restrict__ scal,
-                                                                 double* __restrict__ w) {
-  __shared__ double c[33];
-  if (threadIdx.x < 32) c[threadIdx.x] = (int)threadIdx.x < k ? h2[threadIdx.x] : 0.0;
-  if (threadIdx.x == 32) c[32] = scal[0];
-  __syncthreads();
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double acc = 0.0;
-  for (int q = 0; q < k; ++q) acc += c[q] * V[(long long)q * n + i];
-  w[i] = (w[i] - acc) * c[32];
-}
 
 // Arnoldi step j after the two Gram-Schmidt passes: h = h1 + h2,
 // h_{j+1,j} = sqrt(max(|w|^2 - |h2|^2, 0)), previous rotations applied to
@@ -182,61 +166,85 @@ static __global__ void __launch_bounds__(256) k_gs_update_scale(long long n, int
 // and g_j in LDS; thread 0 then does the Givens step on LDS operands (a
 // serial chain of global-memory round trips otherwise), and the column is
 // written back by j+1 lanes at once.
-static __global__ void __launch_bounds__(256) k_arnoldi_givens(int j, int nparts, const double* __restrict__ part,
+// Second Gram-Schmidt pass, Arnoldi column and normalisation in one launch
+// over the basis rows (grid blocksFor(n)).  Every block reduces the second
+// pass's coefficients h2[0..j] and |w|^2 (h2[j+1]) from the first update's
+// kReduceParts block partials (wave wv takes coefficients wv, wv + 4, ...;
+// each lane sums its partials lane, lane + 64, ... in order, then a wave
+// butterfly), forms hn = sqrt(|w|^2 - |h2|^2) and applies
+//   w = (w - V h2) / hn.
+// Block 0 also completes the Hessenberg column h1 + h2, applies the earlier
+// Givens rotations and the new one, and writes the rotated column, the
+// rotation, the right-hand side and the residual estimate |g_{j+1}| to the
+// device state and the pinned host mirror.  Every block forms h2 and hn
+// with the same operations in the same order, so all agree bit for bit.
+static __global__ void __launch_bounds__(256) k_arnoldi_update(long long n, int j, int nparts,
+                                                                const double* __restrict__ part,
+                                                                const double* __restrict__ V, double* __restrict__ w,
                                                                 double* __restrict__ st, double* __restrict__ host) {
-  __shared__ double h2s[34], hcol[34], css[32], sns[32], gj;
+  __shared__ double h2s[34], hcol[34], css[32], sns[32], gj, scal;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int q = wv; q <= j + 1; q += 4) {
     double v = 0.0;
     for (int i = lane; i < nparts; i += 64) v += part[(long long)q * nparts + i];
     v = groupSum(v, 64);
-    if (lane == 0) {
-      st[fg::kH2 + q] = v;
-      h2s[q] = v;
+    if (lane == 0) h2s[q] = v;
+  }
+  const bool lead = blockIdx.x == 0;
+  if (lead) {
+    if (t <= j) hcol[t] = st[fg::kH1 + t];
+    if (t < j) {
+      css[t] = st[fg::kCs + t];
+      sns[t] = st[fg::kSn + t];
     }
+    if (t == 255) gj = st[fg::kG + j];
   }
-  if (t <= j) hcol[t] = st[fg::kH1 + t];
-  if (t < j) {
-    css[t] = st[fg::kCs + t];
-    sns[t] = st[fg::kSn + t];
-  }
-  if (t == 255) gj = st[fg::kG + j];
   __syncthreads();
   if (t == 0) {
     double hh = 0.0;
-    for (int i = 0; i <= j; ++i) {
-      hcol[i] = hcol[i] + h2s[i];
-      hh += h2s[i] * h2s[i];
-    }
+    for (int i = 0; i <= j; ++i) hh += h2s[i] * h2s[i];
     const double d2 = h2s[j + 1] - hh;
     const double hn = sqrt(d2 > 0.0 ? d2 : 0.0);
-    hcol[j + 1] = hn;
-    for (int i = 0; i < j; ++i) {
-      const double a = hcol[i], b = hcol[i + 1];
-      hcol[i] = css[i] * a + sns[i] * b;
-      hcol[i + 1] = -sns[i] * a + css[i] * b;
+    scal = hn > 0.0 ? 1.0 / hn : 0.0;
+    if (lead) {
+      for (int i = 0; i <= j + 1; ++i) st[fg::kH2 + i] = h2s[i];
+      for (int i = 0; i <= j; ++i) hcol[i] = hcol[i] + h2s[i];
+      hcol[j + 1] = hn;
+      for (int i = 0; i < j; ++i) {
+        const double a = hcol[i], b = hcol[i + 1];
+        hcol[i] = css[i] * a + sns[i] * b;
+        hcol[i + 1] = -sns[i] * a + css[i] * b;
+      }
+      const double a = hcol[j], b = hcol[j + 1];
+      const double d = hypot(a, b);
+      const bool bad = !(d > 0.0) || !isfinite(d) || !isfinite(hn);
+      const double c = bad ? 0.0 : a / d, sgn = bad ? 0.0 : b / d;
+      st[fg::kCs + j] = c;
+      st[fg::kSn + j] = sgn;
+      hcol[j] = d;
+      hcol[j + 1] = 0.0;
+      const double g1 = -sgn * gj, g0 = c * gj;
+      st[fg::kG + j + 1] = g1;
+      st[fg::kG + j] = g0;
+      st[fg::kScal] = scal;
+      host[fg::hG + j] = g0;
+      host[fg::hG + j + 1] = g1;
+      host[fg::hZero + j] = hn == 0.0 ? 1.0 : 0.0;
+      host[fg::hBad + j] = bad ? 1.0 : 0.0;
+      host[fg::hEst + j] = fabs(g1);
     }
-    const double a = hcol[j], b = hcol[j + 1];
-    const double d = hypot(a, b);
-    const bool bad = !(d > 0.0) || !isfinite(d) || !isfinite(hn);
-    const double c = bad ? 0.0 : a / d, sgn = bad ? 0.0 : b / d;
-    st[fg::kCs + j] = c;
-    st[fg::kSn + j] = sgn;
-    hcol[j] = d;
-    hcol[j + 1] = 0.0;
-    const double g1 = -sgn * gj, g0 = c * gj;
-    st[fg::kG + j + 1] = g1;
-    st[fg::kG + j] = g0;
-    st[fg::kScal] = hn > 0.0 ? 1.0 / hn : 0.0;
-    host[fg::hG + j] = g0;
-    host[fg::hG + j + 1] = g1;
-    host[fg::hZero + j] = hn == 0.0 ? 1.0 : 0.0;
-    host[fg::hBad + j] = bad ? 1.0 : 0.0;
-    host[fg::hEst + j] = fabs(g1);
   }
   __syncthreads();
-  if (t <= j + 1) st[fg::kH + t * 32 + j] = hcol[t];
-  if (t <= j) host[fg::hH + t * 32 + j] = hcol[t];
+  if (lead) {
+    if (t <= j + 1) st[fg::kH + t * 32 + j] = hcol[t];
+    if (t <= j) host[fg::hH + t * 32 + j] = hcol[t];
+  }
+  const long long i = (long long)blockIdx.x * blockDim.x + t;
+  if (i >= n) return;
+  const int k = j + 1;
+  double acc = 0.0;
+  for (int q = 0; q < k; ++q) acc += h2s[q] * V[(long long)q * n + i];
+  w[i] = (w[i] - acc) * scal;
 }
 
 inline KrylovResult pcg(hipStream_t s, Reducer& R, int n, const DevOp& A, const DevOp& M, const double* rhs, double rtol,
@@ -384,9 +392,7 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     hipLaunchKernelGGL(k_gs_update_dots, kReduceParts, 256, 0, s, N, j + 1, V, R.partials(), st + fg::kH1, w,
                        R.partials2());
     NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_arnoldi_givens, 1, 256, 0, s, j, kReduceParts, R.partials2(), st, hs);
-    NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_gs_update_scale, nb, 256, 0, s, N, j + 1, V, st + fg::kH2, st + fg::kScal, w);
+    hipLaunchKernelGGL(k_arnoldi_update, nb, 256, 0, s, N, j, kReduceParts, R.partials2(), V, w, st, hs);
     NODALS_LAUNCHED();
     NODALS_HIP(hipEventRecord(W.stepEv[(std::size_t)j], s));
   };
