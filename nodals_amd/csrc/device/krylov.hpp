@@ -76,23 +76,35 @@ struct KrylovWork {
 
 // FGMRES cycle start: r0 = sqrt(beta^2), g = (r0, 0, ...), the scaling of the
 // first basis vector (0 when r0 = 0 so nothing non-finite is produced).
-// Cycle start.  Finishes the reduction of |r|^2 from its kReduceParts
-// block partials with k_reduce_final's tree (one partial per thread, wave
-// butterflies, the four waves in order), so the value is the one the
-// separate final kernel would store; launched with 256 threads.
-static __global__ void __launch_bounds__(256) k_fgmres_start(const double* __restrict__ part, double* __restrict__ st,
-                                                             double* __restrict__ host, int first) {
+// Cycle start over the rows (grid blocksFor(n)): every block finishes the
+// reduction of |r|^2 from its kReduceParts block partials with
+// k_reduce_final's tree (one partial per thread, wave butterflies, the four
+// waves in order), so every block holds the value the separate final kernel
+// would store, and writes its rows of v0 = r / beta; block 0 also resets the
+// Givens right-hand side and writes beta to the device state and the pinned
+// host mirror.
+static __global__ void __launch_bounds__(256) k_fgmres_start(long long n, const double* __restrict__ part,
+                                                             const double* __restrict__ r, double* __restrict__ v0,
+                                                             double* __restrict__ st, double* __restrict__ host,
+                                                             int first) {
   __shared__ double sh[4];
+  __shared__ double scal;
   const double r2 = blockSum256(0.0 + part[threadIdx.x], sh);  // 0 + p: k_reduce_final's first add
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  st[fg::kBeta2] = r2;
-  const double b2 = r2;
-  const double beta = sqrt(b2 > 0.0 ? b2 : 0.0);
-  for (int i = 0; i < 33; ++i) st[fg::kG + i] = 0.0;
-  st[fg::kG] = beta;
-  st[fg::kScal] = beta > 0.0 ? 1.0 / beta : 0.0;
-  host[fg::hG] = beta;
-  if (first) host[fg::hR0] = beta;
+  if (threadIdx.x == 0) {
+    const double beta = sqrt(r2 > 0.0 ? r2 : 0.0);
+    scal = beta > 0.0 ? 1.0 / beta : 0.0;
+    if (blockIdx.x == 0) {
+      st[fg::kBeta2] = r2;
+      for (int i = 0; i < 33; ++i) st[fg::kG + i] = 0.0;
+      st[fg::kG] = beta;
+      st[fg::kScal] = scal;
+      host[fg::hG] = beta;
+      if (first) host[fg::hR0] = beta;
+    }
+  }
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v0[i] = scal * r[i];
 }
 
 // First Gram-Schmidt update fused with the second pass's dot products:
@@ -369,9 +381,7 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
   // cycle start from r (first cycle r = rhs, x = 0)
   auto start = [&](const double* r, bool first) {
     R.dotPartials(s, N, r, r);
-    hipLaunchKernelGGL(k_fgmres_start, 1, 256, 0, s, R.partials(), st, hs, first ? 1 : 0);
-    NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_scale_dev, nb, 256, 0, s, N, st + fg::kScal, r, V);
+    hipLaunchKernelGGL(k_fgmres_start, nb, 256, 0, s, N, R.partials(), r, V, st, hs, first ? 1 : 0);
     NODALS_LAUNCHED();
   };
   bool useZ0 = z0 != nullptr;

@@ -132,13 +132,6 @@ static __global__ void __launch_bounds__(256) k_multi_axpy_dev(long long n, int 
   for (int j = 0; j < k; ++j) s += c[j] * V[(long long)j * n + i];
   y[i] += sign * s;
 }
-// y = scal[0] * x with the scalar in device memory.
-static __global__ void k_scale_dev(long long n, const double* __restrict__ scal, const double* __restrict__ x,
-                                   double* __restrict__ y) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = scal[0] * x[i];
-}
-
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double blockSum256(double v, double* sh) {
   v = groupSum(v, 64);
