@@ -80,13 +80,14 @@ struct KrylovWork {
 // reduction of |r|^2 from its kReduceParts block partials with
 // k_reduce_final's tree (one partial per thread, wave butterflies, the four
 // waves in order), so every block holds the value the separate final kernel
-// would store, and writes its rows of v0 = r / beta; block 0 also resets the
-// Givens right-hand side and writes beta to the device state and the pinned
-// host mirror.
+// would store, and writes its rows of v0 = r / beta (and zeroes its rows of
+// the solution x when one is given: the first cycle from x = 0); block 0
+// also resets the Givens right-hand side and writes beta to the device
+// state and the pinned host mirror.
 static __global__ void __launch_bounds__(256) k_fgmres_start(long long n, const double* __restrict__ part,
                                                              const double* __restrict__ r, double* __restrict__ v0,
-                                                             double* __restrict__ st, double* __restrict__ host,
-                                                             int first) {
+                                                             double* __restrict__ x, double* __restrict__ st,
+                                                             double* __restrict__ host, int first) {
   __shared__ double sh[4];
   __shared__ double scal;
   const double r2 = blockSum256(0.0 + part[threadIdx.x], sh);  // 0 + p: k_reduce_final's first add
@@ -104,7 +105,10 @@ static __global__ void __launch_bounds__(256) k_fgmres_start(long long n, const 
   }
   __syncthreads();
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v0[i] = scal * r[i];
+  if (i < n) {
+    v0[i] = scal * r[i];
+    if (x) x[i] = 0.0;
+  }
 }
 
 // First Gram-Schmidt update fused with the second pass's dot products:
@@ -379,9 +383,11 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
   double* V = W.V.data();
   double* Z = W.Z.data();
   // cycle start from r (first cycle r = rhs, x = 0)
-  auto start = [&](const double* r, bool first) {
+  // zeroX: the cycle starts from x = 0 (the kernel clears x)
+  auto start = [&](const double* r, bool first, bool zeroX) {
     R.dotPartials(s, N, r, r);
-    hipLaunchKernelGGL(k_fgmres_start, nb, 256, 0, s, N, R.partials(), r, V, st, hs, first ? 1 : 0);
+    hipLaunchKernelGGL(k_fgmres_start, nb, 256, 0, s, N, R.partials(), r, V, zeroX ? W.x.data() : nullptr, st, hs,
+                       first ? 1 : 0);
     NODALS_LAUNCHED();
   };
   bool useZ0 = z0 != nullptr;
@@ -442,12 +448,11 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
       out.rel = bn > 0.0 ? rn / bn : 0.0;
       return out;
     }
-    start(W.r.data(), false);
+    start(W.r.data(), false, false);
     prevEst = rn;
     firstCycle = false;
   } else {
-    fill<double>(s, n, W.x.data(), 0.0);
-    start(rhs, true);
+    start(rhs, true, true);
     NODALS_HIP(hipEventRecord(W.stepEv[32], s));
   }
   while (true) {
@@ -537,7 +542,7 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
       return out;
     }
     if (its >= maxit) return out;
-    start(W.r.data(), false);
+    start(W.r.data(), false, false);
     prevEst = beta;
   }
 }
