@@ -290,7 +290,8 @@ static __global__ void k_fine_weights(int nc, const int* __restrict__ cref, cons
   for (int d = 0; d < 3; ++d) W[3ll * t + d] = r * bcoef[3ll * t + d];
 }
 
-// Fine P0 Schur numeric refresh, one nonzero per lane:
+// Fine P0 Schur numeric refresh (L lanes per row, the grid looping over the
+// rows), one nonzero per lane:
 //   val[k] = sum_{shared free DOF g} rAU_g * (B[c][a] . B[c2][b])
 //          = sum (W[c][a] . B[c2][b])
 // (h8_fine_csr_refresh_warp_kernel, h8_precomputed_b.cuh:37-91, restated as
@@ -305,37 +306,36 @@ __global__ void __launch_bounds__(256) k_fine_refresh(
     const double* __restrict__ W, int pin, double* __restrict__ val, float* __restrict__ valf,
     bf16* __restrict__ valb, double* __restrict__ diag, float* __restrict__ diagf, double* __restrict__ l1d,
     float* __restrict__ l1f) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int c = gid / L;
-  const int lane = gid % L;
-  if (c >= nc) return;
-  const int rb = frow[c], re = frow[c + 1], dp = fdiagPos[c];
-  const double* Wc = W + (long long)c * 24;
-  double sabs = 0.0;
-  for (int k = rb + lane; k < re; k += L) {
-    const int c2 = fcol[k];
-    const double* Bn = bcoef + (long long)c2 * 24;
-    double v = 0.0;
-    for (int t = nzOff[k]; t < nzOff[k + 1]; ++t) {
-      const unsigned q = nzTerm[t];
-      const int a = (int)((q >> 3) & 7u), b = (int)(q & 7u);
-      v += Wc[3 * a] * Bn[3 * b] + Wc[3 * a + 1] * Bn[3 * b + 1] + Wc[3 * a + 2] * Bn[3 * b + 2];
+  const int lane = threadIdx.x % L;
+  for (int c = (blockIdx.x * 256 + threadIdx.x) / L; c < nc; c += gridDim.x * (256 / L)) {
+    const int rb = frow[c], re = frow[c + 1], dp = fdiagPos[c];
+    const double* Wc = W + (long long)c * 24;
+    double sabs = 0.0;
+    for (int k = rb + lane; k < re; k += L) {
+      const int c2 = fcol[k];
+      const double* Bn = bcoef + (long long)c2 * 24;
+      double v = 0.0;
+      for (int t = nzOff[k]; t < nzOff[k + 1]; ++t) {
+        const unsigned q = nzTerm[t];
+        const int a = (int)((q >> 3) & 7u), b = (int)(q & 7u);
+        v += Wc[3 * a] * Bn[3 * b] + Wc[3 * a + 1] * Bn[3 * b + 1] + Wc[3 * a + 2] * Bn[3 * b + 2];
+      }
+      if ((c == pin || c2 == pin) && c != c2) v = 0.0;
+      val[k] = v;
+      if (F32) valf[k] = (float)v;
+      if (B16) valb[k] = floatToBf16((float)v);
+      if (L1) sabs += fabs(v);
+      if (k == dp) {
+        diag[c] = v;
+        if (F32 || B16) diagf[c] = (float)v;
+      }
     }
-    if ((c == pin || c2 == pin) && c != c2) v = 0.0;
-    val[k] = v;
-    if (F32) valf[k] = (float)v;
-    if (B16) valb[k] = floatToBf16((float)v);
-    if (L1) sabs += fabs(v);
-    if (k == dp) {
-      diag[c] = v;
-      if (F32 || B16) diagf[c] = (float)v;
-    }
-  }
-  if (L1) {
-    sabs = groupSum(sabs, L);
-    if (lane == 0) {
-      l1d[c] = sabs;
-      if (F32 || B16) l1f[c] = (float)sabs;
+    if (L1) {
+      sabs = groupSum(sabs, L);
+      if (lane == 0) {
+        l1d[c] = sabs;
+        if (F32 || B16) l1f[c] = (float)sabs;
+      }
     }
   }
 }

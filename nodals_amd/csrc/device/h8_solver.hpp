@@ -211,6 +211,7 @@ class H8Solver {
   AmgHierarchy H_;
   hipStream_t stream_ = nullptr;
   hipStream_t stream2_ = nullptr;  // the refresh, overlapped with the momentum solve
+  int cuCount_ = 1;
   hipEvent_t evAsm_ = nullptr, evRef_ = nullptr;
   // pinned: [0] the continuity residual's square sum, [1..6] the momentum
   // audit's sums, read by the SIMPLE loop after its pressure solve
