@@ -760,12 +760,16 @@ void H8Solver::refreshOn(hipStream_t s) {
   hipLaunchKernelGGL(k_fine_weights, blocksFor((long long)nc_ * 8), 256, 0, s, nc_, cref_.data(), bcoef_.data(),
                      rau_.data(), fineW_.data());
   NODALS_LAUNCHED();
-  // At most four 256-thread blocks per CU (half of a CU's wave slots), each
-  // looping over rows: the refresh runs beside the momentum sweep, whose
-  // colour launches would otherwise wait for a full grid to drain.  On the
-  // 40k-cell pipe: 1024 blocks 0.586 s to converge, 768 0.589, 1280 0.593,
-  // 1536 0.601, the full 5040 0.605 (512 finishes after the sweep: 0.605).
-  const int grid = (int)std::min<long long>(blocksFor((long long)nc_ * refreshLanes_), 4LL * cuCount_);
+  // A short refresh (at most five rounds of four 256-thread blocks per CU)
+  // runs on four blocks per CU (half of a CU's wave slots), each looping
+  // over rows: it runs beside the momentum sweep, whose colour launches
+  // would otherwise wait for the full grid to drain.  On the 40k-cell pipe:
+  // 1024 blocks 0.586 s to converge, 768 0.589, 1280 0.593, 1536 0.601, the
+  // full 5040 0.605 (512 finishes after the sweep: 0.605).  A longer one
+  // outlasts the sweep when stretched and keeps its full grid: capped, the
+  // 768k-cell fixed-10 run took 7.40 against 6.82 ms per SIMPLE iteration.
+  const long long full = blocksFor((long long)nc_ * refreshLanes_), cap = 4LL * cuCount_;
+  const int grid = (int)(full <= 5 * cap ? std::min(full, cap) : full);
   const bool f32 = opt_.amgPrecision == "fp32", b16 = opt_.amgPrecision == "bf16";
   const bool l1 = opt_.amgSmoother == "l1jacobi";
 #define NODALS_REFRESH(L, F, B, LL)                                                                                  \
