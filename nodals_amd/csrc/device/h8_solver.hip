@@ -324,6 +324,8 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   cuCount_ = std::max(cuCount_, 1);
   NODALS_HIP(hipEventCreateWithFlags(&evAsm_, hipEventDisableTiming));
   NODALS_HIP(hipEventCreateWithFlags(&evRef_, hipEventDisableTiming));
+  NODALS_HIP(hipEventCreateWithFlags(&evSweep_, hipEventDisableTiming));
+  NODALS_HIP(hipEventCreateWithFlags(&evConv_, hipEventDisableTiming));
   NODALS_HIP(hipHostMalloc((void**)&loopHost_, sizeof(double) * 8, hipHostMallocCoherent));
   red_.init(32);
   upload();
@@ -345,6 +347,8 @@ H8Solver::~H8Solver() {
   if (loopHost_) (void)hipHostFree(loopHost_);
   if (evAsm_) (void)hipEventDestroy(evAsm_);
   if (evRef_) (void)hipEventDestroy(evRef_);
+  if (evSweep_) (void)hipEventDestroy(evSweep_);
+  if (evConv_) (void)hipEventDestroy(evConv_);
   if (stream2_) (void)hipStreamDestroy(stream2_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -619,13 +623,22 @@ void H8Solver::setupAmg() {
 }
 
 // ------------------------------------------------------------------ stages
-void H8Solver::assemble(double alphaU, double rauScale) {
-  requireDevice();
-  require(alphaU > 0.0 && alphaU <= 1.0, "alpha-u must be in (0,1]");
-  const int waves = std::min(nc_, 8192);
-  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, stream_, nc_, cref_.data(), cinvJ_.data(),
+// Cell convection blocks of the current velocity into cbuf_ (one wavefront
+// per cell, `waves` wavefronts looping over the cells).
+void H8Solver::convLocal(hipStream_t s, int waves) {
+  waves = std::max(1, std::min(nc_, waves));
+  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, s, nc_, cref_.data(), cinvJ_.data(),
                      cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), cbuf_.data());
   NODALS_LAUNCHED();
+}
+
+void H8Solver::assemble(double alphaU, double rauScale, bool convReady) {
+  requireDevice();
+  require(alphaU > 0.0 && alphaU <= 1.0, "alpha-u must be in (0,1]");
+  if (convReady)  // formed on the second stream from this velocity (the SIMPLE loop)
+    NODALS_HIP(hipStreamWaitEvent(stream_, evConv_, 0));
+  else
+    convLocal(stream_, 8192);
   if (turbCouple_) eddyDiffusion();
   MomRowsArgs a;
   a.mrow = mrow_.data();
@@ -1114,6 +1127,10 @@ H8RunResult H8Solver::run() {
     R.pupdateMs += ms[4];
   };
   bool pending = false;  // the other buffer holds an unread iteration
+  bool convReady = false;  // the next assembly's convection blocks are queued on the second stream
+  // nothing of an earlier run (one that ended in an error) may still write
+  // the buffers the second stream shares with this one
+  NODALS_HIP(hipStreamSynchronize(stream2_));
   bool converged = false, pressureAll = true, finiteAll = true;
   double cont0 = -1.0, contRel = 1.0;
   long long sumP = 0;
@@ -1128,7 +1145,8 @@ H8RunResult H8Solver::run() {
     for (int it = 1; it <= o.maxOuter; ++it) {
       hipEvent_t* ev = evs[it & 1];
       NODALS_HIP(hipEventRecord(ev[0], stream_));
-      assemble(o.alphaU, o.rauScale);
+      assemble(o.alphaU, o.rauScale, convReady);
+      convReady = false;
       NODALS_HIP(hipEventRecord(ev[1], stream_));
       // the fine Schur refresh needs rAU only: it runs on the second stream
       // beside the momentum solve and the continuity residual
@@ -1154,6 +1172,20 @@ H8RunResult H8Solver::run() {
       }
       const char* dir = momentumWork(o.momentumWork, o.momOmega, it, o.momSweeps);
       NODALS_HIP(hipEventRecord(ev[2], stream_));
+      // The velocity is final for the next assembly once the sweep is done
+      // (the pressure update changes p only): its convection blocks are
+      // formed on the second stream beside the continuity residual and the
+      // pressure solve, on four blocks per CU so that those keep wave slots.
+      // This iteration's row gather has read cbuf_ before the sweep.  In the
+      // coupled SST mode the eddy block needs the k, omega of the step at
+      // the iteration's end, so the blocks are formed in the assembly there.
+      if (!turbCouple_ && it < o.maxOuter) {
+        NODALS_HIP(hipEventRecord(evSweep_, stream_));
+        NODALS_HIP(hipStreamWaitEvent(stream2_, evSweep_, 0));
+        convLocal(stream2_, 16 * cuCount_);
+        NODALS_HIP(hipEventRecord(evConv_, stream2_));
+        convReady = true;
+      }
       continuityLaunch(loopHost_);
       NODALS_HIP(hipEventRecord(ev[3], stream_));
       NODALS_HIP(hipStreamWaitEvent(stream_, evRef_, 0));
@@ -1248,6 +1280,7 @@ H8RunResult H8Solver::run() {
     throw;
   }
   sync();
+  NODALS_HIP(hipStreamSynchronize(stream2_));  // the last iteration's queued convection blocks
   if (pending) harvest(evs[finalIt & 1]);
   for (auto& b : evs)
     for (auto& e : b) (void)hipEventDestroy(e);

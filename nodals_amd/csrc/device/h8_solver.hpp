@@ -97,7 +97,9 @@ class H8Solver {
   const H8Options& options() const { return opt_; }
   hipStream_t stream() const { return stream_; }
 
-  void assemble(double alphaU, double rauScale);
+  // convReady: cbuf_ already holds (or has queued on the second stream,
+  // ending with evConv_) the convection blocks of the current velocity
+  void assemble(double alphaU, double rauScale, bool convReady = false);
   const char* momentumWork(const std::string& mode, double omega, int outer, int sweeps = 1);
   std::array<double, 3> momentumAudit();
   // The audit and the continuity norm in two halves: the launch (sums land
@@ -212,7 +214,8 @@ class H8Solver {
   hipStream_t stream_ = nullptr;
   hipStream_t stream2_ = nullptr;  // the refresh, overlapped with the momentum solve
   int cuCount_ = 1;
-  hipEvent_t evAsm_ = nullptr, evRef_ = nullptr;
+  hipEvent_t evAsm_ = nullptr, evRef_ = nullptr, evSweep_ = nullptr, evConv_ = nullptr;
+  void convLocal(hipStream_t s, int waves);
   // pinned: [0] the continuity residual's square sum, [1..6] the momentum
   // audit's sums, read by the SIMPLE loop after its pressure solve
   double* loopHost_ = nullptr;
