@@ -228,6 +228,37 @@ def test_vcycle_is_symmetric_positive(small_case):
     assert np.abs(H["Tinv"] @ Ac - np.eye(Ac.shape[0])).max() < 1e-9
 
 
+def test_momentum_sweeps_solve_the_relaxed_system(small_case):
+    """mom_sweeps repeats the sweep on one iteration's system: at omega 1,
+    60 sweeps from rest solve the first iteration's relaxed momentum system
+    (the row-L1 relaxation at alpha_u 0.5 makes it diagonally dominant by a
+    factor of two).  A loop that re-assembled between sweeps, or swept
+    another system, would not reach this solution."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    case = small_case
+    _, T = O.tensors()
+    U0 = np.zeros((3, case.nv))
+    aval, conv, _ = O.assemble_central(case, U0, T)
+    delta, diag, _ = O.finalize_relax(case, aval, 0.5)
+    b = O.momentum_rhs(case, conv, O.bt_apply(case, np.zeros(case.nc)), delta, U0)
+    A = O.mom_matrix(case, aval)
+    M = (A - sp.diags(A.diagonal()) + sp.diags(diag)).tocsc()
+    seen = {}
+    O.simple(case, p_solver="pcg", smoother=O.Smoother(kind="jacobi"), max_outer=1, mom_sweeps=60, mom_omega=1.0,
+             callback=lambda it, U, p: seen.setdefault("U", U.copy()))
+    for d in range(3):
+        want = spla.spsolve(M, b[d])
+        assert np.linalg.norm(seen["U"][d] - want) <= 1e-10 * np.linalg.norm(want)
+    # one sweep is far from it
+    seen.clear()
+    O.simple(case, p_solver="pcg", smoother=O.Smoother(kind="jacobi"), max_outer=1, mom_omega=1.0,
+             callback=lambda it, U, p: seen.setdefault("U", U.copy()))
+    want = spla.spsolve(M, b[2])
+    assert np.linalg.norm(seen["U"][2] - want) > 1e-3 * np.linalg.norm(want)
+
+
 def test_simple_reaches_poiseuille(small_case):
     """The oracle's SIMPLE loop converges on the pipe and its pressure drop is
     the Hagen-Poiseuille 32 nu U L / D^2 within the discretisation error."""
