@@ -597,27 +597,6 @@ class Reducer {
     NODALS_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < outs; ++j) out[j] = host_[j];
   }
-  // Device-resident forms: results land in device memory (out[0..]) and
-  // the stream is not synchronised.
-  template <class TA, class TB>
-  void dotDev(hipStream_t s, long long n, const TA* x, const TB* y, double* out) {
-    hipLaunchKernelGGL((k_dot_part<TA, TB>), kReduceParts, 256, 0, s, n, x, y, part_.data());
-    NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_final, 1, 256, 0, s, kReduceParts, part_.data(), out);
-    NODALS_LAUNCHED();
-  }
-  void multidotDev(hipStream_t s, long long n, int k, const double* V, const double* w, double* out,
-                   bool withNorm = false) {
-    const int outs = k + (withNorm ? 1 : 0);
-    if (outs > maxVec_ || k > 32) fail("multidot width exceeds reducer capacity", kErrInternal);
-    if (withNorm)
-      hipLaunchKernelGGL((k_multidot_part<32, true>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
-    else
-      hipLaunchKernelGGL((k_multidot_part<32, false>), kReduceParts, 256, 0, s, n, k, V, w, part_.data());
-    NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_reduce_final, outs, 256, 0, s, kReduceParts, part_.data(), out);
-    NODALS_LAUNCHED();
-  }
   double* partials() { return part_.data(); }
   // a second partials area, for a kernel that reads partials() and writes
   // its own in the same launch
