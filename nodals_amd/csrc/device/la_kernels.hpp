@@ -57,6 +57,18 @@ __device__ __forceinline__ TS rowDot(int k0, int e, const int* __restrict__ col,
   return s;
 }
 
+// Row-block index for XCD locality.  Blocks go to the eight XCDs round
+// robin, and each XCD has its own L2; with the grid padded to a multiple
+// of eight, XCD x takes virtual blocks [x*g/8, (x+1)*g/8), one contiguous
+// eighth of the rows, so a vector written by one such kernel is read back
+// by the next from the same L2.  A bijection on [0, gridDim.x): blocks past
+// the rows exit in the caller's bounds check, whatever the hardware's
+// placement.
+__device__ __forceinline__ int xcdRowBlock() {
+  return (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+}
+inline int xcdGrid(int grid) { return (grid + 7) & ~7; }
+
 // ---------------------------------------------------------------- BLAS-1
 template <class T>
 __global__ void k_fill(long long n, T* x, T a) {
@@ -225,7 +237,7 @@ template <int LANES>
 __global__ void __launch_bounds__(256) k_schur_spmv(int n, const int* __restrict__ row, const int* __restrict__ col,
                                                     const double* __restrict__ val, const double* __restrict__ x,
                                                     double* __restrict__ y) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int gid = xcdRowBlock() * 256 + threadIdx.x;
   const int r = gid / LANES;
   const int lane = gid % LANES;
   if (r >= n) return;
@@ -248,7 +260,7 @@ __global__ void __launch_bounds__(256) k_jacobi_pre_resid(int n, const int* __re
                                                           const int* __restrict__ col, const TM* __restrict__ val,
                                                           const TB* __restrict__ b, const TV* __restrict__ d,
                                                           double omega, TV* __restrict__ t, TV* __restrict__ r) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int gid = xcdRowBlock() * 256 + threadIdx.x;
   const int i = gid / LANES;
   const int lane = gid % LANES;
   if (i >= n) return;
@@ -271,7 +283,7 @@ __global__ void __launch_bounds__(256) k_resid_jacobi_post(int n, const int* __r
                                                            const TB* __restrict__ b, const TV* __restrict__ d,
                                                            double omega, const TV* __restrict__ t,
                                                            TZ* __restrict__ z) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int gid = xcdRowBlock() * 256 + threadIdx.x;
   const int i = gid / LANES;
   const int lane = gid % LANES;
   if (i >= n) return;
@@ -422,7 +434,7 @@ void spmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const
 inline void schurSpmv(hipStream_t s, int lanes, int n, const int* row, const int* col, const double* val,
                       const double* x, double* y, hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
   if (n <= 0) return;
-  const int grid = blocksFor((long long)n * lanes);
+  const int grid = xcdGrid(blocksFor((long long)n * lanes));  // the kernel maps blocks per XCD
 #define NODALS_SCHUR_LAUNCH(LN)                                                                                  \
   if (start)                                                                                                    \
     hipExtLaunchKernelGGL((k_schur_spmv<LN>), dim3(grid), dim3(256), 0, s, start, stop, 0, n, row, col, val, x, y); \
@@ -443,7 +455,7 @@ template <class TM, class TV, class TB = TV>
 void jacobiPreResid(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TB* b,
                     const TV* d, double omega, TV* t, TV* r) {
   if (n <= 0) return;
-  const int grid = blocksFor((long long)n * lanes);
+  const int grid = xcdGrid(blocksFor((long long)n * lanes));  // the kernel maps blocks per XCD
 #define NODALS_JPR(LN) \
   hipLaunchKernelGGL((k_jacobi_pre_resid<LN, TM, TV, TB>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, r)
   switch (lanes) {
@@ -481,7 +493,7 @@ template <class TM, class TV, class TB = TV, class TZ = TV>
 void residJacobiPost(hipStream_t s, int lanes, int n, const int* row, const int* col, const TM* val, const TB* b,
                      const TV* d, double omega, const TV* t, TZ* z) {
   if (n <= 0) return;
-  const int grid = blocksFor((long long)n * lanes);
+  const int grid = xcdGrid(blocksFor((long long)n * lanes));  // the kernel maps blocks per XCD
 #define NODALS_RJP(LN) \
   hipLaunchKernelGGL((k_resid_jacobi_post<LN, TM, TV, TB, TZ>), grid, 256, 0, s, n, row, col, val, b, d, omega, t, z)
   switch (lanes) {
