@@ -172,6 +172,41 @@ static __global__ void __launch_bounds__(256) k_gs_update_dots(long long n, int 
 }
 
 
+// Conditional end of an FGMRES cycle after step k-1, queued by the host when
+// it predicts that step converges, so the update runs while the host is
+// still waiting on the step.  If the step's residual estimate |g_k| meets
+// the target, or its Arnoldi vector vanished (scaling 0), every block solves
+// the rotated triangular system H y = g (k x k) by back substitution in the
+// host's order, contraction off, so y equals the host's, and applies
+// x += Z y with k_multi_axpy's sum.  Otherwise it does nothing.  It reads
+// the same values as the host mirror, so the host's check of the step comes
+// to the same decision and skips its own update exactly when this one ran.
+static __global__ void __launch_bounds__(256) k_fgmres_finish(long long n, int k, double target,
+                                                               const double* __restrict__ Z,
+                                                               const double* __restrict__ st, double* __restrict__ x) {
+  __shared__ double hl[32 * 32], gl[32], yl[32];
+  const int t = threadIdx.x;
+  // every thread reads the same two values, so the exit is uniform
+  if (!(fabs(st[fg::kG + k]) <= target || st[fg::kScal] == 0.0)) return;
+  for (int e = t; e < k * 32; e += 256) hl[e] = st[fg::kH + e];
+  if (t < k) gl[t] = st[fg::kG + t];
+  __syncthreads();
+  if (t == 0) {
+#pragma clang fp contract(off)
+    for (int i = k - 1; i >= 0; --i) {
+      double v = gl[i];
+      for (int l = i + 1; l < k; ++l) v -= hl[i * 32 + l] * yl[l];
+      yl[i] = v / hl[i * 32 + i];
+    }
+  }
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * blockDim.x + t;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = 0; j < k; ++j) acc += yl[j] * Z[(long long)j * n + i];
+  x[i] += acc;
+}
+
 // Arnoldi step j after the two Gram-Schmidt passes: h = h1 + h2,
 // h_{j+1,j} = sqrt(max(|w|^2 - |h2|^2, 0)), previous rotations applied to
 // column j, the new rotation, g updated; the residual estimate and flags go
@@ -458,6 +493,7 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
   while (true) {
     const int limit = std::min(m, maxit - its);
     int queued = 0, checked = -1, k = 0;
+    int finishQueued = -1;  // step after which k_fgmres_finish was queued
     bool done = false;
     auto check = [&](int q) {
       NODALS_HIP(hipEventSynchronize(W.stepEv[(std::size_t)q]));
@@ -477,7 +513,15 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     };
     while (!done) {
       if (checked + 1 <= queued - 1 && predictsConverged()) {
-        check(checked + 1);
+        const int q = checked + 1;
+        if (q == queued - 1) {
+          // nothing is queued after step q: let the device apply the update
+          // if the step converges while the host waits for it
+          hipLaunchKernelGGL(k_fgmres_finish, nb, 256, 0, s, N, q + 1, target, Z, st, W.x.data());
+          NODALS_LAUNCHED();
+          finishQueued = q;
+        }
+        check(q);
         continue;
       }
       if (queued < limit) {
@@ -509,16 +553,19 @@ inline KrylovResult fgmres(hipStream_t s, Reducer& R, int n, const DevOp& A, con
     }
     its += k;
     firstCycle = false;
-    // x += Z y, H y = g (upper triangular k x k, rotated H in host memory)
-    for (int i = k - 1; i >= 0; --i) {
-      double t = hs[fg::hG + i];
-      for (int l = i + 1; l < k; ++l) t -= hs[fg::hH + i * 32 + l] * y[(std::size_t)l];
-      y[(std::size_t)i] = t / hs[fg::hH + i * 32 + i];
+    // x += Z y, H y = g (upper triangular k x k, rotated H in host memory),
+    // unless k_fgmres_finish has applied it after the last checked step
+    if (!(done && finishQueued == checked)) {
+      for (int i = k - 1; i >= 0; --i) {
+        double t = hs[fg::hG + i];
+        for (int l = i + 1; l < k; ++l) t -= hs[fg::hH + i * 32 + l] * y[(std::size_t)l];
+        y[(std::size_t)i] = t / hs[fg::hH + i * 32 + i];
+      }
+      Coef32 c{};
+      for (int i = 0; i < k; ++i) c.c[i] = y[(std::size_t)i];
+      hipLaunchKernelGGL(k_multi_axpy, nb, 256, 0, s, N, k, Z, c, W.x.data(), 1.0);
+      NODALS_LAUNCHED();
     }
-    Coef32 c{};
-    for (int i = 0; i < k; ++i) c.c[i] = y[(std::size_t)i];
-    hipLaunchKernelGGL(k_multi_axpy, nb, 256, 0, s, N, k, Z, c, W.x.data(), 1.0);
-    NODALS_LAUNCHED();
     // A cycle whose residual estimate met the target ends the solve: the
     // estimate is |b - A x| up to FP64 rounding (w = A z_j and its
     // orthogonalisation are FP64 whatever the preconditioner's precision),
