@@ -241,19 +241,46 @@ __device__ __forceinline__ void mom_norm_row(int i, int lane, const int* __restr
                                              const int* __restrict__ mcol, const double* __restrict__ aval,
                                              const double* __restrict__ brhs, const double* __restrict__ u,
                                              double* acc) {
+  // U trips' loads are issued before any accumulation, in the one-at-a-time
+  // order (bit-identical sums); positions past the row end load the row's
+  // own velocity and are not accumulated.  The right-hand side loads first.
+  constexpr int U = L <= 4 ? 4 : 2;
+  double b0 = 0, b1 = 0, b2 = 0;
+  if (lane == 0) {
+    b0 = brhs[3ll * i];
+    b1 = brhs[3ll * i + 1];
+    b2 = brhs[3ll * i + 2];
+  }
+  const int ke = mrow[i + 1];
   double s0 = 0, s1 = 0, s2 = 0;
-  for (int k = mrow[i] + lane; k < mrow[i + 1]; k += L) {
-    const int j = mcol[k];
-    const double a = aval[k];
-    s0 += a * u[3ll * j + 0];
-    s1 += a * u[3ll * j + 1];
-    s2 += a * u[3ll * j + 2];
+  for (int k0 = mrow[i] + lane; k0 < ke; k0 += U * L) {
+    int jq[U];
+    double aq[U], x0[U], x1[U], x2[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int k = k0 + q * L;
+      jq[q] = k < ke ? mcol[k] : i;
+      aq[q] = k < ke ? aval[k] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      x0[q] = u[3ll * jq[q] + 0];
+      x1[q] = u[3ll * jq[q] + 1];
+      x2[q] = u[3ll * jq[q] + 2];
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      if (k0 + q * L < ke) {
+        s0 += aq[q] * x0[q];
+        s1 += aq[q] * x1[q];
+        s2 += aq[q] * x2[q];
+      }
+    }
   }
   s0 = groupSum(s0, L);
   s1 = groupSum(s1, L);
   s2 = groupSum(s2, L);
   if (lane == 0) {
-    const double b0 = brhs[3ll * i], b1 = brhs[3ll * i + 1], b2 = brhs[3ll * i + 2];
     acc[0] += b0 * b0;
     acc[1] += b1 * b1;
     acc[2] += b2 * b2;
