@@ -328,11 +328,23 @@ static __global__ void __launch_bounds__(256) k_continuity(int nc, const int* __
   for (int c = blockIdx.x * 256 + threadIdx.x; c < nc; c += gridDim.x * 256) {
     double s = fixedDiv[c];
     const double* B = bcoef + (long long)c * 24;
+    // all eight gathers are issued before the sum (a fixed DOF reads DOF 0,
+    // in bounds, and is not accumulated); the sum keeps the basis order
+    int gq[8];
+    double xq[24];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) gq[a] = cref[(long long)c * 8 + a];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-      const int g = cref[(long long)c * 8 + a];
-      if (g < 0) continue;
-      s += B[3 * a] * u[3ll * g] + B[3 * a + 1] * u[3ll * g + 1] + B[3 * a + 2] * u[3ll * g + 2];
+      const long long g = gq[a] < 0 ? 0 : gq[a];
+      xq[3 * a] = u[3 * g];
+      xq[3 * a + 1] = u[3 * g + 1];
+      xq[3 * a + 2] = u[3 * g + 2];
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const double t = B[3 * a] * xq[3 * a] + B[3 * a + 1] * xq[3 * a + 1] + B[3 * a + 2] * xq[3 * a + 2];
+      if (gq[a] >= 0) s += t;
     }
     negCont[c] = c == pin ? 0.0 : -s;
     acc += s * s;
