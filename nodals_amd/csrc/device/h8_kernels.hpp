@@ -105,7 +105,21 @@ __device__ __forceinline__ void mom_row(const MomRowsArgs& A, int i, int lane) {
   double sabs = 0.0, vdiag = 0.0;
   for (int k = rb + lane; k < re; k += L) {
     double v = A.mdiff[k];
-    for (int t = A.termOff[k]; t < A.termOff[k + 1]; ++t) v += A.cbuf[A.termIdx[t]];
+    // the entry's convection terms four at a time: the index loads, then the
+    // block loads, then the sums in term order (indices clamped in bounds,
+    // terms past the end not accumulated)
+    const int te = A.termOff[k + 1];
+    for (int t0 = A.termOff[k]; t0 < te; t0 += 4) {
+      int ix[4];
+      double cv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ix[q] = A.termIdx[min(t0 + q, te - 1)];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cv[q] = A.cbuf[ix[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (t0 + q < te) v += cv[q];
+    }
     A.aval[k] = v;
     sabs += fabs(v);
     if (k == dp) vdiag = v;
