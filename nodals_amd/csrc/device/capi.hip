@@ -136,6 +136,7 @@ int nodals_h8_momentum_work(nodals_h8* h, const char* mode, double omega, int32_
 
 int nodals_h8_momentum_audit(nodals_h8* h, double rel[3]) {
   return guarded([&]() {
+    nodals::require(rel != nullptr, "null audit output");
     const auto q = solver(h).momentumAudit();
     for (int d = 0; d < 3; ++d) rel[d] = q[(std::size_t)d];
   });

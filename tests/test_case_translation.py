@@ -3,7 +3,6 @@
 restated against nodals_amd.case, plus launcher behaviour with a stand-in
 executable (the technique of turbulence/sst_gpu_v2/tests/test_software.py:53-85).
 """
-import configparser
 import json
 import os
 import stat
@@ -35,7 +34,7 @@ def case_file(request):
 
 
 def test_annotated_case_selects_cf_pmis_family(case_file):
-    cp = m._read_case(case_file)
+    cp = m.read_case(case_file)
     o = pairs(m.build_gpu_options(cp))
     assert o["--amg-hierarchy"] == "cf"
     assert o["--cf-coarsening"] == "pmis"
@@ -50,11 +49,11 @@ def test_annotated_case_selects_cf_pmis_family(case_file):
     assert o["--fine-csr-refresh-every"] == "1"
     assert o["--momentum-work"] == "fgs1"
     assert o["--run-mode"] == "converge"
-    assert m._precision(cp) == "fp32"
+    assert m.precision(cp) == "fp32"
 
 
 def test_sa_family_one_override_away(case_file):
-    cp = m._read_case(case_file)
+    cp = m.read_case(case_file)
     cp.set("pressure_amg", "hierarchy", "sa")
     cp.set("pressure_amg", "smoother", "cheb2")
     cp.set("pressure", "solver", "pcg")
@@ -68,7 +67,7 @@ def test_sa_family_one_override_away(case_file):
 
 
 def test_l1jacobi_and_outer_chebyshev(case_file):
-    cp = m._read_case(case_file)
+    cp = m.read_case(case_file)
     cp.set("pressure_amg", "smoother", "l1jacobi")
     cp.set("pressure", "solver", "cheb")
     o = pairs(m.build_gpu_options(cp))
@@ -79,7 +78,7 @@ def test_l1jacobi_and_outer_chebyshev(case_file):
 
 
 def test_fixed10_consistency(case_file):
-    cp = m._read_case(case_file)
+    cp = m.read_case(case_file)
     cp.set("run", "run_mode", "fixed10")
     cp.set("simple", "max_iterations", "10")
     o = pairs(m.build_gpu_options(cp))
@@ -90,24 +89,43 @@ def test_fixed10_consistency(case_file):
 
 
 def test_cheb2_without_spectrum_rejected(case_file):
-    cp = m._read_case(case_file)
+    cp = m.read_case(case_file)
     cp.set("pressure_amg", "smoother", "cheb2")
     cp.set("pressure_amg", "spectrum_policy", "off")
     with pytest.raises(ValueError, match="requires spectrum_policy"):
         m.build_gpu_options(cp)
 
 
-def test_unknown_key_rejected(case_file):
-    cp = configparser.ConfigParser(interpolation=None)
-    cp.optionxform = str.lower
-    cp.read(case_file)
+def test_unknown_key_rejected(case_file, tmp_path):
+    cp = m.read_case(case_file)
     cp.set("pressure_amg", "smother", "jacobi")
     with pytest.raises(ValueError, match="unknown GPU case key"):
         m.build_gpu_options(cp)
+    bad = tmp_path / "bad_key.case"
+    bad.write_text("[solver]\nmesh = x\nreynolds = 20\n")
+    with pytest.raises(ValueError, match=r"unknown GPU case key \[solver\] reynolds"):
+        m.read_case(bad)
+    missing = tmp_path / "no_mesh.case"
+    missing.write_text("[solver]\nre = 20\n")
+    with pytest.raises(ValueError, match=r"missing required \[solver\] mesh"):
+        m.build_gpu_options(m.read_case(missing))
+
+
+def test_value_types_checked(case_file):
+    cp = m.read_case(case_file)
+    cp.set("simple", "max_iterations", "many")
+    with pytest.raises(ValueError, match="expected an integer"):
+        m.build_gpu_options(cp)
+    cp = m.read_case(case_file)
+    cp.set("pressure_amg_cf", "aggressive_first", "maybe")
+    with pytest.raises(ValueError, match="expected boolean"):
+        m.build_gpu_options(cp)
+    cp.set("pressure_amg_cf", "aggressive_first", "on")
+    assert pairs(m.build_gpu_options(cp))["--cf-aggressive-first"] == "1"
 
 
 def test_supg_translation(case_file):
-    cp = m._read_case(case_file)
+    cp = m.read_case(case_file)
     for k, v in (("enabled", "true"), ("tau_scale", "0.05"), ("magic", "9.0"), ("form", "implicit"),
                  ("quad_points", "64")):
         cp.set("supg", k, v)
@@ -127,14 +145,14 @@ def test_supg_translation(case_file):
 
 
 def test_set_override_and_fgmres_extension():
-    cp = m._read_case(CASE)
-    m._case_overrides(cp, ["pressure.solver=fgmres", "pressure.restart=20", "pressure_amg.hierarchy=sa"])
+    cp = m.read_case(CASE)
+    m.apply_overrides(cp, ["pressure.solver=fgmres", "pressure.restart=20", "pressure_amg.hierarchy=sa"])
     o = pairs(m.build_gpu_options(cp))
     assert o["--pressure-solver"] == "fgmres" and o["--pressure-restart"] == "20"
     with pytest.raises(ValueError, match="section.key=value"):
-        m._case_overrides(cp, ["novalue"])
+        m.apply_overrides(cp, ["novalue"])
     with pytest.raises(ValueError, match="unknown GPU case key"):
-        m._case_overrides(cp, ["pressure.nope=1"])
+        m.apply_overrides(cp, ["pressure.nope=1"])
 
 
 def test_translated_argv_accepted_by_solver_parser(tmp_path):
@@ -145,8 +163,8 @@ def test_translated_argv_accepted_by_solver_parser(tmp_path):
 
     mesh = meshgen.write_pipe_case(tmp_path / "pipe", core=4, ring=2, nslab=4)
     for hierarchy in ("cf", "sa"):  # cf (PMIS, ext+i, pmax 8) is the translator default
-        cp = m._read_case(CASE)
-        m._case_overrides(cp, [f"solver.mesh={mesh}", f"pressure_amg.hierarchy={hierarchy}"])
+        cp = m.read_case(CASE)
+        m.apply_overrides(cp, [f"solver.mesh={mesh}", f"pressure_amg.hierarchy={hierarchy}"])
         h = H8(m.build_gpu_options(cp), device=False)
         assert h.info["cells"] == (16 + 32) * 4 * 6
         h.close()
@@ -208,3 +226,12 @@ def test_dry_run_prints_command():
     line = [l for l in r.stdout.splitlines() if l.startswith("NODALS_GPU_OPTIONS_JSON=")][0]
     opts = json.loads(line.split("=", 1)[1])
     assert pairs(opts)["--amg-hierarchy"] == "cf"
+
+
+def test_launcher_time_limit_while_printing(tmp_path):
+    """The time limit applies while the solver is still printing: the child
+    group is stopped and the launcher reports 124."""
+    exe = _fake_solver(tmp_path, "import time\nprint('started', flush=True)\ntime.sleep(60)\n")
+    r, out = _run_launcher(tmp_path, exe, ["--set", "run.timeout_seconds=1"])
+    assert r.returncode == 124, r.stdout
+    assert "started" in (out / "gpu_h8_cf_pmis.log").read_text()
