@@ -564,6 +564,59 @@ def richardson(A, M, b, rtol, atol, maxit, omega=1.0):
     return x, out
 
 
+def precond_power(A, M, its, safety):
+    """pressure_precond_power (h8_main_base.cu:613-668): power iteration on
+    M A from v_i = sin(0.731 (i+1)) + 0.27 cos(1.117 (i+1))
+    (g5_power_init_kernel, g5e_fp32_kernels.cuh:151-157), normalised each
+    step; the estimate is the largest of |v.z| and |z| seen, times safety."""
+    n = A.shape[0]
+    g = np.arange(1, n + 1, dtype=np.float64)
+    v = np.sin(0.731 * g) + 0.27 * np.cos(1.117 * g)
+    v = v / math.sqrt(float(v @ v))
+    est = 0.0
+    for _ in range(its):
+        z = M(A @ v)
+        zn = math.sqrt(float(z @ z))
+        ray = float(v @ z)
+        est = max(est, max(abs(ray), zn))
+        v = z / zn
+    return safety * est
+
+
+def cheb_roots(lam_max, low_fraction, degree):
+    """Chebyshev points of [low_fraction * lam_max, lam_max] in the
+    reference's order (cheb_root, h8_main_base.cu:424-428)."""
+    lo = low_fraction * lam_max
+    centre, radius = 0.5 * (lam_max + lo), 0.5 * (lam_max - lo)
+    return [centre - radius * math.cos(math.pi * (2 * k + 1) / (2 * degree)) for k in range(degree)]
+
+
+def cheb_poly(A, M, b, rtol, atol, maxit, lam_max, low_fraction, degree):
+    """pressure_cheb_poly (h8_main_base.cu:670-726): preconditioned
+    Richardson x += z / theta_k, r -= A z / theta_k with z = M r, the step
+    sizes cycling through the Chebyshev points theta_k of
+    [low_fraction lam_max, lam_max]."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    r0 = float(np.linalg.norm(r))
+    if r0 <= atol:
+        return x, {"its": 0, "rel": 0.0, "ok": True, "r0": r0}
+    target = max(atol, rtol * r0)
+    roots = cheb_roots(lam_max, low_fraction, degree)
+    out = {"its": 0, "rel": 1.0, "ok": False, "r0": r0}
+    for k in range(maxit):
+        w = 1.0 / roots[k % degree]
+        z = M(r)
+        x += w * z
+        r -= w * (A @ z)
+        rn = float(np.linalg.norm(r))
+        out.update(its=k + 1, rel=rn / r0)
+        if rn <= target:
+            out["ok"] = True
+            break
+    return x, out
+
+
 def fgmres(A, M, b, rtol, atol, maxit, restart=30):
     """Right-preconditioned flexible GMRES(m) (textbook FGMRES, Saad 1993).
     A cycle whose least-squares residual reaches the target ends the solve:
@@ -644,7 +697,7 @@ def pressure_drop_fit(case: H8Case, p: np.ndarray) -> float:
 def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work="fgs1", p_solver="pcg",
            p_rtol=0.5, p_atol=1e-12, p_max=20, smoother: Optional[Smoother] = None, run_mode="fixed10",
            max_outer=10, simple_tol=1e-6, restart=30, U0=None, p0=None, callback=None, extra_blocks=None,
-           mom_sweeps=1):
+           mom_sweeps=1, cheb_degree=3, cheb_power_its=6, cheb_low_fraction=0.05, cheb_power_safety=1.15):
     """H8 SIMPLE loop (gpu/serial_cuda/h8_main_base.cu:2046-2119): assemble,
     B^T p + RHS, fixed momentum work, continuity, fine refresh, pressure
     Krylov + AMG, p += alpha_p dp; converge mode audits the momentum residual
@@ -661,6 +714,12 @@ def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work=
     U = np.zeros((3, case.nv)) if U0 is None else U0.copy()
     p = np.zeros(case.nc) if p0 is None else p0.copy()
     vc = VCycle(case.hier, smoother)
+    cheb_lam = None
+    if p_solver == "cheb":
+        # lambda_max of M A once, on the setup operator (h8_main_base.cu:1986-1990)
+        Af0, fdiag0 = fine_refresh(case, case.init_rau)
+        vc.set_fine(Af0, fdiag0)
+        cheb_lam = precond_power(Af0, vc.apply, cheb_power_its, cheb_power_safety)
     cont0, cont_rel = None, 1.0
     hist = []
     converged = False
@@ -697,6 +756,8 @@ def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work=
             dp, pr = richardson(Af, vc.apply, rhs, p_rtol, p_atol, p_max)
         elif p_solver == "fgmres":
             dp, pr = fgmres(Af, vc.apply, rhs, p_rtol, p_atol, p_max, restart)
+        elif p_solver == "cheb":
+            dp, pr = cheb_poly(Af, vc.apply, rhs, p_rtol, p_atol, p_max, cheb_lam, cheb_low_fraction, cheb_degree)
         else:
             raise ValueError(p_solver)
         p = p + alpha_p * dp
