@@ -301,6 +301,127 @@ def helix_leg() -> dict:
     return out
 
 
+# Config 5 with the spiral case's wall closure and solver settings
+# (examples/spiral_swept_207k/SPIRAL_SWEPT_ADAPTIVE.case): Spalding wall
+# functions with a one-point face-averaged fit and the Nitsche normal
+# penalty gamma 1e-6 ([wall_function]), skew-symmetric convection
+# ([convection]), deviatoric stress ([stress]), 20 M3 block sweeps at omega
+# 0.7 ([solver.u]), 20 + 20 k-omega sweeps at alpha 0.3 ([sst]), PCG-AMG
+# pressure to rtol 1e-3 on a CF/PMIS extended+i hierarchy with symmetric
+# MCGS ([solver.p], [amg], [native]), alpha_u 0.7 / alpha_p 0.3 ([simple]),
+# rAU scale 2 (the SST application's default).
+def config5_args(mesh: str, precision: str, max_outer: int, tol: float) -> list:
+    return ["--mesh", mesh, "--duct", "1", "--wall-model", "spalding", "--re", "50000", "--run-mode", "converge",
+            "--max-outer", str(max_outer), "--simple-tol", str(tol), "--alpha-u", "0.7", "--alpha-p", "0.3",
+            "--amg-hierarchy", "cf", "--cf-interp", "exti", "--cf-theta", "0.25", "--amg-terminal", "1000",
+            "--amg-smoother", "mcgs", "--amg-mcgs-order", "symmetric", "--pressure-solver", "pcg",
+            "--p-rtol", "1e-3", "--p-atol", "1e-12", "--p-max", "600", "--amg-precision", precision,
+            "--verbose", "0", "--tag", f"config5_{precision}"]
+
+
+def yplus30_sample_scale(re: float) -> float:
+    """Sample-height scale that puts the Spalding sample (the third of the
+    five reference heights d_i D / 0.004 * scale) at y+ = 30 for the Blasius
+    friction velocity u_tau = U sqrt(0.3164 Re^-1/4 / 8): the starting point
+    the reference's adaptive runner refines (run_adaptive_case.py:56-204)."""
+    ut = (0.3164 * re ** -0.25 / 8.0) ** 0.5
+    y = 30.0 / (re * ut)                     # D = U = 1
+    return y / (4.4084029504620439e-4 / 0.004)
+
+
+def config5_context(mesh: str, precision: str, max_outer: int, tol: float):
+    from nodals_amd._lib import H8
+
+    h = H8(config5_args(mesh, precision, max_outer, tol))
+    h.turb_init(["--sst-wall-sample-scale", repr(yplus30_sample_scale(50000.0))])
+    h.wall_init([])
+    return h
+
+
+def config5_reset(h) -> None:
+    """Start state: the inlet plug on every free vertex (zero face bubbles),
+    zero pressure, k and omega at their inlet values on the vertices."""
+    nv, nc = h.info["nv"], h.info["cells"]
+    g2 = h.get("g2free", h.get("mesh_sizes", 4, np.int64)[0] + h.get("mesh_sizes", 4, np.int64)[1], np.int32)
+    npts = int(h.get("mesh_sizes", 4, np.int64)[0])
+    vin = h.get("pipe_geometry", 12)[6:9]
+    vert = g2[:npts]
+    vert = vert[vert >= 0]
+    for d in range(3):
+        u = np.zeros(nv)
+        u[vert] = vin[d]
+        h.set(f"u{d}", u)
+    h.set("p", np.zeros(nc))
+    ns, nvert = h.turb["ns"], h.turb["nvert"]
+    for name, v in (("k", h.turb["k_in"]), ("omega", h.turb["omega_in"])):
+        f = np.zeros(ns)
+        f[:nvert] = v
+        h.set(name, f)
+
+
+def config5_leg(steps: int, warmup: int, max_outer: int, tol: float) -> dict:
+    """Config 5: k-omega SST helical pipe (D = 1, coil radius 5D, pitch 2D,
+    two turns; 576-quad O-grid section x 60 slabs, 6 tets per hex: 207,360
+    tets), Re = 50,000, uniform inlet with 5% intensity and omega from a
+    0.07 D length scale, Spalding wall functions with the sample at y+ 30,
+    from the plug to residual 1e-5; plain FP64 and FP32 AMG inside the FP64
+    Krylov solve."""
+    from nodals_amd import meshgen
+
+    d = ROOT / "_meshes" / "helix_207k"
+    if not (d / "faces").exists():
+        meshgen.write_helix_case(d)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        h = config5_context(str(d), prec, max_outer, tol)
+        try:
+            for _ in range(warmup):
+                config5_reset(h)
+                h.run()
+            times, runs = [], []
+            for _ in range(max(1, steps)):
+                config5_reset(h)
+                t0 = time.perf_counter()
+                r = h.run()
+                times.append(time.perf_counter() - t0)
+                runs.append(r)
+                print(f"[bench] config5 {prec}: {times[-1]:.2f} s, outer={r['outer']} converged={r['converged']} "
+                      f"avgP={r['avg_pressure_its']:.2f} dp={r['pressure_drop']:.6e}", file=sys.stderr, flush=True)
+            nf = h.turb["wall_faces"]
+            fit = h.get("wall_fit", 3 * nf).reshape(nf, 3)
+            shear = h.get("wall_shear", 4 * nf).reshape(nf, 4)
+            area = h.get("wall_area", nf)
+            r = runs[-1]
+            out[prec] = {"s": float(np.mean(times)), "times": times, "outer": r["outer"], "converged": r["converged"],
+                         "final_rel_cont": r["final_rel_cont"], "pressure_its": r["pressure_its"],
+                         "avg_pressure_its": r["avg_pressure_its"], "pressure_drop": r["pressure_drop"],
+                         "wall_shear_mean": float(shear[:, 3].sum() / area.sum()),
+                         "wall_force": [float(x) for x in shear[:, :3].sum(0)],
+                         "yplus_mean": float((fit[:, 2] * area).sum() / area.sum()),
+                         "utau_mean": float((fit[:, 1] * area).sum() / area.sum()),
+                         "stage_ms_per_simple": {k: r[f"{k}_ms"] / max(r["outer"], 1)
+                                                 for k in ("assembly", "momentum", "continuity", "pressure", "turb")},
+                         "cells": h.info["cells"], "free_velocity_dofs": h.info["nv"], "wall_faces": nf}
+            if prec == "fp64":
+                fields = {f"u{d}": h.get(f"u{d}", h.info["nv"]) for d in range(3)}
+                fields["p"] = h.get("p", h.info["cells"])
+                fields["k"] = h.get("k", h.turb["ns"])
+                fields["omega"] = h.get("omega", h.turb["ns"])
+            else:
+                errs = {}
+                for k, ref in fields.items():
+                    n = ref.size
+                    got = h.get(k, n)
+                    errs[k] = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+                out[prec]["rel_l2_vs_fp64"] = errs
+        finally:
+            h.close()
+    for k in ("pressure_drop", "wall_shear_mean"):
+        out["fp32"][f"{k}_rel_diff_vs_fp64"] = abs(out["fp32"][k] - out["fp64"][k]) / max(abs(out["fp64"][k]), 1e-300)
+    out["speedup_fp32_amg_vs_fp64"] = out["fp64"]["s"] / out["fp32"]["s"]
+    return out
+
+
 def learned_leg(mesh: str, steps: int, epochs: int) -> dict:
     """Learned warm start + FP32 AMG on the same pipe: samples recorded from
     the solver's own converged runs at Re 10, 15, 30, 45 (every 4th SIMPLE

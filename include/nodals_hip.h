@@ -221,6 +221,45 @@ int nodals_h8_turb_step(nodals_h8* h, nodals_turb_step_result* res);
  * turbulence, sst_application.inc:2818-2978). */
 int nodals_h8_turb_couple(nodals_h8* h, int32_t on);
 
+/* Wall-function momentum path (the reference's spiral-case momentum:
+ * turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/modules/coupling/
+ * simple_momentum_assembly.inc, simple_pressure_predictor.inc,
+ * simple_pressure_corrector.inc).  Needs a duct context created with
+ * --duct 1 --wall-model spalding and nodals_h8_turb_init.  argv (flags
+ * only): --gamma G (Nitsche normal penalty, SST22D4_GAMMA_N),
+ * --wall-active-points N (SST24B_ACTIVE_POINTS), --wall-zero-limit 0|1,
+ * --convection-form skew_symmetric|advective, --stress-form
+ * deviatoric|vector_laplacian, --m3-mode fgs|sgs, --m3-sweeps N,
+ * --m3-omega W.  After it nodals_h8_run runs the wall-function SIMPLE loop
+ * and nodals_h8_pressure_solve uses the constrained Schur operator.
+ * nodals_h8_get names: "wall_K" [nnz][9] (reference entry order),
+ * "wall_rhs", "wall_rau3", "wall_r3", "wall_delta3" [nv][3], "wall_fit"
+ * [nf][3] (beta, u_tau, y+), "wall_shear" [nf][4] (integrated traction
+ * x, y, z and |traction|), "wall_nuq" [nf][12], and the wall topology
+ * ("wall_normal", "wall_area", "wall_h", "wall_bw", "wall_bp", "wall_qw",
+ * "wall_ql", "wall_cell", "wall_lf", "wall_bub", "wall_vert"). */
+int nodals_h8_wall_init(nodals_h8* h, int argc, const char* const* argv);
+/* Face-averaged Spalding fit (f2_face_averaged_spalding,
+ * modules/wall/stage22d4b_nitsche_live.inc:25), Nitsche face blocks
+ * (s22d4_nitsche_tet_component, :99), component rows and row-L1 relaxation
+ * (s22c_finalize_scalar, modules/wall/segregated_vector_tet_22c.inc:81),
+ * the gamma-free rAU (s22d4i3a_refinv, modules/coupling/
+ * stage22d4i3a_schur.cuh:11), the 3x3 block operator K = T^T M T
+ * (m3_build, modules/wall/stage22d4d_coupled_matrixfree.inc:934) and
+ * T^T on the right-hand side, at the current u, p, k, omega. */
+int nodals_h8_wall_assemble(nodals_h8* h);
+/* The fixed block Gauss-Seidel momentum sweeps on K (m3_fixed_coupled_csr,
+ * stage22d4d_coupled_matrixfree.inc:967; the momentum_solve call of
+ * hxt5e_fgmres.cuh:480-527 in its M3 mode). */
+int nodals_h8_wall_momentum(nodals_h8* h);
+/* y = B T Q R Q T^T B^T x for host vectors of length cells
+ * (S22D4I3ASchur::apply, stage22d4i3a_schur.cuh:70-74). */
+int nodals_h8_wall_schur(nodals_h8* h, const double* x, double* y);
+/* u += T Q R Q T^T B^T dp for a host vector dp (S22D4I3ASchur::correct,
+ * stage22d4i3a_schur.cuh:61-69, with this build's sign convention
+ * p += alpha_p dp). */
+int nodals_h8_wall_correct(nodals_h8* h, const double* dp);
+
 /* ---------------------------------------------------------------- GNN ---
  * Learned pressure-correction warm start (no reference counterpart; the
  * reference's only data-driven pressure aid is the snapshot POD of
@@ -289,6 +328,18 @@ int nodals_wall_spalding(int64_t n, const double* slip, const double* y, double 
  * out[n][6] = u_tau y+ beta u_mid fit_rms point_u_tau_rel_span; valid[n]. */
 int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, const double* y, double nu,
                          double* out, int32_t* valid);
+/* Tangential Spalding fit of a stationary wall per face
+ * (nodals::wall::vector3::fit_active, modules/wall/portable/
+ * spalding_active.hpp:129-154, with active::fit_values :48-128):
+ * samples[n][5][3] velocities at the five wall-normal slots, y[n][5] their
+ * heights, nrm[n][3] the unit wall normal, pic[n][3] the Picard
+ * velocity; count = active points 1..5 (slots {2}, {0,4}, {0,2,4},
+ * {0,1,3,4}, all); zero_limit keeps the fit's molecular nu/y coefficient
+ * when a count < 5 fit returns u_tau = 0 (STAGE27_SPALDING_ZERO_LIMIT).
+ * out[n][3] = beta u_tau y+; valid[n]. */
+int nodals_wall_fit_tangential(int64_t n, const double* samples, const double* y, const double* nrm,
+                               const double* pic, double nu, int32_t count, int32_t zero_limit, double* out,
+                               int32_t* valid);
 /* OpenFOAM-style omega wall target (evaluate_of_auto_host,
  * modules/wall/portable/wall_host.hpp:5-9): in[n][15] = y nu Ut uTau k omega
  * kFloor omegaFloor kappa E beta1 reBlend prodLimit pk nut; shared[n];

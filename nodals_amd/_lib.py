@@ -55,10 +55,13 @@ EXPORTS = [
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
     "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
+    "nodals_wall_fit_tangential",
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
     "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
     "nodals_h8_turb_init", "nodals_h8_turb_step", "nodals_h8_turb_couple",
+    "nodals_h8_wall_init", "nodals_h8_wall_assemble", "nodals_h8_wall_momentum", "nodals_h8_wall_schur",
+    "nodals_h8_wall_correct",
 ]
 
 
@@ -97,6 +100,7 @@ def lib() -> C.CDLL:
     L.nodals_wall_spalding.argtypes = [C.c_int64, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_profile5.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_omega_target.argtypes = [C.c_int64, vp, vp, vp]
+    L.nodals_wall_fit_tangential.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp]
     L.nodals_gnn_create_for_h8.argtypes = [vp, C.c_uint32, C.POINTER(vp)]
     L.nodals_gnn_destroy.argtypes = [vp]
     L.nodals_gnn_info.argtypes = [vp, vp]
@@ -113,6 +117,11 @@ def lib() -> C.CDLL:
     L.nodals_h8_turb_init.argtypes = [vp, C.c_int, vp]
     L.nodals_h8_turb_step.argtypes = [vp, vp]
     L.nodals_h8_turb_couple.argtypes = [vp, C.c_int32]
+    L.nodals_h8_wall_init.argtypes = [vp, C.c_int, vp]
+    L.nodals_h8_wall_assemble.argtypes = [vp]
+    L.nodals_h8_wall_momentum.argtypes = [vp]
+    L.nodals_h8_wall_schur.argtypes = [vp, vp, vp]
+    L.nodals_h8_wall_correct.argtypes = [vp, vp]
     L.nodals_sys_create.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_create_host.argtypes = [C.c_int32, vp, vp, vp, C.c_char_p, C.POINTER(vp)]
     L.nodals_sys_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -266,6 +275,32 @@ class H8:
         r = TurbStepResult()
         _check(lib().nodals_h8_turb_step(self._h, C.byref(r)))
         return r.as_dict()
+
+    def wall_init(self, args: Sequence[str] = ()) -> None:
+        """Wall-function momentum path (duct created with --wall-model
+        spalding, after turb_init); args are its flags, e.g. ["--m3-sweeps",
+        "20"].  Afterwards run() is the wall-function SIMPLE loop."""
+        a = [str(x).encode() for x in args]
+        arr = (C.c_char_p * max(len(a), 1))(*a) if a else None
+        _check(lib().nodals_h8_wall_init(self._h, len(a), arr))
+
+    def wall_assemble(self) -> None:
+        _check(lib().nodals_h8_wall_assemble(self._h))
+
+    def wall_momentum(self) -> None:
+        _check(lib().nodals_h8_wall_momentum(self._h))
+
+    def wall_schur(self, x) -> np.ndarray:
+        nc = self.info["cells"]
+        x = _f64(x, (nc,))
+        y = np.empty(nc)
+        _check(lib().nodals_h8_wall_schur(self._h, _ptr(x), _ptr(y)))
+        return y
+
+    def wall_correct(self, dp) -> None:
+        nc = self.info["cells"]
+        d = _f64(dp, (nc,))
+        _check(lib().nodals_h8_wall_correct(self._h, _ptr(d)))
 
     def gnn_graph(self):
         """(off, nbr) of the face-adjacent cell graph the network runs on."""
@@ -433,6 +468,18 @@ def wall_profile5(U, Us, y, nu):
     out = np.empty((n, 6))
     valid = np.empty(n, dtype=np.int32)
     _check(lib().nodals_wall_profile5(n, _ptr(U), _ptr(Us), _ptr(y), nu, _ptr(out), _ptr(valid)))
+    return out, valid
+
+
+def wall_fit_tangential(samples, y, nrm, pic, nu, count, zero_limit=True):
+    """Tangential Spalding fit per face on the GPU: samples (n,5,3), y (n,5),
+    nrm and pic (n,3); returns out (n,3) = beta u_tau y+ and valid (n,)."""
+    n = int(np.asarray(y).size // 5)
+    s, yy, nn, pp = _f64(samples, (n, 5, 3)), _f64(y, (n, 5)), _f64(nrm, (n, 3)), _f64(pic, (n, 3))
+    out = np.empty((n, 3))
+    valid = np.empty(n, dtype=np.int32)
+    _check(lib().nodals_wall_fit_tangential(n, _ptr(s), _ptr(yy), _ptr(nn), _ptr(pp), nu, int(count),
+                                            1 if zero_limit else 0, _ptr(out), _ptr(valid)))
     return out, valid
 
 

@@ -11,6 +11,7 @@
 #include "gnn.hpp"
 #include "sst_kernels.hpp"
 #include "sys_solver.hpp"
+#include "wall_state.hpp"
 
 struct nodals_h8 {
   std::unique_ptr<nodals::H8Solver> s;
@@ -218,6 +219,7 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
     else if (n == "p" || n == "cont" || n == "dp" || n == "fine_diag" || n == "fine_l1") copyOut(s.getCellField(n), host, count);
     else if (n == "mom_val") copyOut(s.getMomentumValues(), host, count);
     else if (n == "fine_val") copyOut(s.getFineValues(), host, count);
+    else if (n.rfind("wall_", 0) == 0 && n != "wall_distance") copyOut(s.wallField(n), host, count);
     else if (n == "mom_row") copyOut(s.setup().mom.row, host, count);
     else if (n == "mom_col") copyOut(s.setup().mom.col, host, count);
     else if (n == "mom_diff") copyOut(s.setup().mom.val, host, count);
@@ -557,6 +559,12 @@ int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, con
   return guarded([&]() { nodals::profile5Batch(n, U, U_signed, y, nu, out, valid); });
 }
 
+int nodals_wall_fit_tangential(int64_t n, const double* samples, const double* y, const double* nrm,
+                               const double* pic, double nu, int32_t count, int32_t zero_limit, double* out,
+                               int32_t* valid) {
+  return guarded([&]() { nodals::fitTangentialBatch(n, samples, y, nrm, pic, nu, count, zero_limit, out, valid); });
+}
+
 int nodals_wall_omega_target(int64_t n, const double* in, const int32_t* shared, double* out) {
   return guarded([&]() { nodals::omegaWallBatch(n, in, shared, out); });
 }
@@ -720,6 +728,55 @@ int nodals_h8_turb_step(nodals_h8* h, nodals_turb_step_result* res) {
 
 int nodals_h8_turb_couple(nodals_h8* h, int32_t on) {
   return guarded([&]() { solver(h).turbCouple(on != 0); });
+}
+
+int nodals_h8_wall_init(nodals_h8* h, int argc, const char* const* argv) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(argc == 0 || argv != nullptr, "null argv");
+    s.wallInit(nodals::parseWallArgs(argc, argv));
+  });
+}
+
+int nodals_h8_wall_assemble(nodals_h8* h) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    s.wallAssemble();
+    s.sync();
+  });
+}
+
+int nodals_h8_wall_momentum(nodals_h8* h) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    s.wallMomentum();
+    s.sync();
+  });
+}
+
+int nodals_h8_wall_schur(nodals_h8* h, const double* x, double* y) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(x != nullptr && y != nullptr, "null argument");
+    nodals::require(s.hasWall(), "wall-function path not initialised (nodals_h8_wall_init)");
+    const int nc = s.setup().nc;
+    NODALS_HIP(hipMemcpyAsync(s.devScratchA(), x, sizeof(double) * nc, hipMemcpyHostToDevice, s.stream()));
+    s.wallSchur(s.stream(), s.devScratchA(), s.devScratchB());
+    NODALS_HIP(hipMemcpyAsync(y, s.devScratchB(), sizeof(double) * nc, hipMemcpyDeviceToHost, s.stream()));
+    s.sync();
+  });
+}
+
+int nodals_h8_wall_correct(nodals_h8* h, const double* dp) {
+  return guarded([&]() {
+    auto& s = solver(h);
+    nodals::require(dp != nullptr, "null argument");
+    nodals::require(s.hasWall(), "wall-function path not initialised (nodals_h8_wall_init)");
+    const int nc = s.setup().nc;
+    NODALS_HIP(hipMemcpyAsync(s.devScratchA(), dp, sizeof(double) * nc, hipMemcpyHostToDevice, s.stream()));
+    s.wallCorrect(s.devScratchA());
+    s.sync();
+  });
 }
 
 int nodals_h8_attach_gnn(nodals_h8* h, nodals_gnn* g, int32_t warm_start, int32_t record_every) {

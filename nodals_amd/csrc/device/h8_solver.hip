@@ -11,6 +11,8 @@
 
 #include "h8_kernels.hpp"
 #include "sst_volume.hpp"
+#include "wall_kernels.hpp"
+#include "wall_state.hpp"
 
 namespace nodals {
 
@@ -121,6 +123,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--outlet") o.outlet = next();
     else if (a == "--lid") o.lid = next();
     else if (a == "--duct") o.duct = inum(next()) != 0;
+    else if (a == "--wall-model") o.wallModel = next();
     else if (a == "--re") o.re = num(next());
     else if (a == "--bulk") o.bulk = num(next());
     else if (a == "--mixing-length") o.mixingLength = inum(next()) != 0;
@@ -139,7 +142,10 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--max-outer") o.maxOuter = inum(next());
     else if (a == "--alpha-u") o.alphaU = num(next());
     else if (a == "--alpha-p") o.alphaP = num(next());
-    else if (a == "--rau-scale") o.rauScale = num(next());
+    else if (a == "--rau-scale") {
+      o.rauScale = num(next());
+      o.rauScaleGiven = true;
+    }
     else if (a == "--mom-rtol") o.momRtol = num(next());
     else if (a == "--mom-atol") o.momAtol = num(next());
     else if (a == "--mom-drop") o.momDrop = num(next());
@@ -186,6 +192,10 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else fail("H8 usage error: unknown option " + a);
   }
   require(!o.mesh.empty(), "--mesh required");
+  require(o.wallModel == "no_slip" || o.wallModel == "spalding", "--wall-model must be no_slip or spalding");
+  require(o.wallModel == "no_slip" || o.duct, "--wall-model spalding needs --duct 1");
+  // The SST application's default rAU scale (sst_application.inc:69).
+  if (o.wallModel == "spalding" && !o.rauScaleGiven) o.rauScale = 2.0;
   require(o.simpleTol > 0.0 && o.maxOuter >= 1, "H8 invalid SIMPLE controls");
   require(o.alphaU > 0.0 && o.alphaU <= 1.0 && o.alphaP > 0.0 && o.alphaP <= 1.0, "H8 alpha-u/alpha-p must be in (0,1]");
   require(o.rauScale > 0.0 && std::isfinite(o.rauScale), "H8 rau-scale must be finite and > 0");
@@ -257,6 +267,7 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   prm.outlet = opt_.outlet;
   prm.lid = opt_.lid;
   prm.duct = opt_.duct;
+  prm.wallFunction = opt_.wallModel == "spalding";
   prm.re = opt_.re;
   prm.bulk = opt_.bulk;
   prm.alphaU = opt_.alphaU;
@@ -446,6 +457,7 @@ void H8Solver::upload() {
     }
     mdp[(std::size_t)i] = (int)newOfOld[(std::size_t)S_.momDiagPos[(std::size_t)r]];
   }
+  newOfOld_ = newOfOld;
   mrow_.upload(mrow);
   mcol_.upload(mcol);
   mdiagPos_.upload(mdp);
@@ -628,7 +640,7 @@ void H8Solver::setupAmg() {
 void H8Solver::convLocal(hipStream_t s, int waves) {
   waves = std::max(1, std::min(nc_, waves));
   hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, s, nc_, cref_.data(), cinvJ_.data(),
-                     cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), cbuf_.data());
+                     cdet_.data(), u_.data(), fixedVal_.data(), momT_ ? momT_ : centT_.data(), cbuf_.data());
   NODALS_LAUNCHED();
 }
 
@@ -770,8 +782,12 @@ void H8Solver::refresh() {
 }
 
 void H8Solver::refreshOn(hipStream_t s) {
-  hipLaunchKernelGGL(k_fine_weights, blocksFor((long long)nc_ * 8), 256, 0, s, nc_, cref_.data(), bcoef_.data(),
-                     rau_.data(), fineW_.data());
+  if (rau3_)  // wall-function path: B diag(rAU_x, rAU_y, rAU_z) B^T (hxt5b_boundary.cuh:1883)
+    hipLaunchKernelGGL(k_fine_weights3, blocksFor((long long)nc_ * 8), 256, 0, s, nc_, cref_.data(), bcoef_.data(),
+                       rau3_, fineW_.data());
+  else
+    hipLaunchKernelGGL(k_fine_weights, blocksFor((long long)nc_ * 8), 256, 0, s, nc_, cref_.data(), bcoef_.data(),
+                       rau_.data(), fineW_.data());
   NODALS_LAUNCHED();
   // A short refresh (at most five rounds of four 256-thread blocks per CU)
   // runs on four blocks per CU (half of a CU's wave slots), each looping
@@ -1062,7 +1078,10 @@ KrylovResult H8Solver::pressureSolveFrom(const std::string& solver, double rtol,
                                          const double* x0, const double* z0) {
   requireDevice();
   require((!x0 && !z0) || solver == "fgmres", "an initial pressure guess needs the fgmres pressure solver");
-  DevOp A = [this](hipStream_t s, const double* x, double* y) { fineApply(s, x, y); };
+  DevOp A = [this](hipStream_t s, const double* x, double* y) {
+    if (wall_) wallSchur(s, x, y);
+    else fineApply(s, x, y);
+  };
   DevOp M = [this](hipStream_t s, const double* r, double* z) { precondApply(s, r, z); };
   KrylovResult r;
   if (solver == "pcg") r = pcg(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_);
@@ -1098,6 +1117,7 @@ double H8Solver::rauSnapshotRel() {
 // ------------------------------------------------------------------ SIMPLE loop
 H8RunResult H8Solver::run() {
   requireDevice();
+  if (wall_) return runWall();
   const H8Options& o = opt_;
   H8RunResult R;
   R.setupMs = setupMs_;
@@ -1404,6 +1424,19 @@ double H8Solver::timeKernel(const std::string& kernel, int reps, double* bytes) 
   } else if (kernel == "assemble") {
     launch = [this]() { assemble(opt_.alphaU, 1.0); };
     by = b.convLocal + b.momRows;
+  } else if (kernel == "wall_momentum") {  // the wall path's block sweeps (with Q and T)
+    require((bool)wall_, "wall_momentum needs the wall-function path", kErrInvalid);
+    launch = [this]() { wallMomentum(); };
+    by = b.m3Sweep * (double)wallSweeps();
+  } else if (kernel == "wall_assemble") {
+    require((bool)wall_, "wall_assemble needs the wall-function path", kErrInvalid);
+    launch = [this]() { wallAssemble(); };
+    by = b.convLocal + b.momRows;
+  } else if (kernel == "wall_schur") {
+    require((bool)wall_, "wall_schur needs the wall-function path", kErrInvalid);
+    fill<double>(stream_, nc_, tmpA_.data(), 1.0);
+    launch = [this]() { wallSchur(stream_, tmpA_.data(), tmpB_.data()); };
+    by = b.wallSchur;
   } else if (kernel == "copy") {
     // Plain streaming copy of 64 Mi doubles each way (512 MiB, far beyond the
     // caches): the calibration kernel for HBM byte counters.
@@ -1458,6 +1491,12 @@ H8Solver::Bytes H8Solver::algorithmicBytes() const {
   b.convLocal = n * (8 * 4 + 9 * 8 + 8 + 64 * 8) + nv_ * 24.0;
   b.momRows = momNnz_ * (8.0 + 8.0 + 4.0) + (double)(momNnz_) * 1.75 * (4 + 8) + nv_ * (4 * 8.0 + 24 * 3);
   b.continuity = n * (8 * 4 + 24 * 8 + 8 + 8) + nv_ * 24.0;
+  // one forward M3 block sweep: 3x3 blocks + column per nonzero, row and
+  // diagonal pointers, bubble-face map, b (3), u read and write (3 each)
+  b.m3Sweep = momNnz_ * (72.0 + 4.0) + (nv_ + 1) * 4.0 + nv_ * (4.0 + 4.0 + 24.0 + 48.0);
+  // constrained Schur operator: incidence, B table, cell refs, x and y once
+  // each, gamma-free rAU, the wall-DOF map and the output
+  b.wallSchur = (nv_ + 1) * 4.0 + (double)S_.nc * 8 * 4 + n * (24 * 8 + 8 * 4 + 8 + 8) + nv_ * (24.0 + 4.0 + 48.0);
   return b;
 }
 

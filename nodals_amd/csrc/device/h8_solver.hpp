@@ -18,6 +18,7 @@
 #include "../host/amg_setup.hpp"
 #include "../host/h8_setup.hpp"
 #include "../host/turb_setup.hpp"
+#include "../host/wall_setup.hpp"
 #include "amg.hpp"
 #include "gnn.hpp"
 #include "krylov.hpp"
@@ -25,10 +26,16 @@
 
 namespace nodals {
 
+struct WallState;
+
 struct H8Options {
   std::string mesh, tag = "h8", wall = "patch_0_0", inlet = "patch_2_0", outlet = "patch_1_0", vtuOut;
   std::string lid;  // extension: lid-driven cavity on this patch (no inlet/outlet, pressure pinned in cell 0)
   bool duct = false;  // extension: general duct (uniform inflow, hydraulic diameter, patch pressure drop)
+  // extension: "spalding" makes the duct wall a wall-function wall (free
+  // velocity DOFs, Nitsche/Spalding closure; needs the k-omega model)
+  std::string wallModel = "no_slip";
+  bool rauScaleGiven = false;
   double re = 20, bulk = 1, simpleTol = 1e-6;
   double alphaU = .5, alphaP = .5, rauScale = 1.0;
   double momRtol = 1e-6, momAtol = 1e-12, momDrop = .1, momOmega = 1.0;
@@ -182,6 +189,29 @@ class H8Solver {
   // (kIn, omegaIn).
   std::vector<double> turbField(const std::string& name);
   void setTurbField(const std::string& name, const std::vector<double>& v);
+  // Wall-function momentum path (a duct created with --wall-model spalding,
+  // after turbInit): the walls carry the face-averaged Spalding traction and
+  // the Nitsche normal penalty, the wall-face bubbles the exact BF3 mean-flux
+  // constraint u = T x, and the momentum is one 3x3 block operator
+  // K = T^T M T swept by block Gauss-Seidel (the reference's
+  // m3_fixed_coupled_csr).  The pressure Krylov operator becomes
+  // B T Q R Q T^T B^T (R the gamma-free rAU) and the SIMPLE update corrects
+  // the velocity with the same operator; run() runs this loop.
+  void wallInit(const WallParams& P);
+  bool hasWall() const { return (bool)wall_; }
+  // Fit, wall blocks, component rows and relaxation, K, T^T b.
+  void wallAssemble();
+  // Q u, the block sweeps, T u.
+  void wallMomentum();
+  // y = B T Q R Q T^T B^T x on device vectors of length cells.
+  void wallSchur(hipStream_t s, const double* x, double* y);
+  // u += T Q R Q T^T B^T dp for a device vector dp (cells).
+  void wallCorrect(const double* dp);
+  // "wall_K" [nnz][9] (reference entry order), "wall_rhs", "wall_rau3",
+  // "wall_r3", "wall_delta3" [nv][3] (reference free order), "wall_fit"
+  // [nf][3], "wall_shear" [nf][4], "wall_nuq" [nf][12], topology arrays
+  // "wall_normal" [nf][3], "wall_area", "wall_bw", "wall_bp", "wall_qw".
+  std::vector<double> wallField(const std::string& name);
   double* devScratchA() { return tmpA_.data(); }
   double* devScratchB() { return tmpB_.data(); }
   void sync();
@@ -189,6 +219,7 @@ class H8Solver {
   // algorithmic bytes of one application of each hot kernel
   struct Bytes {
     double fineSpmv = 0, refresh = 0, momGsSweep = 0, convLocal = 0, momRows = 0, continuity = 0;
+    double m3Sweep = 0, wallSchur = 0;  // wall-function path
   };
   Bytes algorithmicBytes() const;
 
@@ -268,6 +299,13 @@ class H8Solver {
   int turbCounters(TurbStepResult* R);  // reads the step's counters (syncs)
   std::unique_ptr<TurbState> turb_;
   bool turbCouple_ = false;
+  std::unique_ptr<WallState> wall_;
+  H8RunResult runWall();
+  int wallSweeps() const;
+  int wallCounters(bool sweep);   // reads and checks the wall counters (syncs)
+  std::vector<long long> newOfOld_;  // reference momentum entry -> device entry
+  const double* momT_ = nullptr;     // convection tensor of the momentum (null: centT_)
+  const double* rau3_ = nullptr;     // anisotropic rAU of the fine Schur refresh (null: rau_)
   GnnModel* gnn_ = nullptr;
   bool gnnWarm_ = false, gnnAsGuess_ = false;
   int gnnEvery_ = 0;

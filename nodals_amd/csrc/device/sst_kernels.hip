@@ -57,6 +57,24 @@ __global__ void k_profile5(long long n, const double* __restrict__ U, const doub
   valid[i] = p.valid;
 }
 
+__global__ void k_fit_tangential(long long n, const double* __restrict__ samples, const double* __restrict__ y,
+                                 const double* __restrict__ nrm, const double* __restrict__ pic, double nu, int count,
+                                 int zeroLimit, double* __restrict__ out, int* __restrict__ valid) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sx[5], sy[5], sz[5];
+  for (int m = 0; m < 5; ++m) {
+    sx[m] = samples[15 * i + 3 * m];
+    sy[m] = samples[15 * i + 3 * m + 1];
+    sz[m] = samples[15 * i + 3 * m + 2];
+  }
+  const sst::VecFit f = sst::fitTangential(sx, sy, sz, y + 5 * i, nrm + 3 * i, pic + 3 * i, nu, count, zeroLimit != 0);
+  out[3 * i] = f.beta;
+  out[3 * i + 1] = f.uTau;
+  out[3 * i + 2] = f.yPlus;
+  valid[i] = f.valid;
+}
+
 __global__ void k_omega_wall(long long n, const double* __restrict__ in, const int* __restrict__ shared,
                              double* __restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,6 +164,25 @@ void profile5Batch(long long n, const double* U, const double* Us, const double*
                      dv.data());
   NODALS_LAUNCHED();
   S.out(out, dout, 6 * (std::size_t)n);
+  S.out(valid, dv, (std::size_t)n);
+  S.sync();
+}
+
+void fitTangentialBatch(long long n, const double* samples, const double* y, const double* nrm, const double* pic,
+                        double nu, int count, int zeroLimit, double* out, int* valid) {
+  require(n >= 0 && samples && y && nrm && pic && out && valid, "fit_tangential: null or bad arguments");
+  require(count >= 1 && count <= 5, "fit_tangential: count must be in 1..5");
+  if (n == 0) return;
+  Staging S;
+  DBuf<double> ds = S.in(samples, 15 * (std::size_t)n), dy = S.in(y, 5 * (std::size_t)n);
+  DBuf<double> dn = S.in(nrm, 3 * (std::size_t)n), dp = S.in(pic, 3 * (std::size_t)n), dout;
+  dout.alloc(3 * (std::size_t)n);
+  DBuf<int> dv;
+  dv.alloc((std::size_t)n);
+  hipLaunchKernelGGL(k_fit_tangential, blocksFor(n), 256, 0, S.s, n, ds.data(), dy.data(), dn.data(), dp.data(), nu,
+                     count, zeroLimit, dout.data(), dv.data());
+  NODALS_LAUNCHED();
+  S.out(out, dout, 3 * (std::size_t)n);
   S.out(valid, dv, (std::size_t)n);
   S.sync();
 }

@@ -341,6 +341,197 @@ NODALS_SST_FN Profile fitProfile5(const double* U, const double* Us, const doubl
   return p;
 }
 
+// Spalding least squares over n of the five wall-normal sample slots
+// (modules/wall/portable/spalding_active.hpp:24-128): for n = 1..4 the
+// physical slots {2}, {0,4}, {0,2,4}, {0,1,3,4}; for n = 5 the five-point
+// fit above.  Same bracket, Gauss-Newton with halving line search and
+// golden-section fallback as the five-point fit, on the n active points;
+// the reference height and signed speed are those of point n/2.
+NODALS_SST_FN int activeSlot(int n, int j) {
+  const int ids[5][5] = {{2, -1, -1, -1, -1}, {0, 4, -1, -1, -1}, {0, 2, 4, -1, -1}, {0, 1, 3, 4, -1},
+                         {0, 1, 2, 3, 4}};
+  return (n >= 1 && n <= 5 && j >= 0 && j < n) ? ids[n - 1][j] : -1;
+}
+NODALS_SST_FN unsigned activeMask(int n) {
+  return n == 1 ? 0x04u : n == 2 ? 0x11u : n == 3 ? 0x15u : n == 4 ? 0x1bu : n == 5 ? 0x1fu : 0u;
+}
+NODALS_SST_FN double profileObjectiveN(const double* U, const double* y, int n, double nu, double ut) {
+  if (!(n >= 1 && n <= 5 && ut > 0.0 && nu > 0.0) || !isfinite(ut)) return 1.0e300;
+  double J = 0.0;
+  for (int m = 0; m < n; ++m) {
+    const double r = yPlusOf(U[m] / ut, kKappa, kB) - y[m] * ut / nu;
+    if (!isfinite(r)) return 1.0e300;
+    J += r * r;
+  }
+  return J / (double)n;
+}
+NODALS_SST_FN bool profileGNN(const double* U, const double* y, int n, double nu, double ut, double& J, double& g,
+                              double& h) {
+  if (!(n >= 1 && n <= 5 && nu > 0.0 && ut > 0.0) || !isfinite(ut)) return false;
+  double sum = 0.0;
+  g = 0.0;
+  h = 0.0;
+  for (int m = 0; m < n; ++m) {
+    const double up = U[m] / ut;
+    const double r = yPlusOf(up, kKappa, kB) - y[m] * ut / nu;
+    const double dr = -(up / ut) * dYPlusOf(up, kKappa, kB) - y[m] / nu;
+    if (!isfinite(r) || !isfinite(dr)) return false;
+    sum += r * r;
+    g += r * dr;
+    h += dr * dr;
+  }
+  J = sum / (double)n;
+  return isfinite(J) && isfinite(g) && isfinite(h);
+}
+NODALS_SST_FN Profile fitActive(const double* Uall, const double* Sall, const double* yall, int n, double nu) {
+  if (n == 5) return fitProfile5(Uall, Sall, yall, nu);
+  Profile p{0, 0, 0, 0, 0, 0, 0};
+  if (n < 1 || n > 4 || !(nu > 0.0)) return p;
+  double U[5] = {0, 0, 0, 0, 0}, S[5] = {0, 0, 0, 0, 0}, y[5] = {0, 0, 0, 0, 0};
+  double minRoot = 1.0e300, maxRoot = 0.0, sumRoot = 0.0, Umax = 0.0;
+  int nRoot = 0;
+  for (int m = 0; m < n; ++m) {
+    const int id = activeSlot(n, m);
+    U[m] = Uall[id];
+    S[m] = Sall[id];
+    y[m] = yall[id];
+    if (!(y[m] > 0.0) || !isfinite(y[m]) || !isfinite(U[m]) || !isfinite(S[m]) || (m > 0 && !(y[m] > y[m - 1])))
+      return p;
+    Umax = fmax(Umax, U[m]);
+    double r = 0.0, yp = 0.0, bt = 0.0;
+    if (spaldingRoot(U[m], y[m], nu, r, yp, bt) && r > 0.0 && isfinite(r)) {
+      minRoot = fmin(minRoot, r);
+      maxRoot = fmax(maxRoot, r);
+      sumRoot += r;
+      ++nRoot;
+    }
+  }
+  const int mid = n / 2;
+  p.uMid = S[mid];
+  if (Umax <= 1.0e-14) {
+    p.beta = nu / y[mid];
+    p.valid = isfinite(p.beta) && p.beta > 0.0;
+    return p;
+  }
+  double scale = nRoot ? sumRoot / nRoot : sqrt(nu * Umax / fmax(y[mid], 1.0e-30));
+  if (!(scale > 0.0) || !isfinite(scale)) scale = 1.0;
+  double lo = fmax(0.05 * scale, 1.0e-10), hi = fmax(20.0 * scale, 100.0 * lo);
+  if (nRoot) {
+    lo = fmax(fmin(lo, 0.20 * minRoot), 1.0e-10);
+    hi = fmax(hi, 5.0 * maxRoot);
+  }
+  double ut = fmin(hi, fmax(lo, scale));
+  bool gnOK = true;
+  for (int it = 0; it < 12; ++it) {
+    double J0 = 0.0, g = 0.0, h = 0.0;
+    if (!profileGNN(U, y, n, nu, ut, J0, g, h) || J0 >= 1.0e299 || !(h > 0.0)) {
+      gnOK = false;
+      break;
+    }
+    double step = g / h, trial = fmin(hi, fmax(lo, ut - step));
+    double Jt = profileObjectiveN(U, y, n, nu, trial);
+    for (int ls = 0; ls < 10 && (!isfinite(Jt) || Jt >= 1.0e299 || Jt > J0); ++ls) {
+      step *= 0.5;
+      trial = fmin(hi, fmax(lo, ut - step));
+      Jt = profileObjectiveN(U, y, n, nu, trial);
+    }
+    if (!isfinite(Jt) || Jt >= 1.0e299 || Jt > J0) {
+      gnOK = false;
+      break;
+    }
+    const double rel = fabs(trial - ut) / fmax(ut, 1.0e-30);
+    ut = trial;
+    if (rel < 1.0e-10) break;
+  }
+  double J = profileObjectiveN(U, y, n, nu, ut);
+  if (!gnOK || !(ut > 0.0) || !isfinite(ut) || !isfinite(J) || J >= 1.0e299) {
+    double xa = log(lo), xb = log(hi);
+    const double phi = 0.6180339887498948482;
+    double x1 = xb - phi * (xb - xa), x2 = xa + phi * (xb - xa);
+    double f1 = profileObjectiveN(U, y, n, nu, exp(x1)), f2 = profileObjectiveN(U, y, n, nu, exp(x2));
+    for (int it = 0; it < 28; ++it) {
+      if (f1 <= f2) {
+        xb = x2;
+        x2 = x1;
+        f2 = f1;
+        x1 = xb - phi * (xb - xa);
+        f1 = profileObjectiveN(U, y, n, nu, exp(x1));
+      } else {
+        xa = x1;
+        x1 = x2;
+        f1 = f2;
+        x2 = xa + phi * (xb - xa);
+        f2 = profileObjectiveN(U, y, n, nu, exp(x2));
+      }
+    }
+    ut = exp(f1 <= f2 ? x1 : x2);
+    J = fmin(f1, f2);
+  }
+  if (!(ut > 0.0) || !isfinite(ut) || !isfinite(J) || J >= 1.0e299) return p;
+  p.uTau = ut;
+  p.yPlus = y[mid] * ut / nu;
+  p.beta = fabs(p.uMid) > 1.0e-14 ? ut * ut / fabs(p.uMid) : nu / y[mid];
+  p.fitRms = sqrt(fmax(J, 0.0));
+  p.span = nRoot >= 2 && maxRoot >= minRoot ? (maxRoot - minRoot) / fmax(ut, 1.0e-30) : 0.0;
+  p.valid = isfinite(p.yPlus) && isfinite(p.beta) && p.beta > 0.0 && isfinite(p.fitRms) && isfinite(p.span);
+  return p;
+}
+
+// Tangential Spalding fit of a stationary wall with unit normal n
+// (modules/wall/portable/spalding_vector_core.hpp:54-84 and
+// spalding_active.hpp:129-154): the speeds are the norms of the samples'
+// tangential projections; the Robin coefficient is u_tau^2 over the
+// tangential speed of the Picard functional pic, or the fit's own
+// coefficient when that speed is below 1e-6 max(|u_mid|, 1).  zeroLimit
+// (the reference's STAGE27_SPALDING_ZERO_LIMIT) keeps the fit's molecular
+// limit nu/y when an n < 5 point fit returns u_tau = 0.
+struct VecFit {
+  double beta, uTau, yPlus;
+  int valid;
+};
+NODALS_SST_FN VecFit fitTangential(const double sx[5], const double sy[5], const double sz[5], const double y[5],
+                                   const double n[3], const double pic[3], double nu, int count, bool zeroLimit) {
+  VecFit out{0, 0, 0, 0};
+  const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (count < 1 || count > 5 || !(nu > 0.0) || !isfinite(nu) || !isfinite(n2) || !(fabs(n2 - 1.0) <= 1.e-9) ||
+      !isfinite(pic[0]) || !isfinite(pic[1]) || !isfinite(pic[2]))
+    return out;
+  const unsigned mask = activeMask(count);
+  double U[5] = {0, 0, 0, 0, 0}, Y[5] = {0, 0, 0, 0, 0};
+  double prevY = 0.0;
+  for (int i = 0; i < 5; ++i) {
+    if (count < 5 && !(mask & (1u << i))) {
+      Y[i] = y[i];
+      continue;
+    }
+    if (!isfinite(sx[i]) || !isfinite(sy[i]) || !isfinite(sz[i]) || !isfinite(y[i]) || !(y[i] > 0.0)) return out;
+    if (count < 5) {
+      if (!(y[i] > prevY)) return out;
+      prevY = y[i];
+    } else if (i > 0 && !(y[i] > y[i - 1])) {
+      return out;
+    }
+    const double vn = sx[i] * n[0] + sy[i] * n[1] + sz[i] * n[2];
+    const double tx = sx[i] - vn * n[0], ty = sy[i] - vn * n[1], tz = sz[i] - vn * n[2];
+    U[i] = sqrt(tx * tx + ty * ty + tz * tz);
+    Y[i] = y[i];
+    if (!isfinite(U[i])) return out;
+  }
+  const Profile p = fitActive(U, U, Y, count, nu);
+  if (!p.valid) return out;
+  const double pn = pic[0] * n[0] + pic[1] * n[1] + pic[2] * n[2];
+  const double px = pic[0] - pn * n[0], py = pic[1] - pn * n[1], pz = pic[2] - pn * n[2];
+  const double speed = sqrt(px * px + py * py + pz * pz);
+  const double floor = 1.0e-6 * fmax(fabs(p.uMid), 1.0);
+  if (count < 5 && zeroLimit && !(p.uTau > 0.0)) out.beta = p.beta;
+  else out.beta = speed > floor ? p.uTau * p.uTau / speed : p.beta;
+  if (!isfinite(out.beta) || !(out.beta > 0.0)) return out;
+  out.uTau = p.uTau;
+  out.yPlus = p.yPlus;
+  out.valid = isfinite(p.fitRms) && isfinite(p.yPlus);
+  return out;
+}
+
 // ---------------------------------------------------------------- omega wall target
 struct OmegaWall {
   double omegaTarget, omegaVis, omegaLog, lamFrac, rey, yPlusK, pkAuto, nutSpalding;

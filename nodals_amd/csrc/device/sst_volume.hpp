@@ -90,6 +90,11 @@ __device__ inline void p1bf3Basis(const double l[4], double phi[8], double gr[8]
 // may be mat itself: every lane reads and writes only its own entry).
 // equation 2 is the momentum form: the eddy-viscosity diffusion block
 // sum_q w_q det nu_t grad phi_a . grad phi_b only (no reaction, no rhs).
+// With dmat (momentum form only) every cell also writes the deviatoric
+// cross-component tensor of the implicit deviatoric stress
+// (modules/coupling/implicit_dev_volume_tet.inc:1-68):
+//   dmat[c][ab][3i+j] = sum_q w_q det (nu + nu_t)
+//                       (d_j phi_a d_i phi_b - (2/3) d_i phi_a d_j phi_b).
 static __global__ void __launch_bounds__(256) k_sst_volume(
     int nc, const int* __restrict__ cref, const double* __restrict__ cinvJ, const double* __restrict__ cdet,
     const double* __restrict__ cxy, const double* __restrict__ u, const double* __restrict__ ufix,
@@ -97,8 +102,8 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
     const double* __restrict__ wfix, const double* __restrict__ quad, double radius, double nu, int equation,
     double kFloor, double omFloor, int deviatoric, double* mat, double* __restrict__ rhs,
     const int* __restrict__ sref = nullptr, double nuBase = 0.0, const double* conv = nullptr,
-    const double* __restrict__ vdist = nullptr) {
-  __shared__ double PH[4][kSstQ][8], GP[4][kSstQ][8][3], DW[4][kSstQ], RW[4][kSstQ], SW[4][kSstQ];
+    const double* __restrict__ vdist = nullptr, double* __restrict__ dmat = nullptr) {
+  __shared__ double PH[4][kSstQ][8], GP[4][kSstQ][8][3], DW[4][kSstQ], RW[4][kSstQ], SW[4][kSstQ], NW[4][kSstQ];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + wv;
   const bool live = c < nc;
@@ -156,6 +161,7 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
       DW[wv][q] = wt * s.nut;
       RW[wv][q] = 0.0;
       SW[wv][q] = 0.0;
+      NW[wv][q] = (nu + s.nut) * wt;
     } else {
       DW[wv][q] = wt * ((isK ? s.sigmaK : s.sigmaW) * s.nut + nuBase);
       RW[wv][q] = wt * (isK ? s.reactK : s.reactOmega);
@@ -173,6 +179,16 @@ static __global__ void __launch_bounds__(256) k_sst_volume(
       z += DW[wv][q] * gd + RW[wv][q] * PH[wv][q][a] * PH[wv][q][b];
     }
     mat[(long long)c * 64 + lane] = conv ? z + conv[(long long)c * 64 + lane] : z;
+    if (dmat) {
+      double* Dm = dmat + ((long long)c * 64 + lane) * 9;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double v = 0.0;
+          for (int q = 0; q < kSstQ; ++q)
+            v += NW[wv][q] * (GP[wv][q][a][j] * GP[wv][q][b][i] - (2.0 / 3.0) * GP[wv][q][a][i] * GP[wv][q][b][j]);
+          Dm[3 * i + j] = v;
+        }
+    }
   }
   if (rhs && lane < 8) {
     double z = 0.0;

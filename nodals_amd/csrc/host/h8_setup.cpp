@@ -419,11 +419,19 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
   // Dirichlet mask: every non-outlet boundary face and its vertices
   // (g3_host.hpp:29-55).
   S.fixed.assign((std::size_t)np + nf, 0);
+  S.wallFunction = S.duct && prm.wallFunction;
   for (int f = ni; f < nf; ++f) {
     if (M.facePatch[(std::size_t)f] == S.pipe.outlet) continue;
+    // Wall-function walls are not Dirichlet: their vertex and face DOFs stay
+    // free and carry the weak Nitsche/Spalding wall terms.
+    if (S.wallFunction && M.facePatch[(std::size_t)f] == S.pipe.wall) continue;
     S.fixed[(std::size_t)np + f] = 1;
     for (auto v : M.faces[(std::size_t)f]) S.fixed[(std::size_t)v] = 1;
   }
+  if (S.wallFunction)  // the wall keeps the vertices it shares with the inlet
+    for (int f = ni; f < nf; ++f)
+      if (M.facePatch[(std::size_t)f] == S.pipe.wall)
+        for (auto v : M.faces[(std::size_t)f]) S.fixed[(std::size_t)v] = 0;
   S.g2free.assign(S.fixed.size(), -1);
   std::int32_t next = 0;
   for (int v = 0; v < np; ++v) {
@@ -479,8 +487,11 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
       const auto& F = M.faces[(std::size_t)f];
       for (int d = 0; d < 3; ++d) {
         double vm = 0;
-        for (int j = 0; j < 3; ++j)
-          vm += S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)F[(std::size_t)j]]][(std::size_t)d] / 3.0;
+        for (int j = 0; j < 3; ++j) {
+          // a free (wall-function wall) vertex is taken at the plug value
+          const int sl = S.fixedSlot[(std::size_t)F[(std::size_t)j]];
+          vm += (sl >= 0 ? S.fixedValue[(std::size_t)sl][(std::size_t)d] : S.pipe.inletVelocity[d]) / 3.0;
+        }
         S.fixedValue[(std::size_t)S.fixedSlot[(std::size_t)np + f]][(std::size_t)d] =
             (20.0 / 9.0) * (S.pipe.inletVelocity[d] - vm);
       }

@@ -85,6 +85,9 @@ struct H8Params {
   // face-area-weighted pressures next to the inlet and the outlet, and a
   // geometric wall-distance field on the vertices (for the SST terms).
   bool duct = false;
+  // Duct mode only: the wall patch is a wall-function wall (free velocity
+  // DOFs with a Nitsche/Spalding wall closure) instead of no-slip.
+  bool wallFunction = false;
   double re = 20.0, bulk = 1.0;
   double alphaU = 0.5;
   double rauScale = 1.0;
@@ -95,6 +98,7 @@ struct H8Setup {
   PipeInfo pipe;
   bool cavity = false;
   bool duct = false;
+  bool wallFunction = false;
   int pinCell = -1;  // pressure row pinned (cavity), -1 for none
   // duct mode: cells next to the inlet / outlet with their face areas, and
   // the distance of every vertex to the nearest wall face

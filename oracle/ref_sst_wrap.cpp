@@ -8,6 +8,7 @@
 // against the reference itself.
 #include "physics/sst2003m_distance_core.hpp"
 #include "physics/sst2003m_host.hpp"
+#include "wall/portable/spalding_active.hpp"
 #include "wall/portable/spalding_core.hpp"
 #include "wall/portable/wall_host.hpp"
 
@@ -89,6 +90,51 @@ void ref_of_auto(long long n, const double* in, const int* shared, double* out) 
     const nodals::wall::HostOFState s = nodals::wall::evaluate_of_auto_host(q);
     const double v[8] = {s.omegaTarget, s.omegaVis, s.omegaLog, s.lamFrac, s.rey, s.yPlusK, s.pkAuto, s.nutSpalding};
     for (int j = 0; j < 8; ++j) out[8 * i + j] = v[j];
+  }
+}
+
+// Tangential wall fit of an active-point set (vector3::fit_active,
+// modules/wall/portable/spalding_active.hpp:129-154), stationary wall:
+// samples[n][5][3], y[n][5], nrm[n][3], pic[n][3]; out[n][3] = beta uTau
+// yPlus; valid[n].  The reference's host build returns the fit's own Robin
+// coefficient only when its device constant STAGE27_SPALDING_ZERO_LIMIT is
+// set; with zeroLimit the same selection (active::select_picard_beta with
+// its viscous-limit argument set) is applied here to the same fit_values
+// results, through the reference's own functions.
+void ref_fit_tangential(long long n, const double* samples, const double* y, const double* nrm, const double* pic,
+                        double nu, int count, int zeroLimit, double* out, int* valid) {
+  using namespace nodals::wall;
+  using vector3::V3;
+  for (long long i = 0; i < n; ++i) {
+    V3 s[5];
+    double yy[5];
+    for (int m = 0; m < 5; ++m) {
+      s[m] = V3{samples[15 * i + 3 * m], samples[15 * i + 3 * m + 1], samples[15 * i + 3 * m + 2]};
+      yy[m] = y[5 * i + m];
+    }
+    const V3 nv{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]}, pv{pic[3 * i], pic[3 * i + 1], pic[3 * i + 2]};
+    vector3::Result r = vector3::fit_active(s, yy, nv, pv, V3{0., 0., 0.}, nu, count);
+    if (zeroLimit && count < 5) {
+      double U[5] = {}, Y[5] = {};
+      const unsigned mask = active::active_slot_mask(count);
+      for (int m = 0; m < 5; ++m) {
+        Y[m] = yy[m];
+        if (mask & (1u << m)) U[m] = vector3::norm(vector3::tangent_projection(s[m], nv));
+      }
+      double ut = 0, yp = 0, betaFit = 0, uMid = 0, rms = 0, span = 0;
+      if (active::fit_values(U, U, Y, count, nu, ut, yp, betaFit, uMid, rms, span)) {
+        const double speed = vector3::norm(vector3::tangent_projection(pv, nv));
+        const double floor = 1.e-6 * std::fmax(std::fabs(uMid), 1.);
+        r.beta = active::select_picard_beta(ut, speed, floor, betaFit, count, true);
+        r.valid = std::isfinite(r.beta) && r.beta > 0. && std::isfinite(rms) && std::isfinite(yp);
+        r.uTau = ut;
+        r.yPlus = yp;
+      }
+    }
+    out[3 * i] = r.beta;
+    out[3 * i + 1] = r.uTau;
+    out[3 * i + 2] = r.yPlus;
+    valid[i] = r.valid ? 1 : 0;
   }
 }
 

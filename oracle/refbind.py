@@ -114,6 +114,10 @@ class RefCase:
 REF_SST_LIB = Path(__file__).resolve().parent / "_ref" / "libnodals_ref_sst.so"
 
 
+def port_available() -> bool:
+    return PORT_SST_LIB.exists()
+
+
 def sst_available() -> bool:
     return REF_SST_LIB.exists()
 
@@ -131,6 +135,7 @@ class RefSst:
         L.ref_spalding.argtypes = [C.c_longlong, vp, vp, C.c_double, vp, vp]
         L.ref_profile5.argtypes = [C.c_longlong, vp, vp, vp, C.c_double, vp, vp]
         L.ref_of_auto.argtypes = [C.c_longlong, vp, vp, vp]
+        L.ref_fit_tangential.argtypes = [C.c_longlong, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]
         self.L = L
 
     @staticmethod
@@ -172,6 +177,20 @@ class RefSst:
         self.L.ref_profile5(n, *(self._p(v) for v in a), nu, self._p(out), self._p(valid))
         return out, valid
 
+    def fit_tangential(self, samples, y, nrm, pic, nu, count, zero_limit=True):
+        """Tangential Spalding fit per face (vector3::fit_active): samples
+        (n,5,3), y (n,5), nrm and pic (n,3); out (n,3) = beta u_tau y+, valid."""
+        n = int(np.asarray(y).size // 5)
+        a = [np.ascontiguousarray(samples, dtype=np.float64).reshape(n, 5, 3),
+             np.ascontiguousarray(y, dtype=np.float64).reshape(n, 5),
+             np.ascontiguousarray(nrm, dtype=np.float64).reshape(n, 3),
+             np.ascontiguousarray(pic, dtype=np.float64).reshape(n, 3)]
+        out = np.empty((n, 3))
+        valid = np.empty(n, dtype=np.int32)
+        self.L.ref_fit_tangential(n, *(self._p(v) for v in a), nu, int(count), 1 if zero_limit else 0,
+                                  self._p(out), self._p(valid))
+        return out, valid
+
     def of_auto(self, inp, shared):
         n = int(np.asarray(shared).size)
         i = np.ascontiguousarray(inp, dtype=np.float64).reshape(n, 15)
@@ -201,6 +220,8 @@ class PortSst(RefSst):
         L.ref_spalding = L.port_spalding
         L.ref_profile5 = L.port_profile5
         L.ref_of_auto = L.port_of_auto
+        L.port_fit_tangential.argtypes = [C.c_longlong, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]
+        L.ref_fit_tangential = L.port_fit_tangential
 
     def sst_local(self, gu, k, omega, grad_k, grad_omega, nu, radius, dist=None, pipe_xy=None, k_floor=1e-8,
                   omega_floor=1e-3, deviatoric=False):

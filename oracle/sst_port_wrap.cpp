@@ -52,4 +52,22 @@ void port_of_auto(long long n, const double* in, const int* shared, double* out)
   }
 }
 
+void port_fit_tangential(long long n, const double* samples, const double* y, const double* nrm, const double* pic,
+                         double nu, int count, int zeroLimit, double* out, int* valid) {
+  for (long long i = 0; i < n; ++i) {
+    double sx[5], sy[5], sz[5];
+    for (int m = 0; m < 5; ++m) {
+      sx[m] = samples[15 * i + 3 * m];
+      sy[m] = samples[15 * i + 3 * m + 1];
+      sz[m] = samples[15 * i + 3 * m + 2];
+    }
+    const nodals::sst::VecFit f =
+        nodals::sst::fitTangential(sx, sy, sz, y + 5 * i, nrm + 3 * i, pic + 3 * i, nu, count, zeroLimit != 0);
+    out[3 * i] = f.beta;
+    out[3 * i + 1] = f.uTau;
+    out[3 * i + 2] = f.yPlus;
+    valid[i] = f.valid;
+  }
+}
+
 }  // extern "C"

@@ -20,8 +20,12 @@ from oracle import h8_oracle as O
 
 pytestmark = pytest.mark.gpu
 
-BASE = ["--max-outer", "10", "--alpha-u", "0.5", "--alpha-p", "0.5", "--p-rtol", "0.5", "--p-max", "20",
+# p-max 200: with p-max 20 several of these smoothers miss the 0.5 target in
+# the first iterations, and the solver then stops as the reference does
+# (h8_main_base.cu:2101), so the cap is raised for oracle and GPU alike.
+BASE = ["--max-outer", "10", "--alpha-u", "0.5", "--alpha-p", "0.5", "--p-rtol", "0.5", "--p-max", "200",
         "--verbose", "0"]
+P_MAX = 200
 
 
 def rel(a, b):
@@ -38,6 +42,7 @@ PRECOND_CASES = [
      _smoother("cheb2", cheb_degree=3, low_fraction=0.1)),
     ("sa", "l1jacobi", ["--amg-jacobi-omega", "0.8", "--amg-jacobi-fine-sweeps", "2"],
      _smoother("l1jacobi", jacobi_omega=0.8, jacobi_fine_sweeps=2)),
+    ("sa", "l1jacobi", [], _smoother("l1jacobi")),  # one sweep: the fused V-cycle kernels
     ("sa", "mcgs", [], _smoother("mcgs")),
     ("cf", "cheb2", ["--cf-interp", "exti", "--cf-pmax", "8"], _smoother("cheb2")),
     ("cf", "mcgs", ["--cf-interp", "exti", "--cf-pmax", "8", "--amg-mcgs-order", "forward", "--amg-mcgs-omega", "0.9"],
@@ -73,8 +78,13 @@ FIXED10_CASES = [
     # (label, native flags, oracle simple() keywords)
     ("cheb2+pcg (reference binary defaults)", ["--amg-smoother", "cheb2", "--pressure-solver", "pcg"],
      dict(p_solver="pcg", smoother=_smoother("cheb2"))),
-    ("l1jacobi+richardson", ["--amg-smoother", "l1jacobi", "--pressure-solver", "richardson"],
-     dict(p_solver="richardson", smoother=_smoother("l1jacobi"))),
+    # L1-Jacobi inside FGMRES: inside PCG (17-29 iterations per solve here)
+    # the loop amplifies rounding so much that a 1e-13 perturbation of the
+    # start state moves the oracle's own ten-iteration fields by 6e-6, which
+    # no cross-device comparison can resolve; inside FGMRES the same
+    # perturbation moves them by 4e-10.
+    ("l1jacobi+fgmres", ["--amg-smoother", "l1jacobi", "--pressure-solver", "fgmres"],
+     dict(p_solver="fgmres", smoother=_smoother("l1jacobi"))),
     ("jacobi+outer cheb", ["--amg-smoother", "jacobi", "--pressure-solver", "cheb"],
      dict(p_solver="cheb", smoother=_smoother("jacobi"))),
     ("cheb2+outer cheb degree 4", ["--amg-smoother", "cheb2", "--pressure-solver", "cheb", "--pressure-cheb-degree", "4",
@@ -97,7 +107,7 @@ def test_simple_fixed10_options_match_oracle(mesh_dir, label, flags, kw):
     case = O.case_from_product(h)
     r = h.run()
     assert r["pass_"] == 1 and r["outer"] == 10
-    ref = O.simple(case, max_outer=10, **kw)
+    ref = O.simple(case, max_outer=10, p_max=P_MAX, **kw)
     for d in range(3):
         assert rel(h.get(f"u{d}", case.nv), ref["U"][d]) < 1e-9, (label, d)
     assert rel(h.get("p", case.nc), ref["p"]) < 1e-9, label
@@ -133,7 +143,7 @@ def test_simple_fixed10_benchmark_sizes_match_oracle(bench_meshes, which):
     case = O.case_from_product(h)
     r = h.run()
     assert r["pass_"] == 1 and r["outer"] == 10
-    ref = O.simple(case, max_outer=10, **kw)
+    ref = O.simple(case, max_outer=10, p_max=P_MAX, **kw)
     errs = {f"u{d}": rel(h.get(f"u{d}", case.nv), ref["U"][d]) for d in range(3)}
     errs["p"] = rel(h.get("p", case.nc), ref["p"])
     print(f"[gpu] {which} fixed10 relative L2 vs oracle: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()),
