@@ -339,18 +339,13 @@ def config5_context(mesh: str, precision: str, max_outer: int, tol: float):
 
 
 def config5_reset(h) -> None:
-    """Start state: the inlet plug on every free vertex (zero face bubbles),
-    zero pressure, k and omega at their inlet values on the vertices."""
+    """Start state: the potential-flow start of the wall path (computed once
+    at wall init), zero pressure, k and omega at their inlet values on the
+    vertices."""
     nv, nc = h.info["nv"], h.info["cells"]
-    g2 = h.get("g2free", h.get("mesh_sizes", 4, np.int64)[0] + h.get("mesh_sizes", 4, np.int64)[1], np.int32)
-    npts = int(h.get("mesh_sizes", 4, np.int64)[0])
-    vin = h.get("pipe_geometry", 12)[6:9]
-    vert = g2[:npts]
-    vert = vert[vert >= 0]
+    u0 = h.get("wall_u0", 3 * nv).reshape(nv, 3)
     for d in range(3):
-        u = np.zeros(nv)
-        u[vert] = vin[d]
-        h.set(f"u{d}", u)
+        h.set(f"u{d}", np.ascontiguousarray(u0[:, d]))
     h.set("p", np.zeros(nc))
     ns, nvert = h.turb["ns"], h.turb["nvert"]
     for name, v in (("k", h.turb["k_in"]), ("omega", h.turb["omega_in"])):
@@ -364,7 +359,7 @@ def config5_leg(steps: int, warmup: int, max_outer: int, tol: float) -> dict:
     two turns; 576-quad O-grid section x 60 slabs, 6 tets per hex: 207,360
     tets), Re = 50,000, uniform inlet with 5% intensity and omega from a
     0.07 D length scale, Spalding wall functions with the sample at y+ 30,
-    from the plug to residual 1e-5; plain FP64 and FP32 AMG inside the FP64
+    from the potential-flow start to residual 1e-5; plain FP64 and FP32 AMG inside the FP64
     Krylov solve."""
     from nodals_amd import meshgen
 

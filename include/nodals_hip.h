@@ -230,14 +230,20 @@ int nodals_h8_turb_couple(nodals_h8* h, int32_t on);
  * --wall-active-points N (SST24B_ACTIVE_POINTS), --wall-zero-limit 0|1,
  * --convection-form skew_symmetric|advective, --stress-form
  * deviatoric|vector_laplacian, --m3-mode fgs|sgs, --m3-sweeps N,
- * --m3-omega W.  After it nodals_h8_run runs the wall-function SIMPLE loop
+ * --m3-omega W, --initial-velocity potential|plug (default potential: the
+ * potential-flow start of modules/initialization/
+ * stage27_potential_host.hpp:44-167), --potential-tol T (its Laplace
+ * solve, default 1e-10).  After it nodals_h8_run runs the wall-function SIMPLE loop
  * and nodals_h8_pressure_solve uses the constrained Schur operator.
  * nodals_h8_get names: "wall_K" [nnz][9] (reference entry order),
  * "wall_rhs", "wall_rau3", "wall_r3", "wall_delta3" [nv][3], "wall_fit"
  * [nf][3] (beta, u_tau, y+), "wall_shear" [nf][4] (integrated traction
  * x, y, z and |traction|), "wall_nuq" [nf][12], and the wall topology
  * ("wall_normal", "wall_area", "wall_h", "wall_bw", "wall_bp", "wall_qw",
- * "wall_ql", "wall_cell", "wall_lf", "wall_bub", "wall_vert"). */
+ * "wall_ql", "wall_cell", "wall_lf", "wall_bub", "wall_vert"), the initial
+ * velocity "wall_u0" [nv][3] and "wall_start" [7] (Laplace iterations,
+ * final relative residual, inlet flux, outlet flux, relative imbalance,
+ * largest wall face-mean normal velocity, largest vertex speed). */
 int nodals_h8_wall_init(nodals_h8* h, int argc, const char* const* argv);
 /* Face-averaged Spalding fit (f2_face_averaged_spalding,
  * modules/wall/stage22d4b_nitsche_live.inc:25), Nitsche face blocks

@@ -49,12 +49,17 @@ WallParams parseWallArgs(int argc, const char* const* argv) {
       P.m3Symmetric = sv == "sgs";
     } else if (f == "--m3-sweeps") P.m3Sweeps = (int)num(f, v);
     else if (f == "--m3-omega") P.m3Omega = num(f, v);
+    else if (f == "--initial-velocity") {
+      require(sv == "potential" || sv == "plug", "--initial-velocity must be potential or plug");
+      P.potential = sv == "potential";
+    } else if (f == "--potential-tol") P.potentialTol = num(f, v);
     else fail("unknown wall option " + f);
   }
   require(P.gammaN >= 0.0, "--gamma must be >= 0");
   require(P.activePoints >= 1 && P.activePoints <= 5, "--wall-active-points must be in 1..5");
   require(P.m3Sweeps >= 1 && P.m3Sweeps <= 60, "--m3-sweeps must be in 1..60");
   require(P.m3Omega > 0.0 && P.m3Omega < 2.0, "--m3-omega must be in (0, 2)");
+  require(P.potentialTol > 0.0 && P.potentialTol < 1.0, "--potential-tol must be in (0, 1)");
   return P;
 }
 
@@ -66,6 +71,31 @@ void H8Solver::wallInit(const WallParams& P) {
   auto st = std::make_unique<WallState>();
   st->P = P;
   st->W = buildWallTopo(M_, S_, turb_->T);
+  // Initial velocity of the free DOFs: the potential-flow start or the inlet
+  // plug (the inlet velocity on every free vertex, zero on the face bubbles:
+  // plug_initial, hxt5b_boundary.cuh:698-700).  Fixed DOFs keep their
+  // Dirichlet values.
+  st->u0.assign(3 * (std::size_t)nv_, 0.0);
+  st->startReport.assign(7, 0.0);
+  {
+    const int np = M_.nPoints(), nent = np + M_.nFaces();
+    if (P.potential) {
+      const PotentialStart ps = potentialStart(M_, S_, P.potentialTol);
+      for (int e = 0; e < nent; ++e) {
+        const int g = S_.g2free[(std::size_t)e];
+        if (g < 0) continue;
+        for (int d = 0; d < 3; ++d) st->u0[3 * (std::size_t)g + d] = ps.value[3 * (std::size_t)e + d];
+      }
+      st->startReport = {(double)ps.iterations, ps.rel, ps.inletFlux, ps.outletFlux, ps.imbalanceRel, ps.wallFluxMax,
+                         ps.maxSpeed};
+    } else {
+      for (int v = 0; v < np; ++v) {
+        const int g = S_.g2free[(std::size_t)v];
+        if (g < 0) continue;
+        for (int d = 0; d < 3; ++d) st->u0[3 * (std::size_t)g + d] = S_.pipe.inletVelocity[d];
+      }
+    }
+  }
   if (!hasDevice()) {
     wall_ = std::move(st);
     return;
@@ -174,16 +204,10 @@ void H8Solver::wallInit(const WallParams& P) {
     momT_ = nullptr;
   }
   rau3_ = w.rau3.data();
-  // start from the inlet plug: the inlet velocity on every free vertex, zero
-  // on the face bubbles (plug_initial, hxt5b_boundary.cuh:698-700)
   {
     std::vector<double> u0((std::size_t)nv_ * 3, 0.0);
-    const int np = M_.nPoints();
-    for (int v = 0; v < np; ++v) {
-      const int g = S_.g2free[(std::size_t)v];
-      if (g < 0) continue;
-      for (int d = 0; d < 3; ++d) u0[3 * (std::size_t)dv(g) + d] = S_.pipe.inletVelocity[d];
-    }
+    for (int g = 0; g < nv_; ++g)
+      for (int d = 0; d < 3; ++d) u0[3 * (std::size_t)dv(g) + d] = w.u0[3 * (std::size_t)g + d];
     u_.upload(u0);
   }
   wall_ = std::move(st);
@@ -395,6 +419,8 @@ std::vector<double> H8Solver::wallField(const std::string& n) {
   if (n == "wall_lf") return std::vector<double>(W.lf.begin(), W.lf.end());
   if (n == "wall_bub") return std::vector<double>(W.bub.begin(), W.bub.end());
   if (n == "wall_vert") return std::vector<double>(W.vert.begin(), W.vert.end());
+  if (n == "wall_u0") return w.u0;
+  if (n == "wall_start") return w.startReport;
   requireDevice();
   sync();
   auto perDof = [&](const DBuf<double>& b) {  // device [nv][3] -> reference free order

@@ -2,6 +2,7 @@
 // k-omega SST path with Nitsche/Spalding walls; see wall_kernels.hpp).
 #pragma once
 
+#include "../host/potential_start.hpp"
 #include "../host/wall_setup.hpp"
 #include "common.hpp"
 
@@ -14,12 +15,17 @@ namespace nodals {
 //   --convection-form skew_symmetric|advective
 //   --stress-form deviatoric|vector_laplacian
 //   --m3-mode fgs|sgs  --m3-sweeps N  --m3-omega W
+//   --initial-velocity potential|plug  --potential-tol T
 // Unknown flags raise kErrInvalid.
 WallParams parseWallArgs(int argc, const char* const* argv);
 
 struct WallState {
   WallParams P;
   WallTopo W;
+  // initial velocity [nv][3] in reference free order, and the potential
+  // solve's report (iterations, rel, inlet flux, outlet flux, imbalance,
+  // largest wall face-mean flux, largest vertex speed; zeros for the plug)
+  std::vector<double> u0, startReport;
   // topology (device numbering)
   DBuf<int> wcell, wlf, bub, vert, dof, dofOff, dofFace, cellWallOff, cellWall, blkOff, blkIdx, bubFace, wallDof;
   DBuf<unsigned char> isBub;

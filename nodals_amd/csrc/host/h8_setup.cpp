@@ -690,6 +690,30 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
     for (int i = 0; i < 4; ++i) z += M.points[(std::size_t)M.tets[(std::size_t)c][(std::size_t)i]][2] * .25;
     S.cellZ[(std::size_t)c] = z;
   }
+  // Wall-function walls: B_N = B - int_wall (n . u), the divergence without
+  // the wall's normal flux (s22d3_patch_normal_wall_divergence,
+  // turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/modules/wall/
+  // stage22d3_wall_b_host.inc:4-20, applied to the production B at
+  // sst_application.inc:802-808).  On a wall face the three vertex traces
+  // integrate to area/3 and the face's own bubble to 27/60 area (the
+  // reference's 12-point rule is exact for both); every other basis has zero
+  // trace there.  Wall DOFs are free, so the fixed divergence is unchanged.
+  if (S.wallFunction) {
+    const Patch& wp = M.patches[(std::size_t)S.pipe.wall];
+    for (int f = wp.start; f < wp.start + wp.count; ++f) {
+      const int c = M.owner[(std::size_t)f];
+      int lf = -1;
+      for (int i = 0; i < 4; ++i)
+        if (M.oppFace[(std::size_t)c][(std::size_t)i] == f) lf = i;
+      if (lf < 0) fail("wall face " + std::to_string(f) + " is not a face of its owner cell");
+      const V3 sf = outwardAreaVector(M, f);
+      for (int a = 0; a < 8; ++a) {
+        const double w = a < 4 ? (a == lf ? 0.0 : 1.0 / 3.0) : (a == 4 + lf ? 27.0 / 60.0 : 0.0);
+        if (w == 0.0) continue;
+        for (int d = 0; d < 3; ++d) S.bcoef[(std::size_t)c * 24 + (std::size_t)(3 * a + d)] -= w * sf[(std::size_t)d];
+      }
+    }
+  }
   S.zMin = 1e300;
   S.zMax = -1e300;
   for (const auto& x : M.points) {

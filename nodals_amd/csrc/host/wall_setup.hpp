@@ -34,6 +34,11 @@ struct WallParams {
   bool m3Symmetric = false; // forward (fgs) or symmetric (sgs) block sweeps (NODALS_M3_MODE)
   int m3Sweeps = 20;        // NODALS_M3_SWEEPS
   double m3Omega = 0.7;     // NODALS_M3_OMEGA
+  // Initial velocity: the potential-flow start (default, the case's
+  // [initialization] mode = potential_flow) or the inlet plug
+  // (STAGE27_INITIAL_VELOCITY), and the Laplace solve's tolerance.
+  bool potential = true;
+  double potentialTol = 1e-10;
 };
 
 struct WallTopo {

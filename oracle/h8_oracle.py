@@ -435,8 +435,20 @@ def greedy_colors(A: sp.csr_matrix) -> np.ndarray:
     return color
 
 
+def mcgs_plan(A: sp.csr_matrix, colors: np.ndarray):
+    """Per-colour (rows, off-diagonal rows of A) of the multicolour sweep."""
+    off = A.tocsr(copy=True)
+    off.setdiag(0.0)
+    ncol = int(colors.max()) + 1
+    out = []
+    for c in range(ncol):
+        rows = np.flatnonzero(colors == c)
+        out.append((rows, off[rows]))
+    return out
+
+
 def _smooth(A: sp.csr_matrix, diag, l1, b, opt: Smoother, lam: float, jac_sweeps: int, mcgs_sweeps: int,
-            colors=None) -> np.ndarray:
+            colors=None, plan=None) -> np.ndarray:
     """Smoother from x = 0 (G4AMG::jacobi_*/cheb_*/mcgs_*, h8_main_base.cu:429-519)."""
     if opt.kind in ("jacobi", "l1jacobi"):
         den = l1 if opt.kind == "l1jacobi" else diag
@@ -454,15 +466,13 @@ def _smooth(A: sp.csr_matrix, diag, l1, b, opt: Smoother, lam: float, jac_sweeps
         return x
     if opt.kind == "mcgs":
         x = np.zeros_like(b)
-        off = A.copy()
-        off.setdiag(0.0)
-        ncol = int(colors.max()) + 1
+        if plan is None:
+            plan = mcgs_plan(A, colors)
         passes = {"forward": [False], "backward": [True], "symmetric": [False, True]}[opt.mcgs_order]
         for _ in range(mcgs_sweeps):
             for rev in passes:
-                for c in (range(ncol - 1, -1, -1) if rev else range(ncol)):
-                    rows = np.flatnonzero(colors == c)
-                    gs = (b[rows] - off[rows] @ x) / diag[rows]
+                for rows, off in (plan[::-1] if rev else plan):
+                    gs = (b[rows] - off @ x) / diag[rows]
                     x[rows] = x[rows] + opt.mcgs_omega * (gs - x[rows])
         return x
     raise ValueError(opt.kind)
@@ -477,14 +487,17 @@ class VCycle:
         self.fine = None
         self.coarse_colors = []
         self.l1 = [np.asarray(abs(A).sum(axis=1)).ravel() for A in hier["A"]]
+        self.coarse_plans = []
         if opt.kind == "mcgs":
             self.coarse_colors = [greedy_colors(A) for A in hier["A"][:-1]]
+            self.coarse_plans = [mcgs_plan(A, c) for A, c in zip(hier["A"][:-1], self.coarse_colors)]
 
     def set_fine(self, A: sp.csr_matrix, diag: np.ndarray):
         self.fine = A
         self.fdiag = diag
         self.fl1 = np.asarray(abs(A).sum(axis=1)).ravel()
         self.fcolors = greedy_colors(A) if self.opt.kind == "mcgs" else None
+        self.fplan = mcgs_plan(A, self.fcolors) if self.opt.kind == "mcgs" else None
 
     def _level(self, l, b):
         H, o = self.H, self.opt
@@ -494,24 +507,27 @@ class VCycle:
         diag = A.diagonal()
         lam = float(H["level_lambda"][l]) if o.kind == "cheb2" else 1.0
         cols = self.coarse_colors[l] if o.kind == "mcgs" else None
-        x = _smooth(A, diag, self.l1[l], b, o, lam, o.jacobi_coarse_sweeps, o.mcgs_coarse_sweeps, cols)
+        plan = self.coarse_plans[l] if o.kind == "mcgs" else None
+        x = _smooth(A, diag, self.l1[l], b, o, lam, o.jacobi_coarse_sweeps, o.mcgs_coarse_sweeps, cols, plan)
         r = b - A @ x
         P = H["P"][l + 1]
         xc = self._level(l + 1, P.T @ r)
         x = x + P @ xc
         r = b - A @ x
-        x = x + _smooth(A, diag, self.l1[l], r, o, lam, o.jacobi_coarse_sweeps, o.mcgs_coarse_sweeps, cols)
+        x = x + _smooth(A, diag, self.l1[l], r, o, lam, o.jacobi_coarse_sweeps, o.mcgs_coarse_sweeps, cols, plan)
         return x
 
     def apply(self, b: np.ndarray) -> np.ndarray:
         A, o = self.fine, self.opt
         lam = self.H["fine_lambda"]
-        z = _smooth(A, self.fdiag, self.fl1, b, o, lam, o.jacobi_fine_sweeps, o.mcgs_fine_sweeps, self.fcolors)
+        z = _smooth(A, self.fdiag, self.fl1, b, o, lam, o.jacobi_fine_sweeps, o.mcgs_fine_sweeps, self.fcolors,
+                    self.fplan)
         r = b - A @ z
         P = self.H["P"][0]
         z = z + P @ self._level(0, P.T @ r)
         r = b - A @ z
-        return z + _smooth(A, self.fdiag, self.fl1, r, o, lam, o.jacobi_fine_sweeps, o.mcgs_fine_sweeps, self.fcolors)
+        return z + _smooth(A, self.fdiag, self.fl1, r, o, lam, o.jacobi_fine_sweeps, o.mcgs_fine_sweeps, self.fcolors,
+                           self.fplan)
 
 
 def pcg(A, M, b, rtol, atol, maxit):

@@ -98,8 +98,8 @@ def volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius
     nc = Uc.shape[0]
     Lq, Wq = keast24()
     phi, gr = basis(Lq)                                   # (24,8), (24,8,3)
-    gp = np.einsum("qaj,cjd->cqad", gr, cell_invJ)                    # (nc,24,8,3)
-    gu = np.einsum("cai,cqad->cqid", Uc, gp).reshape(nc, 24, 9)       # du_i/dx_d row-major
+    gp = np.matmul(gr[None], cell_invJ[:, None])                      # (nc,24,8,3): sum_j gr[q,a,j] invJ[c,j,d]
+    gu = np.matmul(Uc.transpose(0, 2, 1)[:, None], gp).reshape(nc, 24, 9)  # du_i/dx_d row-major
     kv = Kc @ phi.T                                                   # (nc,24)
     wv = Wc @ phi.T
     gk = np.einsum("ca,cqad->cqd", Kc, gp)
@@ -122,6 +122,8 @@ def volume_terms_cells(ref_sst, Uc, Kc, Wc, cell_invJ, cell_det, cell_xy, radius
         D, R, S = wt * out[..., 3] * out[..., 0], wt * out[..., 10], wt * out[..., 8]
     else:
         D, R, S = wt * out[..., 4] * out[..., 0], wt * out[..., 11], wt * out[..., 9]
-    mat = np.einsum("cq,cqad,cqbd->cab", D, gp, gp) + np.einsum("cq,qa,qb->cab", R, phi, phi)
+    G = gp.transpose(0, 1, 3, 2).reshape(nc, 24 * 3, 8)                # [c, (q, d), a]
+    DG = (np.repeat(D, 3, axis=1))[:, :, None] * G
+    mat = np.matmul(DG.transpose(0, 2, 1), G) + np.einsum("cq,qa,qb->cab", R, phi, phi, optimize=True)
     rhs = np.einsum("cq,qa->ca", S, phi)
     return mat, rhs, D

@@ -133,12 +133,82 @@ def b_matrix(case: H.H8Case) -> sp.csr_matrix:
                          shape=(case.nc, 3 * case.nv))
 
 
+def potential_start(m: TO.Mesh, pipe_geometry, g2free, nv: int, tol=None):
+    """Potential-flow start (stage27_potential_host.hpp:44-167): the P1
+    Laplace problem  int grad(phi).grad(v) = int_inlet v (U_in . n)  with
+    phi = 0 on the outlet vertices and Neumann walls (solved here directly,
+    scipy's sparse LU, where the reference runs Jacobi PCG to 1e-10), the
+    mass-lumped projection of the cellwise grad(phi) onto the vertices, and
+    one normal BF3 bubble per boundary face that sets its mean normal
+    velocity: U_in . n on the inlet, zero on the wall, and on the outlet the
+    projected value plus one uniform correction that balances the inlet
+    flux.  Returns (U (3, nv) in free numbering -- fixed DOFs left out --,
+    every entity's value (n_entities, 3), report dict)."""
+    from scipy.sparse.linalg import spsolve
+
+    pts, tets = m.points, m.tets
+    npt = pts.shape[0]
+    uin = np.asarray(pipe_geometry[6:9], dtype=np.float64)
+    inlet, wall, outlet = (int(pipe_geometry[9]), int(pipe_geometry[10]), int(pipe_geometry[11]))
+    P = pts[tets]                                                       # (nc,4,3)
+    J = np.stack([P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]], axis=2)   # columns
+    det = np.linalg.det(J)
+    vol = np.abs(det) / 6.0
+    G = np.zeros((tets.shape[0], 4, 3))
+    G[:, 1:] = np.linalg.inv(J)                                         # rows: grad of lambda_1..3
+    G[:, 0] = -G[:, 1:].sum(1)
+    Kl = np.einsum("c,cai,cbi->cab", vol, G, G)
+    rows = np.repeat(tets, 4, axis=1).ravel()
+    cols = np.tile(tets, (1, 4)).ravel()
+    K = sp.csr_matrix((Kl.ravel(), (rows, cols)), shape=(npt, npt))
+    rhs = np.zeros(npt)
+    bfaces = np.flatnonzero(m.face_patch >= 0)
+    plans = {int(f): TO.face_plan(m, int(f)) for f in bfaces}
+    outlet_v = np.zeros(npt, dtype=bool)
+    for f in bfaces:
+        if m.face_patch[f] == outlet:
+            outlet_v[m.faces[f]] = True
+        elif m.face_patch[f] == inlet:
+            _, _, n, area = plans[int(f)]
+            rhs[m.faces[f]] += float(uin @ n) * area / 3.0
+    free = ~outlet_v
+    phi = np.zeros(npt)
+    phi[free] = spsolve(K[free][:, free].tocsc(), rhs[free])
+    gphi = np.einsum("ca,cai->ci", phi[tets], G)
+    mass = np.bincount(tets.ravel(), weights=np.repeat(vol / 4.0, 4), minlength=npt)
+    u = np.zeros((npt, 3))
+    for d in range(3):
+        u[:, d] = np.bincount(tets.ravel(), weights=np.repeat(vol / 4.0 * gphi[:, d], 4), minlength=npt) / mass
+    ent = np.zeros((m.ns, 3))
+    ent[:npt] = u
+    in_flux = sum(plans[int(f)][3] * float(uin @ plans[int(f)][2]) for f in bfaces if m.face_patch[f] == inlet)
+    out_f = [int(f) for f in bfaces if m.face_patch[f] == outlet]
+    out_area = sum(plans[f][3] for f in out_f)
+    out_proj = sum(plans[f][3] * float(u[m.faces[f]].mean(0) @ plans[f][2]) for f in out_f)
+    corr_out = (-in_flux - out_proj) / out_area
+    for f in bfaces:
+        _, _, n, area = plans[int(f)]
+        before = float(u[m.faces[f]].mean(0) @ n)
+        kind = m.face_patch[f]
+        goal = float(uin @ n) if kind == inlet else (before + corr_out if kind == outlet else 0.0)
+        ent[npt + f] = (goal - before) / 0.45 * n
+    U = np.zeros((3, nv))
+    g2 = np.asarray(g2free)
+    e = np.flatnonzero(g2 >= 0)
+    U[:, g2[e]] = ent[e].T
+    return U, ent, dict(phi=phi, inlet_flux=in_flux, outlet_area=out_area, outlet_projected=out_proj)
+
+
 class WallPath:
     """Assembly, momentum sweeps, Schur operator and SIMPLE loop."""
 
     def __init__(self, case: H.H8Case, step: TO.Step, ref_sst, gamma_n=1e-6, active=1, zero_limit=True, skew=True,
-                 deviatoric=True, m3_sweeps=20, m3_omega=0.7, symmetric=False, rau_scale=2.0, alpha_u=0.7):
+                 deviatoric=True, m3_sweeps=20, m3_omega=0.7, symmetric=False, rau_scale=2.0, alpha_u=0.7,
+                 fitter=None):
+        """fitter: object with the fit_tangential method of
+        refbind.RefSst (the default, the reference's compiled fit)."""
         self.case, self.step, self.ref = case, step, ref_sst
+        self.fitter = fitter if fitter is not None else ref_sst
         self.W = Wall(step, case)
         self.gamma_n, self.active, self.zero_limit = gamma_n, active, zero_limit
         self.skew, self.deviatoric = skew, deviatoric
@@ -165,7 +235,7 @@ class WallPath:
             vq = np.einsum("fqb,fqbd->fqd", phi, Uc[cells])
             samp[:, m] = np.einsum("fq,fqd->fd", W.qw, vq) / W.qw.sum(1)[:, None]
             yy[:, m] = (W.qw * st.heights[m]).sum(1) / W.qw.sum(1)
-        out, valid = self.ref.fit_tangential(samp, yy, W.n, pic, st.nu, self.active, self.zero_limit)
+        out, valid = self.fitter.fit_tangential(samp, yy, W.n, pic, st.nu, self.active, self.zero_limit)
         return out, valid
 
     def wall_nue(self, U):
@@ -198,8 +268,11 @@ class WallPath:
         _, gr = S.basis(Lq)
         gp = np.einsum("qaj,cjd->cqad", gr, case.cell_invJ)
         NW = D + case.cell_det[:, None] * Wq[None, :] * st.nu
-        return (np.einsum("cq,cqaj,cqbi->cabij", NW, gp, gp)
-                - (2.0 / 3.0) * np.einsum("cq,cqai,cqbj->cabij", NW, gp, gp))
+        nc = gp.shape[0]
+        G = gp.reshape(nc, gp.shape[1], 24)
+        Mx = np.matmul((NW[:, :, None] * G).transpose(0, 2, 1), G).reshape(nc, 8, 3, 8, 3)  # [c, a, x, b, y]
+        # nu_eff (d_j phi_a d_i phi_b - 2/3 d_i phi_a d_j phi_b) at block (a, b), entry (i, j)
+        return Mx.transpose(0, 1, 3, 4, 2) - (2.0 / 3.0) * Mx.transpose(0, 1, 3, 2, 4)
 
     # -- assembly -------------------------------------------------------------
     def plug(self):
@@ -236,20 +309,15 @@ class WallPath:
         wv = np.zeros((nnz, 3))
         pv = np.zeros((nnz, 3))
         cross = np.zeros((nnz, 3, 3))
-        for f in range(W.nf):
-            c = W.cell[f]
-            for a in range(8):
-                if ref[c, a] < 0:
-                    continue
-                for b in range(8):
-                    if ref[c, b] < 0:
-                        continue
-                    k = case.mom_row[ref[c, a]] + slot[c, a, b]
-                    wv[k] += blk[f, a, b]
-                    pv[k] += pen[f, a, b]
-                    nn = np.outer(W.n[f], W.n[f])
-                    np.fill_diagonal(nn, 0.0)
-                    cross[k] += nn * cpl[f, a, b]
+        rc = ref[W.cell]                                                 # (nf,8)
+        both = (rc[:, :, None] >= 0) & (rc[:, None, :] >= 0)              # (nf,8,8)
+        kpos = case.mom_row[np.where(rc >= 0, rc, 0)][:, :, None] + slot[W.cell]
+        nn = np.einsum("fi,fj->fij", W.n, W.n) * (1.0 - np.eye(3))[None]
+        fidx = np.broadcast_to(np.arange(W.nf)[:, None, None], both.shape)[both]
+        kk = kpos[both]                                                  # face-major, (a, b) row-major
+        np.add.at(wv, kk, blk[both])
+        np.add.at(pv, kk, pen[both])
+        np.add.at(cross, kk, nn[fidx] * cpl[both][:, None, None])
         vals = aval[:, None] + wv                                        # (nnz,3)
         fac = 1.0 / self.alpha_u - 1.0
         l1 = np.zeros((nv, 3))

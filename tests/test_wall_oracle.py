@@ -16,7 +16,10 @@ product's host wall setup, on a small helix (host setup only, no GPU):
 * the skew convection block satisfies C + C^T = boundary flux integral of
   (a.n) phi_a phi_b over the cell's faces;
 * many block Gauss-Seidel sweeps (omega 0.7) reach the direct solve of K on
-  the constrained space.
+  the constrained space;
+* the potential-flow start is the plug on a straight pipe (a linear
+  potential, exact in P1), and the product's host solve equals the oracle's
+  direct solve on the helix.
 """
 import numpy as np
 import pytest
@@ -40,6 +43,7 @@ def wcase(tmp_path_factory):
     mesh = meshgen.write_helix_case(tmp_path_factory.mktemp("wo") / "helix", core=4, ring=2, nslab=12)
     h = H8(["--mesh", str(mesh), "--duct", "1", "--wall-model", "spalding", "--re", "5000", "--max-outer", "10",
             "--verbose", "0"], device=False)
+    h.mesh_path = str(mesh)
     h.turb_init([])
     h.wall_init([])
     case = H.case_from_product(h)
@@ -222,3 +226,84 @@ def test_block_sweeps_reach_constrained_direct_solve(wcase):
     Ux = wp2.momentum(A2)
     x_direct_phys = (W.T @ x_direct).reshape(case.nv, 3).T
     assert max(np.linalg.norm(Ux[d] - x_direct_phys[d]) / np.linalg.norm(x_direct_phys[d]) for d in range(3)) < 1e-8
+
+
+def test_potential_start_straight_pipe_is_the_plug(tmp_path):
+    """On a straight pipe the potential is linear along the axis, which P1
+    represents exactly: the start is the plug on every free vertex, with
+    zero face bubbles (closed form).  The product's host solve and the
+    oracle's direct solve both give it; a reversed inlet flux gives the
+    negated plug, so the test sees the sign of the source."""
+    from nodals_amd import meshgen
+    from nodals_amd._lib import H8
+
+    mesh = meshgen.write_pipe_case(tmp_path / "pipe", core=4, ring=2, nslab=10)
+    h = H8(["--mesh", str(mesh), "--duct", "1", "--wall-model", "spalding", "--re", "5000", "--max-outer", "10",
+            "--verbose", "0"],
+           device=False)
+    h.turb_init([])
+    h.wall_init(["--potential-tol", "1e-13"])
+    case = H.case_from_product(h, with_hierarchy=False)
+    m = TO.mesh_from_product(h)
+    geo = h.get("pipe_geometry", 12)
+    g2 = h.get("g2free", m.ns, np.int32)
+    U, ent, rep = WO.potential_start(m, geo, g2, case.nv)
+    plug = np.asarray(geo[6:9])
+    npt = m.nvert
+    vfree = g2[:npt] >= 0
+    assert np.abs(ent[:npt] - plug[None, :]).max() < 1e-9
+    assert np.abs(ent[npt:]).max() < 1e-8
+    got = h.get("wall_u0", 3 * case.nv).reshape(case.nv, 3)
+    assert np.abs(got[g2[:npt][vfree]] - plug[None, :]).max() < 1e-9
+    start = h.get("wall_start", 7)
+    assert start[0] > 0 and start[1] < 1e-13 and start[4] < 1e-10 and start[5] < 1e-10
+    flipped = geo.copy()
+    flipped[6:9] = -flipped[6:9]
+    _, ent2, _ = WO.potential_start(m, flipped, g2, case.nv)
+    assert np.abs(ent2[:npt] + plug[None, :]).max() < 1e-9
+
+
+def test_potential_start_product_matches_oracle(wcase):
+    """The product's host potential start (Jacobi PCG to 1e-10) against the
+    oracle's direct solve on the helix; every wall face's mean normal
+    velocity is zero (T leaves it unchanged) and the outlet carries the
+    inlet flux."""
+    h, case, st, wp = wcase
+    m = st.m
+    U, ent, rep = WO.potential_start(m, h.get("pipe_geometry", 12), h.get("g2free", m.ns, np.int32), case.nv)
+    got = h.get("wall_u0", 3 * case.nv).reshape(case.nv, 3).T
+    assert np.linalg.norm(got - U) / np.linalg.norm(U) < 1e-7
+    assert np.linalg.norm(wp.constrain(U) - U) < 1e-12 * np.linalg.norm(U)
+    start = h.get("wall_start", 7)
+    assert start[1] < 1e-10 and start[4] < 1e-10 and start[5] < 1e-10
+    assert abs(start[2] - rep["inlet_flux"]) < 1e-12 * abs(rep["inlet_flux"])
+
+
+def test_wall_divergence_drops_the_wall_normal_flux(wcase):
+    """With wall-function walls the product's B is B_N = B - int_wall (n.u):
+    its coefficients equal the no-slip context's volume B on the same mesh
+    minus the oracle's 12-point quadrature of every wall face's traces times
+    the outward normal; the face's own bubble then drops out of B_N, and on
+    velocities that satisfy the wall-flux constraint B_N = B."""
+    from nodals_amd._lib import H8
+
+    h, case, st, wp = wcase
+    W = wp.W
+    h0 = H8(["--mesh", h.mesh_path, "--duct", "1", "--re", "5000", "--max-outer", "10", "--verbose", "0"], device=False)
+    nc = case.nc
+    b_vol = h0.get("bcoef", nc * 24).reshape(nc, 8, 3)
+    h0.close()
+    I = np.einsum("fq,fqa->fa", W.qw, W.bw)                             # (nf, 8): integral of each trace
+    patch = np.zeros_like(b_vol)
+    np.add.at(patch, W.cell, I[:, :, None] * W.n[:, None, :])
+    assert np.abs(case.bcoef - (b_vol - patch)).max() < 1e-14 * np.abs(b_vol).max()
+    assert np.abs(case.bcoef - (b_vol + patch)).max() > 1e-3 * np.abs(b_vol).max()     # a flipped sign shows
+    own_bubble = case.bcoef[W.cell, 4 + W.lf]
+    assert np.abs(own_bubble).max() < 1e-14 * np.abs(b_vol).max()
+    # on constrained velocities the two divergences agree
+    rng = np.random.default_rng(4)
+    x = wp.constrain(rng.standard_normal((3, case.nv)))
+    ug = H.cell_values(case, x)
+    div_n = np.einsum("cad,cad->c", case.bcoef, ug)
+    div_v = np.einsum("cad,cad->c", b_vol, ug)
+    assert np.abs(div_n - div_v).max() < 1e-13 * np.abs(div_v).max()
