@@ -346,6 +346,18 @@ int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, con
 int nodals_wall_fit_tangential(int64_t n, const double* samples, const double* y, const double* nrm,
                                const double* pic, double nu, int32_t count, int32_t zero_limit, double* out,
                                int32_t* valid);
+/* The five-point fit of nodals_wall_profile5 with the same arguments and
+ * results, allocating nothing: no hipMalloc, hipFree or allocating library
+ * call.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it).  Inputs and outputs are staged through a fixed
+ * device array of the library, 11.3 MB on each GPU it runs on, held until the
+ * process ends, so n is at most 65536 (larger n is rejected with an error
+ * before any device work).  Runs on the calling thread's current device and
+ * returns when the results are in out/valid.  Calls that overlap on one GPU
+ * share that array and race. */
+int nodals_wall_profile5_noalloc(int64_t n, const double* U, const double* U_signed, const double* y, double nu,
+                                 double* out, int32_t* valid);
 /* OpenFOAM-style omega wall target (evaluate_of_auto_host,
  * modules/wall/portable/wall_host.hpp:5-9): in[n][15] = y nu Ut uTau k omega
  * kFloor omegaFloor kappa E beta1 reBlend prodLimit pk nut; shared[n];

@@ -55,7 +55,7 @@ EXPORTS = [
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
     "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
-    "nodals_wall_fit_tangential",
+    "nodals_wall_profile5_noalloc", "nodals_wall_fit_tangential",
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
     "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
     L.nodals_wall_spalding.argtypes = [C.c_int64, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_profile5.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_omega_target.argtypes = [C.c_int64, vp, vp, vp]
+    L.nodals_wall_profile5_noalloc.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_fit_tangential.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp]
     L.nodals_gnn_create_for_h8.argtypes = [vp, C.c_uint32, C.POINTER(vp)]
     L.nodals_gnn_destroy.argtypes = [vp]
@@ -468,6 +469,16 @@ def wall_profile5(U, Us, y, nu):
     out = np.empty((n, 6))
     valid = np.empty(n, dtype=np.int32)
     _check(lib().nodals_wall_profile5(n, _ptr(U), _ptr(Us), _ptr(y), nu, _ptr(out), _ptr(valid)))
+    return out, valid
+
+
+def wall_profile5_noalloc(U, Us, y, nu):
+    """wall_profile5 through the allocation-free entry point (n <= 65536)."""
+    n = int(np.asarray(U).size // 5)
+    U, Us, y = _f64(U, (n, 5)), _f64(Us, (n, 5)), _f64(y, (n, 5))
+    out = np.empty((n, 6))
+    valid = np.empty(n, dtype=np.int32)
+    _check(lib().nodals_wall_profile5_noalloc(n, _ptr(U), _ptr(Us), _ptr(y), nu, _ptr(out), _ptr(valid)))
     return out, valid
 
 

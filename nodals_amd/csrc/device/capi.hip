@@ -559,6 +559,11 @@ int nodals_wall_profile5(int64_t n, const double* U, const double* U_signed, con
   return guarded([&]() { nodals::profile5Batch(n, U, U_signed, y, nu, out, valid); });
 }
 
+int nodals_wall_profile5_noalloc(int64_t n, const double* U, const double* U_signed, const double* y, double nu,
+                                 double* out, int32_t* valid) {
+  return guarded([&]() { nodals::profile5BatchNoAlloc(n, U, U_signed, y, nu, out, valid); });
+}
+
 int nodals_wall_fit_tangential(int64_t n, const double* samples, const double* y, const double* nrm,
                                const double* pic, double nu, int32_t count, int32_t zero_limit, double* out,
                                int32_t* valid) {

@@ -1,6 +1,8 @@
 // Batched pointwise SST and wall-law kernels (one thread per point / face).
 #include "sst_kernels.hpp"
 
+#include <mutex>
+
 #include "common.hpp"
 #include "sst_physics.hpp"
 
@@ -110,6 +112,36 @@ struct Staging {
   void sync() { NODALS_HIP(hipStreamSynchronize(s)); }
 };
 
+// Staging memory of the allocation-free five-point fit (profile5BatchNoAlloc):
+// U, U_signed and y ([n][5] doubles each), out ([n][6] doubles) and valid
+// ([n] ints), packed in that order for n <= kNoAllocMaxFaces, in device
+// memory the code object reserves on every GPU it is loaded on.  Host code
+// never passes `scratch` itself to a kernel or copy: each device's address of
+// it is taken once with hipGetSymbolAddress and used from then on.
+constexpr long long kNoAllocMaxFaces = 65536;
+constexpr std::size_t kNoAllocBytes = (std::size_t)kNoAllocMaxFaces * (21 * sizeof(double) + sizeof(int));
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+unsigned char* g_scratchPtr[kMaxDevices] = {};
+
+// The calling thread's current device's address of `scratch`, looked up on
+// the first call on that device.
+unsigned char* scratchOnCurrentDevice() {
+  int dev = -1;
+  NODALS_HIP(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kMaxDevices, "wall_profile5_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void* ptr = nullptr;
+    NODALS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    require(ptr != nullptr, "wall_profile5_noalloc: no device address for the staging array");
+    g_scratchPtr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratchPtr[dev];
+}
+
 }  // namespace
 
 void sstLocalBatch(long long n, const double* dist, const double* xy, double radius, double nu, const double* gu,
@@ -166,6 +198,30 @@ void profile5Batch(long long n, const double* U, const double* Us, const double*
   S.out(out, dout, 6 * (std::size_t)n);
   S.out(valid, dv, (std::size_t)n);
   S.sync();
+}
+
+void profile5BatchNoAlloc(long long n, const double* U, const double* Us, const double* y, double nu, double* out,
+                          int* valid) {
+  require(n >= 0 && U && Us && y && out && valid, "wall_profile5_noalloc: null or bad arguments");
+  require(n <= kNoAllocMaxFaces, "wall_profile5_noalloc: more than 65536 faces (the staging array's capacity)");
+  if (n == 0) return;
+  unsigned char* base = scratchOnCurrentDevice();
+  const std::size_t n5 = 5 * (std::size_t)n;
+  double* dU = reinterpret_cast<double*>(base);
+  double* dUs = dU + n5;
+  double* dy = dUs + n5;
+  double* dout = dy + n5;
+  int* dvalid = reinterpret_cast<int*>(dout + 6 * (std::size_t)n);
+  // Every output element is written by the kernel, so nothing left from an
+  // earlier call needs clearing.  Null stream: a call is complete on return.
+  NODALS_HIP(hipMemcpyAsync(dU, U, n5 * sizeof(double), hipMemcpyHostToDevice, nullptr));
+  NODALS_HIP(hipMemcpyAsync(dUs, Us, n5 * sizeof(double), hipMemcpyHostToDevice, nullptr));
+  NODALS_HIP(hipMemcpyAsync(dy, y, n5 * sizeof(double), hipMemcpyHostToDevice, nullptr));
+  hipLaunchKernelGGL(k_profile5, blocksFor(n), 256, 0, nullptr, n, dU, dUs, dy, nu, dout, dvalid);
+  NODALS_LAUNCHED();
+  NODALS_HIP(hipMemcpyAsync(out, dout, 6 * (std::size_t)n * sizeof(double), hipMemcpyDeviceToHost, nullptr));
+  NODALS_HIP(hipMemcpyAsync(valid, dvalid, (std::size_t)n * sizeof(int), hipMemcpyDeviceToHost, nullptr));
+  NODALS_HIP(hipStreamSynchronize(nullptr));
 }
 
 void fitTangentialBatch(long long n, const double* samples, const double* y, const double* nrm, const double* pic,

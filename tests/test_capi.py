@@ -119,3 +119,14 @@ def test_cell_graph_is_the_face_adjacency(mesh_dir):
         if len(cs) == 2:
             assert (cs[0], cs[1]) in pairs
     h.close()
+
+
+def test_noalloc_profile5_rejects_oversize_before_device_work():
+    """More than 65536 faces is refused with an error before any HIP call
+    (so this runs without a GPU as well)."""
+    from nodals_amd import _lib
+
+    n = 65537
+    z = np.zeros((n, 5))
+    with pytest.raises(_lib.NodalsError, match="65536"):
+        _lib.wall_profile5_noalloc(z, z, z + 1.0, 1e-5)
