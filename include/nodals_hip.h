@@ -54,6 +54,9 @@ typedef struct {
   int64_t pressure_its;
   int32_t warm_starts, fallbacks, recorded, reserved2;
   double turb_ms;
+  /* Wall-function path: the last iteration's relative changes of u, k and
+   * omega and its continuity residual after the correction. */
+  double final_du, final_dk, final_dw, final_cont;
 } nodals_h8_run_result;
 
 /* Last error message of the calling thread ("" if none). */
@@ -110,6 +113,16 @@ int nodals_h8_pressure_update(nodals_h8* h, double alpha_p);
 
 /* The SIMPLE loop of h8_main_base.cu:2046-2119 with its marker lines. */
 int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res);
+/* Wall-function path: `count` SIMPLE iterations numbered first_it ..
+ * first_it + count - 1, continuing from the current u, p, k and omega, so a
+ * run can be cut into windows around checkpoints (the SST application's
+ * Stage31 checkpoint cadence, sst_application.inc, STAGE31_CHECKPOINT_EVERY);
+ * res->outer is the last iteration's number. */
+int nodals_h8_run_window(nodals_h8* h, int32_t first_it, int32_t count, nodals_h8_run_result* res);
+/* ASCII VTK unstructured grid (.vtu) of the current fields: U, k, omega and
+ * the wall distance at the vertices, p per cell (the SST application's
+ * flow_sst.vtu, --vtu; SST_G2_VTU_EVERY). */
+int nodals_h8_write_vtu(nodals_h8* h, const char* path);
 
 /* Host copies in reference numbering.  Names (double unless noted):
  *   u0 u1 u2 (free DOFs), b0 b1 b2, rau delta diag (free DOFs),
@@ -129,10 +142,13 @@ int nodals_h8_fine_spmv(nodals_h8* h, const double* x, double* y);
  * h8_main_base.cu:557-564) on the current fine operator. */
 int nodals_h8_precond(nodals_h8* h, const double* r, double* z);
 int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double* ms_avg, double* bytes);
-/* Launch probe of the fine Schur SpMV inside the solver's own calls (the
- * Krylov operator of h8_main_base.cu:567-726): enable=1 starts bracketing
- * every such launch with HIP events on the context stream; enable=0 stops
- * and returns the launch count and the summed launch durations (ms).  No
+/* Launch probe inside the solver's own calls: enable=1 starts bracketing
+ * every fine Schur SpMV launch (the Krylov operator of
+ * h8_main_base.cu:567-726), enable=2 every momentum Gauss-Seidel colour
+ * launch (g4_mcgs_color_kernel, g5e_fp32_kernels.cuh:106-116; the wall
+ * path's M3 block colour, stage22d4d_coupled_matrixfree.inc:843-889) with
+ * HIP events on the context stream; enable=0 stops and returns the launch
+ * count and the summed launch durations (ms).  No
  * reference counterpart (measurement hook, like the reference's
  * G5StageEvents, h8_main_base.cu:2048-2105). */
 int nodals_h8_probe(nodals_h8* h, int32_t enable, int64_t* launches, double* total_ms);

@@ -36,7 +36,8 @@ class RunResult(C.Structure):
                 ("momentum_ms", C.c_double), ("continuity_ms", C.c_double), ("pressure_ms", C.c_double),
                 ("pupdate_ms", C.c_double), ("setup_ms", C.c_double), ("pressure_its", C.c_int64),
                 ("warm_starts", C.c_int32), ("fallbacks", C.c_int32), ("recorded", C.c_int32),
-                ("reserved2", C.c_int32), ("turb_ms", C.c_double)]
+                ("reserved2", C.c_int32), ("turb_ms", C.c_double), ("final_du", C.c_double),
+                ("final_dk", C.c_double), ("final_dw", C.c_double), ("final_cont", C.c_double)]
 
     def as_dict(self) -> Dict[str, float]:
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
@@ -48,7 +49,7 @@ EXPORTS = [
     "nodals_last_error", "nodals_version",
     "nodals_h8_create", "nodals_h8_create_host", "nodals_h8_destroy", "nodals_h8_info", "nodals_h8_assemble", "nodals_h8_momentum_work",
     "nodals_h8_momentum_audit", "nodals_h8_continuity", "nodals_h8_pressure_refresh", "nodals_h8_pressure_solve",
-    "nodals_h8_pressure_update", "nodals_h8_run", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
+    "nodals_h8_pressure_update", "nodals_h8_run", "nodals_h8_run_window", "nodals_h8_write_vtu", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
     "nodals_h8_precond",
     "nodals_h8_time_kernel", "nodals_h8_probe",
     "nodals_sys_create", "nodals_sys_create_host", "nodals_sys_get", "nodals_sys_destroy", "nodals_sys_info",
@@ -89,6 +90,8 @@ def lib() -> C.CDLL:
     L.nodals_h8_pressure_solve.argtypes = [vp, C.c_char_p, C.c_double, C.c_double, C.c_int32, C.POINTER(SolveResult)]
     L.nodals_h8_pressure_update.argtypes = [vp, C.c_double]
     L.nodals_h8_run.argtypes = [vp, C.POINTER(RunResult)]
+    L.nodals_h8_run_window.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(RunResult)]
+    L.nodals_h8_write_vtu.argtypes = [vp, C.c_char_p]
     L.nodals_h8_get.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     L.nodals_h8_set.argtypes = [vp, C.c_char_p, vp, C.c_int64]
     L.nodals_h8_fine_spmv.argtypes = [vp, vp, vp]
@@ -229,6 +232,16 @@ class H8:
         _check(lib().nodals_h8_run(self._h, C.byref(r)))
         return r.as_dict()
 
+    def run_window(self, first_it: int, count: int) -> Dict[str, float]:
+        """Wall-function path: `count` iterations numbered from first_it,
+        continuing from the current state."""
+        r = RunResult()
+        _check(lib().nodals_h8_run_window(self._h, int(first_it), int(count), C.byref(r)))
+        return r.as_dict()
+
+    def write_vtu(self, path) -> None:
+        _check(lib().nodals_h8_write_vtu(self._h, str(path).encode()))
+
     def fine_spmv(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
@@ -316,11 +329,13 @@ class H8:
         mode = {True: 1, "guess": 1, "direction": 2, False: 0, None: 0}[warm_start]
         _check(lib().nodals_h8_attach_gnn(self._h, gnn._h if gnn is not None else None, mode, record_every))
 
-    def probe_start(self):
-        _check(lib().nodals_h8_probe(self._h, 1, None, None))
+    def probe_start(self, kind: int = 1):
+        """kind 1: fine Schur SpMV launches; kind 2: momentum Gauss-Seidel
+        colour launches."""
+        _check(lib().nodals_h8_probe(self._h, int(kind), None, None))
 
     def probe_stop(self):
-        """(launches, summed ms) of the fine-SpMV launches since probe_start."""
+        """(launches, summed ms) of the probed launches since probe_start."""
         n, ms = C.c_int64(), C.c_double()
         _check(lib().nodals_h8_probe(self._h, 0, C.byref(n), C.byref(ms)))
         return n.value, ms.value

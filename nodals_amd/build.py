@@ -6,6 +6,7 @@ see csrc/host/h8_setup.hpp), and links in-tree:
 
     nodals_amd/libnodals_hip.so
     nodals_amd/bin/nodals_gpu_h8_fp64   (+ _fp32, _bf16 copies)
+    nodals_amd/bin/nodals_sst           (the SST case launcher's native solver)
 
 Usage: ``python -m nodals_amd.build`` or ``nodals_amd.build.build()``.
 """
@@ -79,6 +80,10 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     _run([HIPCC, "-o", str(exe), str(main_o), f"-L{PKG}", "-lnodals_hip", "-Wl,-rpath,$ORIGIN/.."], verbose)
     for suffix in ("fp32", "bf16"):
         shutil.copy2(exe, BIN / f"nodals_gpu_h8_{suffix}")
+    sst_o = OBJ / "sst_main.o"
+    _run([HIPCC] + HOST_FLAGS + ["-c", str(CSRC / "sst_main.cpp"), "-o", str(sst_o)], verbose)
+    _run([HIPCC, "-o", str(BIN / "nodals_sst"), str(sst_o), f"-L{PKG}", "-lnodals_hip", "-Wl,-rpath,$ORIGIN/.."],
+         verbose)
     return LIB
 
 

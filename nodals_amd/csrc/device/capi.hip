@@ -165,31 +165,49 @@ int nodals_h8_pressure_update(nodals_h8* h, double alpha_p) {
   return guarded([&]() { solver(h).pressureUpdate(alpha_p); });
 }
 
+namespace {
+void fillRunResult(const nodals::H8RunResult& r, nodals_h8_run_result* res) {
+  if (!res) return;
+  res->outer = r.outer;
+  res->converged = r.converged;
+  res->pass = r.pass;
+  res->reserved = 0;
+  res->final_rel_cont = r.finalRelCont;
+  res->avg_pressure_its = r.avgPressureIts;
+  res->pressure_drop = r.pressureDrop;
+  res->exact_pressure_drop = r.exactPressureDrop;
+  res->loop_ms = r.loopMs;
+  res->assembly_ms = r.assemblyMs;
+  res->momentum_ms = r.momentumMs;
+  res->continuity_ms = r.continuityMs;
+  res->pressure_ms = r.pressureMs;
+  res->pupdate_ms = r.pupdateMs;
+  res->setup_ms = r.setupMs;
+  res->pressure_its = r.pressureIts;
+  res->warm_starts = r.warmStarts;
+  res->fallbacks = r.fallbacks;
+  res->recorded = r.recorded;
+  res->reserved2 = 0;
+  res->turb_ms = r.turbMs;
+  res->final_du = r.finalDu;
+  res->final_dk = r.finalDk;
+  res->final_dw = r.finalDw;
+  res->final_cont = r.finalCont;
+}
+}  // namespace
+
 int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res) {
+  return guarded([&]() { fillRunResult(solver(h).run(), res); });
+}
+
+int nodals_h8_run_window(nodals_h8* h, int32_t first_it, int32_t count, nodals_h8_run_result* res) {
+  return guarded([&]() { fillRunResult(solver(h).runWindow(first_it, count), res); });
+}
+
+int nodals_h8_write_vtu(nodals_h8* h, const char* path) {
   return guarded([&]() {
-    const nodals::H8RunResult r = solver(h).run();
-    if (!res) return;
-    res->outer = r.outer;
-    res->converged = r.converged;
-    res->pass = r.pass;
-    res->reserved = 0;
-    res->final_rel_cont = r.finalRelCont;
-    res->avg_pressure_its = r.avgPressureIts;
-    res->pressure_drop = r.pressureDrop;
-    res->exact_pressure_drop = r.exactPressureDrop;
-    res->loop_ms = r.loopMs;
-    res->assembly_ms = r.assemblyMs;
-    res->momentum_ms = r.momentumMs;
-    res->continuity_ms = r.continuityMs;
-    res->pressure_ms = r.pressureMs;
-    res->pupdate_ms = r.pupdateMs;
-    res->setup_ms = r.setupMs;
-    res->pressure_its = r.pressureIts;
-    res->warm_starts = r.warmStarts;
-    res->fallbacks = r.fallbacks;
-    res->recorded = r.recorded;
-    res->reserved2 = 0;
-    res->turb_ms = r.turbMs;
+    nodals::require(path != nullptr, "null VTU path");
+    solver(h).writeVtu(path);
   });
 }
 
@@ -396,7 +414,7 @@ int nodals_h8_time_kernel(nodals_h8* h, const char* kernel, int32_t reps, double
 int nodals_h8_probe(nodals_h8* h, int32_t enable, int64_t* launches, double* total_ms) {
   return guarded([&]() {
     if (enable) {
-      solver(h).probeStart();
+      solver(h).probeStart(enable);
       return;
     }
     long long n = 0;

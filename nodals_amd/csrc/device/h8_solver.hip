@@ -189,11 +189,23 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--amg-precision") o.amgPrecision = next();
     else if (a == "--pressure-restart") o.pressureRestart = inum(next());
     else if (a == "--verbose") o.verbose = inum(next());
+    else if (a == "--duct-diameter") o.ductDiameter = num(next());
+    else if (a == "--inlet-velocity") {
+      const std::string v = next();
+      double x[3];
+      require(std::sscanf(v.c_str(), "%lf,%lf,%lf", &x[0], &x[1], &x[2]) == 3, "--inlet-velocity expects x,y,z");
+      for (int d = 0; d < 3; ++d) o.inletVelocity[d] = x[d];
+      o.inletVelocityGiven = true;
+    } else if (a == "--sst-cont-atol") o.sstContAtol = num(next());
+    else if (a == "--print-every") o.printEvery = inum(next());
     else fail("H8 usage error: unknown option " + a);
   }
   require(!o.mesh.empty(), "--mesh required");
   require(o.wallModel == "no_slip" || o.wallModel == "spalding", "--wall-model must be no_slip or spalding");
   require(o.wallModel == "no_slip" || o.duct, "--wall-model spalding needs --duct 1");
+  require(o.ductDiameter >= 0.0 && std::isfinite(o.ductDiameter), "--duct-diameter must be >= 0");
+  require(!o.inletVelocityGiven || o.duct, "--inlet-velocity needs --duct 1");
+  require(o.sstContAtol >= 0.0 && o.printEvery >= 1, "--sst-cont-atol must be >= 0 and --print-every >= 1");
   // The SST application's default rAU scale (sst_application.inc:69).
   if (o.wallModel == "spalding" && !o.rauScaleGiven) o.rauScale = 2.0;
   require(o.simpleTol > 0.0 && o.maxOuter >= 1, "H8 invalid SIMPLE controls");
@@ -260,7 +272,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
 // ------------------------------------------------------------------ setup
 H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   const auto t0 = std::chrono::steady_clock::now();
-  M_ = readPolyMesh(opt_.mesh);
+  M_ = readMesh(opt_.mesh);
   H8Params prm;
   prm.wall = opt_.wall;
   prm.inlet = opt_.inlet;
@@ -272,6 +284,9 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   prm.bulk = opt_.bulk;
   prm.alphaU = opt_.alphaU;
   prm.rauScale = opt_.rauScale;
+  prm.ductDiameter = opt_.ductDiameter;
+  prm.inletVelocityGiven = opt_.inletVelocityGiven;
+  for (int d = 0; d < 3; ++d) prm.inletVelocity[d] = opt_.inletVelocity[d];
   S_ = buildH8Setup(M_, prm);
   F_ = buildFineTopo(S_);
   int powerIts = 0;
@@ -696,11 +711,27 @@ const char* H8Solver::momentumWork(const std::string& mode, double omega, int ou
       if (r1 <= r0) continue;
       const int rs = colorSplit_[(std::size_t)c];
       const int nbL = blocksFor((long long)(rs - r0) * kLongLanes), nbS = blocksFor((long long)(r1 - rs) * kShortLanes);
+      if (probeOn_ && probeKind_ == 2) {  // timed from its own dispatch packet
+        hipExtLaunchKernelGGL((k_mom_gs<kLongLanes, kShortLanes>), dim3(nbL + nbS), dim3(256), 0, stream_,
+                              probeEv_[2 * (std::size_t)probeUsed_], probeEv_[2 * (std::size_t)probeUsed_ + 1], 0, r0,
+                              rs, r1, nbL, mrow_.data(), mcol_.data(), aval_.data(), diag_.data(), brhs_.data(),
+                              u_.data(), omega);
+        NODALS_LAUNCHED();
+        if (++probeUsed_ == kProbePairs) probeHarvest();
+        continue;
+      }
       hipLaunchKernelGGL((k_mom_gs<kLongLanes, kShortLanes>), nbL + nbS, 256, 0, stream_, r0, rs, r1, nbL,
                          mrow_.data(), mcol_.data(), aval_.data(), diag_.data(), brhs_.data(), u_.data(), omega);
       NODALS_LAUNCHED();
     }
   };
+  if (probeOn_ && probeKind_ == 2) {  // probed sweeps run outside the graph
+    for (int k = 0; k < sweeps; ++k) {
+      if (mode == "fgs1" || (mode == "altgs1" && (outer & 1)) || mode == "sgs1") sweep(false);
+      if ((mode == "altgs1" && !(outer & 1)) || mode == "sgs1") sweep(true);
+    }
+    return mode == "sgs1" ? "FWD+BWD" : ((mode == "altgs1" && !(outer & 1)) ? "BWD" : "FWD");
+  }
   // One colour launch per colour; the unit (a sweep, or a forward and a
   // backward sweep) repeated `sweeps` times is replayed from one graph.
   const std::string key = graphKey({u_.data(), aval_.data(), brhs_.data()}, omega) + "x" + std::to_string(sweeps);
@@ -830,7 +861,7 @@ void H8Solver::refreshOn(hipStream_t s) {
 }
 
 void H8Solver::fineApply(hipStream_t s, const double* x, double* y) {
-  if (!(probeOn_ && s == stream_)) {
+  if (!(probeOn_ && probeKind_ == 1 && s == stream_)) {
     schurSpmv(s, fineLanes_, nc_, frow_.data(), fcol_.data(), fval_.data(), x, y);
     return;
   }
@@ -839,8 +870,10 @@ void H8Solver::fineApply(hipStream_t s, const double* x, double* y) {
   if (++probeUsed_ == kProbePairs) probeHarvest();
 }
 
-void H8Solver::probeStart() {
+void H8Solver::probeStart(int kind) {
   requireDevice();
+  require(kind == 1 || kind == 2, "probe kind must be 1 (fine SpMV) or 2 (momentum Gauss-Seidel)", kErrInvalid);
+  probeKind_ = kind;
   if (probeEv_.empty()) {
     probeEv_.resize(2 * (std::size_t)kProbePairs);
     for (hipEvent_t& e : probeEv_) NODALS_HIP(hipEventCreate(&e));
@@ -1117,7 +1150,7 @@ double H8Solver::rauSnapshotRel() {
 // ------------------------------------------------------------------ SIMPLE loop
 H8RunResult H8Solver::run() {
   requireDevice();
-  if (wall_) return runWall();
+  if (wall_) return runWall(1, opt_.maxOuter);
   const H8Options& o = opt_;
   H8RunResult R;
   R.setupMs = setupMs_;

@@ -69,6 +69,19 @@ struct H8Options {
   // times the identity, so an M3 forward sweep is this coloured sweep)
   int momSweeps = 1;
   int verbose = 1;
+  // Duct mode: the case's diameter for nu = bulk D / Re (0: the inlet's
+  // hydraulic diameter) and an explicit inlet velocity vector (the SST
+  // case's [inlet.main] velocity; unset: bulk along the inward inlet normal).
+  double ductDiameter = 0.0;
+  bool inletVelocityGiven = false;
+  double inletVelocity[3] = {0, 0, 0};
+  // Wall-function path: the SST application's convergence gate
+  // (compiled_gate.cpp:6-13) when > 0 -- the continuity residual after the
+  // correction at most this absolute tolerance and the relative k and omega
+  // changes at most simpleTol -- instead of the relative gate; and the
+  // cadence of the progress line.
+  double sstContAtol = 0.0;
+  int printEvery = 10;
 };
 
 // Parses the reference H8 argv (h8_main_base.cu:1787-1850) and validates it
@@ -89,6 +102,7 @@ struct H8RunResult {
   long long pressureIts = 0;  // total Krylov iterations of the run
   int warmStarts = 0, fallbacks = 0, recorded = 0;
   double turbMs = 0;          // k-omega steps of a coupled run
+  double finalDu = 0, finalDk = 0, finalDw = 0, finalCont = 0;  // wall path: last iteration's changes
 };
 
 class H8Solver {
@@ -125,6 +139,12 @@ class H8Solver {
   void pressureUpdate(double alphaP);
   double rauSnapshotRel();
   H8RunResult run();
+  // Wall-function path: `count` SIMPLE iterations numbered from `firstIt`,
+  // continuing from the current state (chunks of one run, for checkpoints).
+  H8RunResult runWindow(int firstIt, int count);
+  // ASCII VTK unstructured grid of the current fields: velocity, k, omega
+  // and wall distance at the vertices, pressure per cell.
+  void writeVtu(const std::string& path);
 
   // Host access in reference numbering (free DOFs as in the reference).
   std::vector<double> getVelocity(int component);
@@ -151,10 +171,13 @@ class H8Solver {
   // Mean duration (ms) of `reps` launches of a hot kernel measured with HIP
   // events on the context stream; *bytes = algorithmic bytes per launch.
   double timeKernel(const std::string& kernel, int reps, double* bytes);
-  // Launch probe: while on, every fine-SpMV launch the solver makes (the
-  // Krylov operator) is bracketed by HIP events on the context stream;
-  // probeStop returns the launch count and the summed launch durations.
-  void probeStart();
+  // Launch probe: while on, every launch of the probed kernel the solver
+  // makes is bracketed by HIP events on the context stream (kind 1: the fine
+  // Schur SpMV, the Krylov operator; kind 2: the momentum Gauss-Seidel colour
+  // passes, k_mom_gs or the wall path's k_m3_gs, launched outside the sweep
+  // graph while probed); probeStop returns the launch count and the summed
+  // launch durations.
+  void probeStart(int kind = 1);
   void probeStop(long long* launches, double* totalMs);
   // Learned warm start: the cell graph and normalised wall distance the
   // network sees, and attaching a network to predict each pressure
@@ -300,7 +323,7 @@ class H8Solver {
   std::unique_ptr<TurbState> turb_;
   bool turbCouple_ = false;
   std::unique_ptr<WallState> wall_;
-  H8RunResult runWall();
+  H8RunResult runWall(int firstIt, int count);
   int wallSweeps() const;
   int wallCounters(bool sweep);   // reads and checks the wall counters (syncs)
   std::vector<long long> newOfOld_;  // reference momentum entry -> device entry
@@ -315,6 +338,7 @@ class H8Solver {
   std::vector<hipEvent_t> probeEv_;
   int probeUsed_ = 0;
   bool probeOn_ = false;
+  int probeKind_ = 1;
   long long probeLaunches_ = 0;
   double probeMs_ = 0;
 };

@@ -337,7 +337,7 @@ void H8Solver::wallMomentum() {
       if (r1 <= r0) continue;
       const int rs = colorSplit_[(std::size_t)c];
       const int nbL = blocksFor((long long)(rs - r0) * kLongLanes), nbS = blocksFor((long long)(r1 - rs) * kShortLanes);
-      if (probeOn_) {  // the launch probe times each colour from its own dispatch packet
+      if (probeOn_ && probeKind_ == 2) {  // the launch probe times each colour from its own dispatch packet
         hipExtLaunchKernelGGL((k_m3_gs<kLongLanes, kShortLanes>), dim3(nbL + nbS), dim3(256), 0, s,
                               probeEv_[2 * (std::size_t)probeUsed_], probeEv_[2 * (std::size_t)probeUsed_ + 1], 0,
                               r0, rs, r1, nbL, mrow_.data(), mcol_.data(), mdiagPos_.data(), w.K.data(),
@@ -457,6 +457,7 @@ std::vector<double> H8Solver::wallField(const std::string& n) {
                          w.qw.data(), w.bp.data(), w.fit.data(), cref_.data(), u_.data(), fixedVal_.data(),
                          w.shear.data());
       NODALS_LAUNCHED();
+      sync();  // the kernel runs on the context stream, the copy below does not
     }
     std::vector<double> v = w.shear.host();
     v.resize(4 * (std::size_t)W.nf);
@@ -467,7 +468,18 @@ std::vector<double> H8Solver::wallField(const std::string& n) {
 }
 
 // ------------------------------------------------------------------ loop
-H8RunResult H8Solver::runWall() {
+namespace {
+int lastItOf(int firstIt, int count) { return firstIt + count - 1; }
+}  // namespace
+
+H8RunResult H8Solver::runWindow(int firstIt, int count) {
+  requireDevice();
+  require((bool)wall_, "run_window needs the wall-function path (nodals_h8_wall_init)", kErrInvalid);
+  require(firstIt >= 1 && count >= 1, "run_window: first iteration and count must be >= 1", kErrInvalid);
+  return runWall(firstIt, count);
+}
+
+H8RunResult H8Solver::runWall(int firstIt, int count) {
   const H8Options& o = opt_;
   WallState& w = *wall_;
   TurbState& st = *turb_;
@@ -485,7 +497,8 @@ H8RunResult H8Solver::runWall() {
   double cont0 = 0.0, contRel = 1.0;
   const auto t0 = std::chrono::steady_clock::now();
   try {
-    for (int it = 1; it <= o.maxOuter; ++it) {
+    const int lastIt = firstIt + count - 1;
+    for (int it = firstIt; it <= lastIt; ++it) {
       NODALS_HIP(hipMemcpyAsync(w.uPrev.data(), u_.data(), sizeof(double) * 3 * (std::size_t)nv_,
                                 hipMemcpyDeviceToDevice, stream_));
       NODALS_HIP(hipEventRecord(ev[0], stream_));
@@ -532,14 +545,23 @@ H8RunResult H8Solver::runWall() {
       R.continuityMs += ms[2];
       R.pressureMs += ms[3];
       R.turbMs += ms[4];
-      if (it == 1) cont0 = std::max(cnPost, 1e-300);
+      if (it == firstIt) cont0 = std::max(cnPost, 1e-300);
       contRel = cnPost / cont0;
       R.outer = it;
       R.finalRelCont = contRel;
-      const bool done = it > 1 && contRel <= o.simpleTol && du <= o.simpleTol && dk <= o.simpleTol && dw <= o.simpleTol;
-      if (verbose && (it == 1 || it % 10 == 0 || done || it == o.maxOuter)) {
-        std::printf("NODALS_SST_WALL it=%d contPre=%.6e contPost=%.6e contRel=%.6e dU=%.3e dK=%.3e dOmega=%.3e "
-                    "pIts=%d kClip=%d omegaClip=%d status=PASS\n",
+      // the relative gate, or the SST application's gate (mass: absolute
+      // continuity; turbulence: relative k and omega changes)
+      const bool done = o.sstContAtol > 0.0
+                            ? (cnPost <= o.sstContAtol && dk <= o.simpleTol && dw <= o.simpleTol)
+                            : (it > firstIt && contRel <= o.simpleTol && du <= o.simpleTol && dk <= o.simpleTol &&
+                               dw <= o.simpleTol);
+      R.finalDu = du;
+      R.finalDk = dk;
+      R.finalDw = dw;
+      R.finalCont = cnPost;
+      if (verbose && (it == firstIt || it % o.printEvery == 0 || done || it == lastIt)) {
+        std::printf("NODALS_SST_RESIDUALS it=%d contPre=%.6e contPost=%.6e contRel=%.6e dU=%.3e dK=%.3e "
+                    "dOmega=%.3e pIts=%d kClip=%d omegaClip=%d status=PASS\n",
                     it, cn, cnPost, contRel, du, dk, dw, pr.its, tr.kClips, tr.omegaClips);
         std::fflush(stdout);
       }
@@ -554,10 +576,71 @@ H8RunResult H8Solver::runWall() {
   }
   destroyEvents();
   R.loopMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  R.avgPressureIts = R.outer ? (double)R.pressureIts / R.outer : 0.0;
+  R.avgPressureIts = R.outer >= firstIt ? (double)R.pressureIts / (R.outer - firstIt + 1) : 0.0;
   R.pressureDrop = pressureDropPatches(S_, p_.host());
-  R.pass = (o.runMode == "fixed10") ? (R.outer == o.maxOuter || R.converged) : R.converged;
+  R.pass = (o.runMode == "fixed10") ? (R.outer == lastItOf(firstIt, count) || R.converged) : R.converged;
   return R;
+}
+
+}  // namespace nodals
+
+namespace nodals {
+
+void H8Solver::writeVtu(const std::string& path) {
+  requireDevice();
+  sync();
+  const int np = M_.nPoints(), nc = M_.nCells();
+  const std::vector<double> ud = u_.host(), pd = p_.host();
+  std::vector<double> uv((std::size_t)np * 3, 0.0);
+  for (int v = 0; v < np; ++v) {
+    const int g = S_.g2free[(std::size_t)v];
+    for (int d = 0; d < 3; ++d) {
+      if (g >= 0) uv[3 * (std::size_t)v + d] = ud[3 * (std::size_t)pinv_[(std::size_t)g] + d];
+      else if (S_.fixedSlot[(std::size_t)v] >= 0)
+        uv[3 * (std::size_t)v + d] = S_.fixedValue[(std::size_t)S_.fixedSlot[(std::size_t)v]][(std::size_t)d];
+    }
+  }
+  std::vector<double> kv, wv;
+  if (turb_) {
+    kv = turb_->k.host();
+    wv = turb_->w.host();
+  }
+  std::FILE* f = std::fopen(path.c_str(), "w");
+  require(f != nullptr, "cannot write VTU file " + path);
+  std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+                  "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n<Piece NumberOfPoints=\"%d\" NumberOfCells=\"%d\">\n",
+               np, nc);
+  std::fprintf(f, "<PointData Vectors=\"U\">\n<DataArray type=\"Float64\" Name=\"U\" NumberOfComponents=\"3\" "
+                  "format=\"ascii\">\n");
+  for (int v = 0; v < np; ++v)
+    std::fprintf(f, "%.17g %.17g %.17g\n", uv[3 * (std::size_t)v], uv[3 * (std::size_t)v + 1],
+                 uv[3 * (std::size_t)v + 2]);
+  std::fprintf(f, "</DataArray>\n");
+  auto scalar = [&](const char* name, const std::vector<double>& a) {
+    std::fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n", name);
+    for (int v = 0; v < np; ++v) std::fprintf(f, "%.17g\n", a[(std::size_t)v]);
+    std::fprintf(f, "</DataArray>\n");
+  };
+  if (turb_) {
+    scalar("k", kv);
+    scalar("omega", wv);
+  }
+  if (!S_.wallDistance.empty()) scalar("wallDistance", S_.wallDistance);
+  std::fprintf(f, "</PointData>\n<CellData Scalars=\"p\">\n<DataArray type=\"Float64\" Name=\"p\" format=\"ascii\">\n");
+  for (int c = 0; c < nc; ++c) std::fprintf(f, "%.17g\n", pd[(std::size_t)c]);
+  std::fprintf(f, "</DataArray>\n</CellData>\n<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" "
+                  "format=\"ascii\">\n");
+  for (const auto& x : M_.points) std::fprintf(f, "%.17g %.17g %.17g\n", x[0], x[1], x[2]);
+  std::fprintf(f, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int32\" Name=\"connectivity\" "
+                  "format=\"ascii\">\n");
+  for (const auto& t : M_.tets) std::fprintf(f, "%d %d %d %d\n", t[0], t[1], t[2], t[3]);
+  std::fprintf(f, "</DataArray>\n<DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
+  for (int c = 0; c < nc; ++c) std::fprintf(f, "%d\n", 4 * (c + 1));
+  std::fprintf(f, "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+  for (int c = 0; c < nc; ++c) std::fprintf(f, "10\n");
+  std::fprintf(f, "</DataArray>\n</Cells>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+  const bool ok = std::fclose(f) == 0;
+  require(ok, "error writing VTU file " + path);
 }
 
 }  // namespace nodals

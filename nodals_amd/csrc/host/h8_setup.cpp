@@ -162,9 +162,11 @@ PipeInfo makeDuct(const TetMesh& M, const H8Params& prm) {
     P.inletNormal[d] = sfSum[(std::size_t)d] / P.inletProjectedArea;
     P.inletVelocity[d] = -prm.bulk * P.inletNormal[d];
   }
+  if (prm.inletVelocityGiven)
+    for (int d = 0; d < 3; ++d) P.inletVelocity[d] = prm.inletVelocity[d];
   P.cx = cen[0] / P.inletArea;
   P.cy = cen[1] / P.inletArea;
-  P.D = 2.0 * std::sqrt(P.inletArea / 3.14159265358979323846);
+  P.D = prm.ductDiameter > 0.0 ? prm.ductDiameter : 2.0 * std::sqrt(P.inletArea / 3.14159265358979323846);
   P.R = 0.5 * P.D;
   P.nu = prm.bulk * P.D / prm.re;
   P.circleArea = P.inletArea;

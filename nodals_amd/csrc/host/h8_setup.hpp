@@ -88,6 +88,11 @@ struct H8Params {
   // Duct mode only: the wall patch is a wall-function wall (free velocity
   // DOFs with a Nitsche/Spalding wall closure) instead of no-slip.
   bool wallFunction = false;
+  // Duct mode only: nu = bulk * ductDiameter / re when ductDiameter > 0
+  // (else the inlet's hydraulic diameter), and an explicit inlet velocity.
+  double ductDiameter = 0.0;
+  bool inletVelocityGiven = false;
+  double inletVelocity[3] = {0, 0, 0};
   double re = 20.0, bulk = 1.0;
   double alphaU = 0.5;
   double rauScale = 1.0;

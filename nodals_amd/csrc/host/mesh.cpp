@@ -254,6 +254,15 @@ int TetMesh::patchIndex(const std::string& name) const {
   return -1;
 }
 
+namespace {
+
+// Cells of a face list (owner/neighbour/patches set): the tetrahedra as the
+// sorted vertex set of their four faces with vertices 1 and 2 swapped for a
+// negative Jacobian, and oppFace.
+void finishTetMesh(TetMesh& M, const std::string& dir);
+
+}  // namespace
+
 TetMesh readPolyMesh(const std::string& dir) {
   TetMesh M;
   M.points = readPoints(dir + "/points");
@@ -261,6 +270,13 @@ TetMesh readPolyMesh(const std::string& dir) {
   M.owner = readLabels(dir + "/owner");
   M.neighbour = readLabels(dir + "/neighbour");
   M.patches = readBoundary(dir + "/boundary");
+  finishTetMesh(M, dir);
+  return M;
+}
+
+namespace {
+
+void finishTetMesh(TetMesh& M, const std::string& dir) {
   require(M.owner.size() == M.faces.size(), "owner size != faces size in " + dir);
   require(M.neighbour.size() <= M.faces.size(), "neighbour longer than faces in " + dir);
   M.facePatch.assign(M.faces.size(), -1);
@@ -329,7 +345,153 @@ TetMesh readPolyMesh(const std::string& dir) {
       M.oppFace[(std::size_t)c][(std::size_t)i] = found;
     }
   }
+}
+
+template <class T>
+void readRaw(std::ifstream& f, T* dst, std::size_t n, const std::string& what) {
+  if (n == 0) return;
+  f.read(reinterpret_cast<char*>(dst), (std::streamsize)(n * sizeof(T)));
+  require((bool)f, "HXT1: short read (" + what + ")");
+}
+
+}  // namespace
+
+bool isHxt1File(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  char magic[8] = {};
+  f.read(magic, 8);
+  return (bool)f && std::string(magic, 8) == "HXT1BIN1";
+}
+
+// HXT1 binary (turbulence/sst_gpu_v2/source/hybrid_hxt1/hxt1_mesh.hpp:
+// 20-131, written by tools/prepare_hxt1.py): magic "HXT1BIN1", uint32
+// version 1 and a reserved word, thirteen uint64 counts (nv nhex ntet
+// nhexface ntetface nvel npressure ninterface nwallhex ninhex nintet nouthex
+// nouttet), then points [nv][3] f64, hexes [nhex][8] i32, tets [ntet][4] i32,
+// hex / tet velocity DOFs [nhex][14] / [ntet][8] i32, hex / tet pressure ids,
+// hex / tet geometry [nhex][82] / [ntet][12] f64, interface records [][9],
+// and the boundary records (cell, local face, bubble) [][3] i32 of the hex
+// wall, hex and tet inlet, hex and tet outlet, with nothing after them.
+// Tetrahedral meshes only.  Local face lf of a tet is the one opposite
+// vertex lf.  Faces become an OpenFOAM-ordered face list: internal faces by
+// (owner, neighbour), then the boundary faces as patches patch_0_0 (wall:
+// every boundary face that is neither inlet nor outlet), patch_2_0 (inlet)
+// and patch_1_0 (outlet), the H8 duct patch names.
+TetMesh readHxt1(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  require((bool)f, "cannot open HXT1 mesh " + path);
+  char magic[8] = {};
+  f.read(magic, 8);
+  require((bool)f && std::string(magic, 8) == "HXT1BIN1", "bad HXT1 magic in " + path);
+  std::uint32_t version = 0, reserved = 0;
+  readRaw(f, &version, 1, "version");
+  readRaw(f, &reserved, 1, "reserved");
+  require(version == 1, "unsupported HXT1 version in " + path);
+  std::uint64_t h[13] = {};
+  readRaw(f, h, 13, "header");
+  const std::uint64_t nv = h[0], nhex = h[1], ntet = h[2], ninter = h[7], nwallhex = h[8], ninhex = h[9],
+                      nintet = h[10], nouthex = h[11], nouttet = h[12];
+  require(nhex == 0 && ninter == 0 && nwallhex == 0 && ninhex == 0 && nouthex == 0,
+          "HXT1 mesh with hexahedra: only tetrahedral meshes are supported (" + path + ")");
+  require(nv > 0 && ntet > 0 && nv < (1ull << 31) && ntet < (1ull << 31), "HXT1 sizes out of range in " + path);
+  std::vector<double> pts((std::size_t)nv * 3);
+  readRaw(f, pts.data(), pts.size(), "points");
+  std::vector<std::int32_t> tets((std::size_t)ntet * 4), tvel((std::size_t)ntet * 8), tpres((std::size_t)ntet);
+  readRaw(f, tets.data(), tets.size(), "tets");
+  readRaw(f, tvel.data(), tvel.size(), "tet velocity dofs");
+  readRaw(f, tpres.data(), tpres.size(), "tet pressure ids");
+  std::vector<double> tgeom((std::size_t)ntet * 12);
+  readRaw(f, tgeom.data(), tgeom.size(), "tet geometry");
+  std::vector<std::int32_t> inTet((std::size_t)nintet * 3), outTet((std::size_t)nouttet * 3);
+  readRaw(f, inTet.data(), inTet.size(), "inlet tet records");
+  readRaw(f, outTet.data(), outTet.size(), "outlet tet records");
+  char extra = 0;
+  f.read(&extra, 1);
+  require(f.gcount() == 0, "trailing bytes in HXT1 mesh " + path);
+  for (std::int32_t v : tets) require(v >= 0 && (std::uint64_t)v < nv, "HXT1 tet vertex out of range");
+
+  static const int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+  struct Side {
+    std::array<std::int32_t, 3> key;
+    std::int32_t cell, lf;
+  };
+  std::vector<Side> sides;
+  sides.reserve((std::size_t)ntet * 4);
+  for (std::int32_t c = 0; c < (std::int32_t)ntet; ++c)
+    for (int lf = 0; lf < 4; ++lf) {
+      std::array<std::int32_t, 3> k;
+      for (int j = 0; j < 3; ++j) k[(std::size_t)j] = tets[(std::size_t)c * 4 + (std::size_t)kFace[lf][j]];
+      std::sort(k.begin(), k.end());
+      sides.push_back({k, c, lf});
+    }
+  std::sort(sides.begin(), sides.end(), [](const Side& a, const Side& b) {
+    return a.key != b.key ? a.key < b.key : a.cell < b.cell;
+  });
+  // boundary kind of (cell, local face): 1 inlet, 2 outlet
+  std::vector<std::uint8_t> kind((std::size_t)ntet * 4, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    const std::vector<std::int32_t>& R = pass ? outTet : inTet;
+    for (std::size_t r = 0; r < R.size(); r += 3) {
+      const std::int32_t c = R[r], lf = R[r + 1];
+      require(c >= 0 && (std::uint64_t)c < ntet && lf >= 0 && lf < 4, "HXT1 boundary record out of range");
+      kind[(std::size_t)c * 4 + (std::size_t)lf] = (std::uint8_t)(pass + 1);
+    }
+  }
+  struct IFace {
+    std::int32_t own, nei;
+    std::array<std::int32_t, 3> v;
+  };
+  std::vector<IFace> inner;
+  std::vector<std::array<std::int32_t, 3>> bnd[3];
+  std::vector<std::int32_t> bown[3];
+  for (std::size_t i = 0; i < sides.size();) {
+    std::size_t j = i + 1;
+    while (j < sides.size() && sides[j].key == sides[i].key) ++j;
+    require(j - i <= 2, "non-manifold face in HXT1 mesh " + path);
+    const Side& a = sides[i];
+    std::array<std::int32_t, 3> fv;
+    for (int q = 0; q < 3; ++q) fv[(std::size_t)q] = tets[(std::size_t)a.cell * 4 + (std::size_t)kFace[a.lf][q]];
+    if (j - i == 2) {
+      require(kind[(std::size_t)a.cell * 4 + (std::size_t)a.lf] == 0 &&
+                  kind[(std::size_t)sides[i + 1].cell * 4 + (std::size_t)sides[i + 1].lf] == 0,
+              "HXT1 boundary record on an internal face");
+      inner.push_back({a.cell, sides[i + 1].cell, fv});
+    } else {
+      const int k = kind[(std::size_t)a.cell * 4 + (std::size_t)a.lf];  // 0 wall, 1 inlet, 2 outlet
+      bnd[k].push_back(fv);
+      bown[k].push_back(a.cell);
+    }
+    i = j;
+  }
+  std::sort(inner.begin(), inner.end(),
+            [](const IFace& a, const IFace& b) { return a.own != b.own ? a.own < b.own : a.nei < b.nei; });
+  TetMesh M;
+  M.points.resize((std::size_t)nv);
+  for (std::size_t v = 0; v < (std::size_t)nv; ++v) M.points[v] = {pts[3 * v], pts[3 * v + 1], pts[3 * v + 2]};
+  for (const IFace& q : inner) {
+    M.faces.push_back(q.v);
+    M.owner.push_back(q.own);
+    M.neighbour.push_back(q.nei);
+  }
+  static const char* kNames[3] = {"patch_0_0", "patch_2_0", "patch_1_0"};
+  for (int k = 0; k < 3; ++k) {
+    Patch P;
+    P.name = kNames[k];
+    P.start = (std::int32_t)M.faces.size();
+    P.count = (std::int32_t)bnd[k].size();
+    for (std::size_t r = 0; r < bnd[k].size(); ++r) {
+      M.faces.push_back(bnd[k][r]);
+      M.owner.push_back(bown[k][r]);
+    }
+    M.patches.push_back(P);
+  }
+  require(M.patches[1].count == (std::int32_t)nintet && M.patches[2].count == (std::int32_t)nouttet,
+          "HXT1 inlet/outlet records do not match the mesh's boundary faces");
+  finishTetMesh(M, path);
   return M;
 }
+
+TetMesh readMesh(const std::string& path) { return isHxt1File(path) ? readHxt1(path) : readPolyMesh(path); }
 
 }  // namespace nodals

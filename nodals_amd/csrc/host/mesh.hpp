@@ -42,6 +42,14 @@ struct TetMesh {
 // the tetrahedra.  Throws nodals::Error on malformed input.
 TetMesh readPolyMesh(const std::string& dir);
 
+// HXT1 binary mesh of the SST application (tetrahedra only), as a face list
+// with the H8 duct patch names: patch_0_0 wall, patch_2_0 inlet, patch_1_0
+// outlet.  Throws nodals::Error on malformed input.
+TetMesh readHxt1(const std::string& path);
+bool isHxt1File(const std::string& path);
+// readHxt1 for a file that starts with the HXT1 magic, readPolyMesh otherwise.
+TetMesh readMesh(const std::string& path);
+
 double det3(const double J[3][3]);
 // Inverse by cofactors divided by the determinant, the operation order of the
 // reference's inv3 (foam_mesh_g2.hpp:54) so that setup values are bitwise

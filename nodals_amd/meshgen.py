@@ -384,6 +384,44 @@ def write_helix_case(directory: os.PathLike, core: int = 12, ring: int = 9, nsla
     return write_polymesh(directory, mesh)
 
 
+def write_helix_msh2(path: os.PathLike, core: int = 12, ring: int = 9, nslab: int = 60, diameter: float = 1.0,
+                     coil_radius_d: float = 5.0, pitch_d: float = 2.0, turns: float = 2.0) -> Path:
+    """The helix of write_helix_case as a Gmsh ASCII 2.2 file with the SST
+    application's physical tags (inlet 1, outlet 2, wall 3, volume 4)."""
+    from nodals_amd import hxt1
+
+    tmp = Path(path).with_suffix(".polymesh_tmp")
+    write_helix_case(tmp, core, ring, nslab, diameter, coil_radius_d, pitch_d, turns)
+    pts, tets = helical_pipe(core, ring, nslab, diameter, coil_radius_d, pitch_d, turns)
+    faces = np.asarray(_read_foam_faces(tmp / "faces"))
+    patches = _read_foam_boundary(tmp / "boundary")
+    groups = {}
+    for name, start, count in patches:
+        groups[name] = faces[start:start + count]
+    hxt1.write_msh2(path, pts, tets, groups["patch_2_0"], groups["patch_1_0"], groups["patch_0_0"])
+    import shutil
+    shutil.rmtree(tmp)
+    return Path(path)
+
+
+def _read_foam_faces(path: Path):
+    out = []
+    for line in Path(path).read_text().splitlines():
+        line = line.strip()
+        if line.startswith("3(") and line.endswith(")"):
+            out.append([int(v) for v in line[2:-1].split()])
+    return out
+
+
+def _read_foam_boundary(path: Path):
+    text = Path(path).read_text()
+    out = []
+    import re
+    for m in re.finditer(r"(\w+)\s*\{[^}]*?nFaces\s+(\d+);[^}]*?startFace\s+(\d+);", text):
+        out.append((m.group(1), int(m.group(3)), int(m.group(2))))
+    return out
+
+
 def write_cavity_case(directory: os.PathLike, n: int = 16) -> Path:
     """Lid-driven cavity polyMesh: ``lid`` at z=1 and ``walls`` elsewhere."""
     pts, tets = kuhn_cube(n)
