@@ -111,6 +111,28 @@ int nodals_h8_pressure_solve(nodals_h8* h, const char* solver, double rtol, doub
 /* p += alpha_p dp (h8_main_base.cu:2102). */
 int nodals_h8_pressure_update(nodals_h8* h, double alpha_p);
 
+/* hxt5e::momentum_solve (turbulence/sst_gpu_v2/source/
+ * puretet_profile5_eq5_hxt5b/hxt5e_fgmres.cuh:480-527) in its fixed-sweep
+ * modes on the assembled momentum system: "m3-fgs" / "m3-sgs" (the
+ * wall-function path's 3x3 block sweeps, nodals_h8_wall_assemble first) or
+ * "fgs1" / "sgs1" / "altgs1" (the H8 component sweeps, nodals_h8_assemble
+ * first), `sweeps` times with damping omega.  res: its = sweeps, r0 / rel =
+ * ||b - A u|| / ||b|| before / after (largest component for H8). */
+int nodals_h8_momentum_solve(nodals_h8* h, const char* mode, int32_t sweeps, double omega, nodals_solve_result* res);
+/* hxt4c::pcg_amg (hybrid_amg.cuh:642-698): PCG preconditioned by the AMG
+ * V-cycle on the live pressure operator (the fine Schur matrix, or the
+ * wall path's constrained Schur operator) for a host right-hand side rhs
+ * [cells]; the solution x [cells] is written to host memory.  The AMG fine
+ * level is the one of the last refresh.  No snapshot POD. */
+int nodals_h8_pcg_amg(nodals_h8* h, const double* rhs, double* x, double rtol, double atol, int32_t maxit,
+                      nodals_solve_result* res);
+/* hxt5c::scalar_solve (modules/transport/scalar_linear_solver.inc:3-136)
+ * for equation 0 (k) or 1 (omega) at the current velocity: the equation's
+ * masked-P1 assembly and its fixed multicolour sweeps (fgs-fixed), as in
+ * one nodals_h8_turb_step.  res: its = sweeps, r0 / rel = relative
+ * residual before / after the sweeps. */
+int nodals_h8_scalar_solve(nodals_h8* h, int32_t equation, nodals_solve_result* res);
+
 /* The SIMPLE loop of h8_main_base.cu:2046-2119 with its marker lines. */
 int nodals_h8_run(nodals_h8* h, nodals_h8_run_result* res);
 /* Wall-function path: `count` SIMPLE iterations numbered first_it ..
@@ -374,6 +396,16 @@ int nodals_wall_fit_tangential(int64_t n, const double* samples, const double* y
  * share that array and race. */
 int nodals_wall_profile5_noalloc(int64_t n, const double* U, const double* U_signed, const double* y, double nu,
                                  double* out, int32_t* valid);
+/* Q1+BF2 hexahedral element blocks (the SST application's 14-DOF hex:
+ * modules/foundation/hxt2_foundation.inc:166-225 hex_q1_ref, hex_basis_ref,
+ * hex_metric, phys_grad14, hex_volume_kernel; hxt4b_foundation.inc:246-345
+ * hex_conv_assemble_cached), 3x3x3 Gauss: coords[n][8][3] (vertex order of
+ * HexConn), u[n][14][3] (8 vertices, then the face bubbles z-, z+, y-, x+,
+ * y+, x-); diff[n][14][14] = nu * sum_q det w grad(phi_a).grad(phi_b),
+ * conv[n][14][14] = sum_q det w phi_a (u_q . grad(phi_b)); bad = number of
+ * Gauss points with a non-positive Jacobian (their terms are left out). */
+int nodals_hex_element_blocks(int64_t n, const double* coords, const double* u, double nu, double* diff,
+                              double* conv, int32_t* bad);
 /* OpenFOAM-style omega wall target (evaluate_of_auto_host,
  * modules/wall/portable/wall_host.hpp:5-9): in[n][15] = y nu Ut uTau k omega
  * kFloor omegaFloor kappa E beta1 reBlend prodLimit pk nut; shared[n];

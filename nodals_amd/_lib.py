@@ -49,14 +49,15 @@ EXPORTS = [
     "nodals_last_error", "nodals_version",
     "nodals_h8_create", "nodals_h8_create_host", "nodals_h8_destroy", "nodals_h8_info", "nodals_h8_assemble", "nodals_h8_momentum_work",
     "nodals_h8_momentum_audit", "nodals_h8_continuity", "nodals_h8_pressure_refresh", "nodals_h8_pressure_solve",
-    "nodals_h8_pressure_update", "nodals_h8_run", "nodals_h8_run_window", "nodals_h8_write_vtu", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
+    "nodals_h8_pressure_update", "nodals_h8_momentum_solve", "nodals_h8_pcg_amg", "nodals_h8_scalar_solve",
+    "nodals_h8_run", "nodals_h8_run_window", "nodals_h8_write_vtu", "nodals_h8_get", "nodals_h8_set", "nodals_h8_fine_spmv",
     "nodals_h8_precond",
     "nodals_h8_time_kernel", "nodals_h8_probe",
     "nodals_sys_create", "nodals_sys_create_host", "nodals_sys_get", "nodals_sys_destroy", "nodals_sys_info",
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
     "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
-    "nodals_wall_profile5_noalloc", "nodals_wall_fit_tangential",
+    "nodals_wall_profile5_noalloc", "nodals_wall_fit_tangential", "nodals_hex_element_blocks",
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
     "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
@@ -89,6 +90,9 @@ def lib() -> C.CDLL:
     L.nodals_h8_pressure_refresh.argtypes = [vp]
     L.nodals_h8_pressure_solve.argtypes = [vp, C.c_char_p, C.c_double, C.c_double, C.c_int32, C.POINTER(SolveResult)]
     L.nodals_h8_pressure_update.argtypes = [vp, C.c_double]
+    L.nodals_h8_momentum_solve.argtypes = [vp, C.c_char_p, C.c_int32, C.c_double, C.POINTER(SolveResult)]
+    L.nodals_h8_pcg_amg.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int32, C.POINTER(SolveResult)]
+    L.nodals_h8_scalar_solve.argtypes = [vp, C.c_int32, C.POINTER(SolveResult)]
     L.nodals_h8_run.argtypes = [vp, C.POINTER(RunResult)]
     L.nodals_h8_run_window.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(RunResult)]
     L.nodals_h8_write_vtu.argtypes = [vp, C.c_char_p]
@@ -105,6 +109,7 @@ def lib() -> C.CDLL:
     L.nodals_wall_omega_target.argtypes = [C.c_int64, vp, vp, vp]
     L.nodals_wall_profile5_noalloc.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_fit_tangential.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp]
+    L.nodals_hex_element_blocks.argtypes = [C.c_int64, vp, vp, C.c_double, vp, vp, vp]
     L.nodals_gnn_create_for_h8.argtypes = [vp, C.c_uint32, C.POINTER(vp)]
     L.nodals_gnn_destroy.argtypes = [vp]
     L.nodals_gnn_info.argtypes = [vp, vp]
@@ -226,6 +231,24 @@ class H8:
 
     def pressure_update(self, alpha_p: float) -> None:
         _check(lib().nodals_h8_pressure_update(self._h, alpha_p))
+
+    def momentum_solve(self, mode: str, sweeps: int, omega: float) -> Dict[str, float]:
+        r = SolveResult()
+        _check(lib().nodals_h8_momentum_solve(self._h, mode.encode(), int(sweeps), float(omega), C.byref(r)))
+        return r.as_dict()
+
+    def pcg_amg(self, rhs, rtol: float, atol: float, maxit: int):
+        """(x, result) of PCG-AMG on the live pressure operator."""
+        b = _f64(rhs)
+        x = np.empty_like(b)
+        r = SolveResult()
+        _check(lib().nodals_h8_pcg_amg(self._h, _ptr(b), _ptr(x), rtol, atol, int(maxit), C.byref(r)))
+        return x, r.as_dict()
+
+    def scalar_solve(self, equation: int) -> Dict[str, float]:
+        r = SolveResult()
+        _check(lib().nodals_h8_scalar_solve(self._h, int(equation), C.byref(r)))
+        return r.as_dict()
 
     def run(self) -> Dict[str, float]:
         r = RunResult()
@@ -507,6 +530,18 @@ def wall_fit_tangential(samples, y, nrm, pic, nu, count, zero_limit=True):
     _check(lib().nodals_wall_fit_tangential(n, _ptr(s), _ptr(yy), _ptr(nn), _ptr(pp), nu, int(count),
                                             1 if zero_limit else 0, _ptr(out), _ptr(valid)))
     return out, valid
+
+
+def hex_element_blocks(coords, u, nu):
+    """Q1+BF2 hexahedral diffusion and convection blocks on the GPU:
+    coords (n,8,3), u (n,14,3) -> (diff (n,14,14), conv (n,14,14), bad)."""
+    n = int(np.asarray(coords).size // 24)
+    X, U = _f64(coords, (n, 8, 3)), _f64(u, (n, 14, 3))
+    D = np.empty((n, 14, 14))
+    Cv = np.empty((n, 14, 14))
+    bad = C.c_int32(0)
+    _check(lib().nodals_hex_element_blocks(n, _ptr(X), _ptr(U), float(nu), _ptr(D), _ptr(Cv), C.byref(bad)))
+    return D, Cv, int(bad.value)
 
 
 def wall_omega_target(inp, shared):

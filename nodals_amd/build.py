@@ -38,7 +38,7 @@ DEVICE_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
 HOST_SOURCES = ["host/mesh.cpp", "host/h8_setup.cpp", "host/amg_setup.cpp", "host/turb_setup.cpp",
                 "host/wall_setup.cpp", "host/potential_start.cpp"]
 DEVICE_SOURCES = ["device/h8_solver.hip", "device/sys_solver.hip", "device/sst_kernels.hip", "device/gnn.hip", "device/turb_step.hip",
-                   "device/wall_solver.hip", "device/capi.hip"]
+                   "device/wall_solver.hip", "device/hex_kernels.hip", "device/capi.hip"]
 
 
 def _run(cmd, verbose):

@@ -75,6 +75,13 @@ void fillResult(const nodals::dev::KrylovResult& r, nodals_solve_result* res) {
 
 }  // namespace
 
+namespace nodals {
+namespace dev {
+void hexElementBlocks(long long n, const double* coords, const double* u, double nu, double* diff, double* conv,
+                      int* badOut);
+}  // namespace dev
+}  // namespace nodals
+
 extern "C" {
 
 const char* nodals_last_error(void) { return g_error.c_str(); }
@@ -159,6 +166,22 @@ int nodals_h8_pressure_solve(nodals_h8* h, const char* solverName, double rtol, 
     const auto r = solver(h).pressureSolve(solverName, rtol, atol, maxit);
     fillResult(r, res);
   });
+}
+
+int nodals_h8_momentum_solve(nodals_h8* h, const char* mode, int32_t sweeps, double omega, nodals_solve_result* res) {
+  return guarded([&]() {
+    nodals::require(mode != nullptr, "null momentum mode");
+    fillResult(solver(h).momentumSolve(mode, sweeps, omega), res);
+  });
+}
+
+int nodals_h8_pcg_amg(nodals_h8* h, const double* rhs, double* x, double rtol, double atol, int32_t maxit,
+                      nodals_solve_result* res) {
+  return guarded([&]() { fillResult(solver(h).pcgAmg(rhs, x, rtol, atol, maxit), res); });
+}
+
+int nodals_h8_scalar_solve(nodals_h8* h, int32_t equation, nodals_solve_result* res) {
+  return guarded([&]() { fillResult(solver(h).scalarSolve(equation), res); });
 }
 
 int nodals_h8_pressure_update(nodals_h8* h, double alpha_p) {
@@ -586,6 +609,15 @@ int nodals_wall_fit_tangential(int64_t n, const double* samples, const double* y
                                const double* pic, double nu, int32_t count, int32_t zero_limit, double* out,
                                int32_t* valid) {
   return guarded([&]() { nodals::fitTangentialBatch(n, samples, y, nrm, pic, nu, count, zero_limit, out, valid); });
+}
+
+int nodals_hex_element_blocks(int64_t n, const double* coords, const double* u, double nu, double* diff,
+                              double* conv, int32_t* bad) {
+  return guarded([&]() {
+    int b = 0;
+    nodals::dev::hexElementBlocks(n, coords, u, nu, diff, conv, &b);
+    if (bad) *bad = b;
+  });
 }
 
 int nodals_wall_omega_target(int64_t n, const double* in, const int32_t* shared, double* out) {

@@ -132,6 +132,19 @@ class H8Solver {
   double continuity();
   void refresh();
   dev::KrylovResult pressureSolve(const std::string& solver, double rtol, double atol, int maxit);
+  // hxt5e::momentum_solve (hxt5e_fgmres.cuh:480-527) in its fixed-sweep
+  // modes: "m3-fgs" / "m3-sgs" on the wall path's block system, "fgs1" /
+  // "sgs1" / "altgs1" on the H8 momentum; returns sweeps and the relative
+  // residuals before (r0) and after (rel).
+  dev::KrylovResult momentumSolve(const std::string& mode, int sweeps, double omega);
+  // hxt4c::pcg_amg (hybrid_amg.cuh:642-698): PCG with the AMG preconditioner
+  // on the live pressure operator for a host right-hand side; x (host) in
+  // and out.
+  dev::KrylovResult pcgAmg(const double* rhs, double* x, double rtol, double atol, int maxit);
+  // hxt5c::scalar_solve (scalar_linear_solver.inc:3-136) for k (0) or omega
+  // (1) at the current velocity: the equation's assembly and its fixed
+  // sweeps, with the relative residuals before and after the sweeps.
+  dev::KrylovResult scalarSolve(int equation);
   // The same from an initial guess (device, cells; fgmres only).
   // x0: initial guess; z0: first Krylov direction (fgmres only).
   dev::KrylovResult pressureSolveFrom(const std::string& solver, double rtol, double atol, int maxit,
@@ -225,7 +238,11 @@ class H8Solver {
   // Fit, wall blocks, component rows and relaxation, K, T^T b.
   void wallAssemble();
   // Q u, the block sweeps, T u.
-  void wallMomentum();
+  // The M3 block sweeps; sweeps / omega / symmetric < 0 take the wall
+  // options; rel0 / rel1, when given, receive ||b - K x|| / ||b|| of the
+  // block system before and after the sweeps.
+  void wallMomentum(int sweeps = -1, double omega = -1.0, int symmetric = -1, double* rel0 = nullptr,
+                    double* rel1 = nullptr);
   // y = B T Q R Q T^T B^T x on device vectors of length cells.
   void wallSchur(hipStream_t s, const double* x, double* y);
   // u += T Q R Q T^T B^T dp for a device vector dp (cells).
@@ -316,7 +333,7 @@ class H8Solver {
   DBuf<double> sstXY_, sstQuad_;
   void sstGeometry();
   // One k (equation 0) or omega (1) update of the transport step.
-  int turbEquation(int equation);
+  int turbEquation(int equation, double* rel0 = nullptr, double* rel1 = nullptr);
   void turbAdvance();       // convection + k + omega, queued on the stream
   void eddyDiffusion();     // adds the nu_t diffusion blocks to cbuf_
   int turbCounters(TurbStepResult* R);  // reads the step's counters (syncs)

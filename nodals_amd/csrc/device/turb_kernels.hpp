@@ -281,5 +281,20 @@ static __global__ void k_turb_floor(int n, double floorv, double* __restrict__ x
   }
 }
 
+// Residual r = b - (offdiag(A) x + d x) of a scalar system in the form the
+// multicolour sweep solves (the stored diagonal entry replaced by d), one
+// thread per row.
+static __global__ void k_scalar_resid(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                      const double* __restrict__ val, const double* __restrict__ d,
+                                      const double* __restrict__ b, const double* __restrict__ x,
+                                      double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = b[i] - d[i] * x[i];
+  for (int k = row[i]; k < row[i + 1]; ++k)
+    if (col[k] != i) s -= val[k] * x[col[k]];
+  r[i] = s;
+}
+
 }  // namespace dev
 }  // namespace nodals

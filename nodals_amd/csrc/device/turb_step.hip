@@ -149,7 +149,7 @@ void H8Solver::turbInit(const TurbParams& P) {
   turb_ = std::move(st);
 }
 
-int H8Solver::turbEquation(int eq) {
+int H8Solver::turbEquation(int eq, double* rel0, double* rel1) {
   TurbState& st = *turb_;
   const TurbTopo& T = st.T;
   const TurbParams& P = st.P;
@@ -200,6 +200,16 @@ int H8Solver::turbEquation(int eq) {
   else NODALS_TURB_GATHER(8);
 #undef NODALS_TURB_GATHER
   NODALS_LAUNCHED();
+  auto relResid = [&]() {
+    if (st.resid.size() < (std::size_t)T.nvert) st.resid.alloc((std::size_t)std::max(T.nvert, 1));
+    hipLaunchKernelGGL(k_scalar_resid, blocksFor(T.nvert), 256, 0, s, T.nvert, st.row.data(), st.col.data(),
+                       st.val.data(), st.diag.data(), st.rhs.data(), x, st.resid.data());
+    NODALS_LAUNCHED();
+    const double rr = red_.dot(s, T.nvert, st.resid.data(), st.resid.data());
+    const double bb = red_.dot(s, T.nvert, st.rhs.data(), st.rhs.data());
+    return std::sqrt(rr) / std::max(std::sqrt(bb), 1e-300);
+  };
+  if (rel0) *rel0 = relResid();
   // the fixed sweeps (sweeps x colours launches on fixed buffers) replay
   // from one captured graph per equation
   const double meanRow = (double)T.col.size() / std::max(T.nvert, 1);
@@ -212,10 +222,31 @@ int H8Solver::turbEquation(int eq) {
                                   st.val.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
       }
   });
+  if (rel1) *rel1 = relResid();
   hipLaunchKernelGGL(k_turb_floor, blocksFor(T.nvert), 256, 0, s, T.nvert, eq == 0 ? P.kFloor : P.omegaFloor, x,
                      st.counters.data() + eq);
   NODALS_LAUNCHED();
   return nb;
+}
+
+KrylovResult H8Solver::scalarSolve(int equation) {
+  requireDevice();
+  require((bool)turb_, "scalar_solve needs the k-omega model (nodals_h8_turb_init)", kErrInvalid);
+  require(equation == 0 || equation == 1, "scalar_solve: equation 0 (k) or 1 (omega)", kErrInvalid);
+  TurbState& st = *turb_;
+  hipLaunchKernelGGL(k_fill<int>, 1, 256, 0, stream_, 4ll, st.counters.data(), 0);
+  NODALS_LAUNCHED();
+  const int waves = std::min(nc_, 8192);
+  hipLaunchKernelGGL(k_conv_local, blocksFor((long long)waves * 64), 256, 0, stream_, nc_, cref_.data(),
+                     cinvJ_.data(), cdet_.data(), u_.data(), fixedVal_.data(), centT_.data(), st.conv.data());
+  NODALS_LAUNCHED();
+  KrylovResult r;
+  turbEquation(equation, &r.r0, &r.rel);
+  TurbStepResult tr;
+  turbCounters(&tr);
+  r.its = equation == 0 ? st.P.kSweeps : st.P.omegaSweeps;
+  r.ok = std::isfinite(r.rel);
+  return r;
 }
 
 void H8Solver::turbAdvance() {

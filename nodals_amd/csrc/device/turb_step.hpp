@@ -43,6 +43,7 @@ struct TurbState {
   DBuf<int> fcell, flf, scell, counters;
   DBuf<double> fgeom, wgeom, sl, sphi, coef, target, conv, blk, brhs, val, diag, rhs, k, w;
   DBuf<double> vdist;  // vertex wall distances (duct mode), empty for the straight pipe
+  DBuf<double> resid;  // residual of a scalar_solve norm
   const double* vdistPtr() const { return vdist.size() ? vdist.data() : nullptr; }
   std::vector<int> colorOff;
   GraphCache sweepGraph[2];  // k, omega

@@ -654,5 +654,26 @@ static __global__ void k_wall_shear(int nf, const int* __restrict__ wcell, const
   for (int d = 0; d < 4; ++d) out[4 * f + d] = t[d];
 }
 
+// Residual r = b - K x of the M3 block system (every 3x3 block of the row,
+// the diagonal one included), one thread per row; for residual norms.
+static __global__ void k_m3_resid(int n, const int* __restrict__ mrow, const int* __restrict__ mcol,
+                                  const double* __restrict__ K, const double* __restrict__ b,
+                                  const double* __restrict__ x, double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s0 = b[3ll * i], s1 = b[3ll * i + 1], s2 = b[3ll * i + 2];
+  for (int k = mrow[i]; k < mrow[i + 1]; ++k) {
+    const long long j = mcol[k];
+    const double* Kk = K + 9ll * k;
+    const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+    s0 -= Kk[0] * x0 + Kk[1] * x1 + Kk[2] * x2;
+    s1 -= Kk[3] * x0 + Kk[4] * x1 + Kk[5] * x2;
+    s2 -= Kk[6] * x0 + Kk[7] * x1 + Kk[8] * x2;
+  }
+  r[3ll * i] = s0;
+  r[3ll * i + 1] = s1;
+  r[3ll * i + 2] = s2;
+}
+
 }  // namespace dev
 }  // namespace nodals

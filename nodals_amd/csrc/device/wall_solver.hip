@@ -319,16 +319,28 @@ void H8Solver::wallAssemble() {
   }
 }
 
-void H8Solver::wallMomentum() {
+void H8Solver::wallMomentum(int sweeps, double omega, int symmetric, double* rel0, double* rel1) {
   requireDevice();
   require((bool)wall_, "wall-function path not initialised (nodals_h8_wall_init)", kErrInvalid);
   WallState& w = *wall_;
   const int nf = w.W.nf;
   hipStream_t s = stream_;
+  const int nSweeps = sweeps >= 0 ? sweeps : w.P.m3Sweeps;
+  const double om = omega > 0.0 ? omega : w.P.m3Omega;
+  const bool sym = symmetric >= 0 ? symmetric != 0 : w.P.m3Symmetric;
   if (nf > 0) {  // Q: the parameter vector carries no bubble-normal part
     hipLaunchKernelGGL(k_wall_t, blocksFor(nf), 256, 0, s, nf, w.bub.data(), w.vert.data(), w.wn.data(), u_.data(), 1);
     NODALS_LAUNCHED();
   }
+  auto relResid = [&]() {
+    hipLaunchKernelGGL(k_m3_resid, blocksFor(nv_), 256, 0, s, nv_, mrow_.data(), mcol_.data(), w.K.data(),
+                       brhs_.data(), u_.data(), w.y3.data());
+    NODALS_LAUNCHED();
+    const double rr = red_.dot(s, 3 * nv_, w.y3.data(), w.y3.data());
+    const double bb = red_.dot(s, 3 * nv_, brhs_.data(), brhs_.data());
+    return std::sqrt(rr) / std::max(std::sqrt(bb), 1e-300);
+  };
+  if (rel0) *rel0 = relResid();
   const int ncol = (int)colorOff_.size() - 1;
   auto pass = [&](bool backward) {
     for (int cc = 0; cc < ncol; ++cc) {
@@ -341,22 +353,22 @@ void H8Solver::wallMomentum() {
         hipExtLaunchKernelGGL((k_m3_gs<kLongLanes, kShortLanes>), dim3(nbL + nbS), dim3(256), 0, s,
                               probeEv_[2 * (std::size_t)probeUsed_], probeEv_[2 * (std::size_t)probeUsed_ + 1], 0,
                               r0, rs, r1, nbL, mrow_.data(), mcol_.data(), mdiagPos_.data(), w.K.data(),
-                              brhs_.data(), u_.data(), w.bubFace.data(), w.wn.data(), w.P.m3Omega,
-                              w.counters.data() + 3);
+                              brhs_.data(), u_.data(), w.bubFace.data(), w.wn.data(), om, w.counters.data() + 3);
         NODALS_LAUNCHED();
         if (++probeUsed_ == kProbePairs) probeHarvest();
       } else {
         hipLaunchKernelGGL((k_m3_gs<kLongLanes, kShortLanes>), nbL + nbS, 256, 0, s, r0, rs, r1, nbL, mrow_.data(),
                            mcol_.data(), mdiagPos_.data(), w.K.data(), brhs_.data(), u_.data(), w.bubFace.data(),
-                           w.wn.data(), w.P.m3Omega, w.counters.data() + 3);
+                           w.wn.data(), om, w.counters.data() + 3);
         NODALS_LAUNCHED();
       }
     }
   };
-  for (int k = 0; k < w.P.m3Sweeps; ++k) {
+  for (int k = 0; k < nSweeps; ++k) {
     pass(false);
-    if (w.P.m3Symmetric) pass(true);
+    if (sym) pass(true);
   }
+  if (rel1) *rel1 = relResid();
   if (nf > 0) {  // T: back to the physical velocity
     hipLaunchKernelGGL(k_wall_t, blocksFor(nf), 256, 0, s, nf, w.bub.data(), w.vert.data(), w.wn.data(), u_.data(), 0);
     NODALS_LAUNCHED();
@@ -641,6 +653,47 @@ void H8Solver::writeVtu(const std::string& path) {
   std::fprintf(f, "</DataArray>\n</Cells>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
   const bool ok = std::fclose(f) == 0;
   require(ok, "error writing VTU file " + path);
+}
+
+}  // namespace nodals
+
+namespace nodals {
+
+KrylovResult H8Solver::momentumSolve(const std::string& mode, int sweeps, double omega) {
+  requireDevice();
+  require(sweeps >= 1 && sweeps <= 60 && omega > 0.0 && omega < 2.0, "momentum_solve: sweeps in 1..60, omega in (0,2)",
+          kErrInvalid);
+  KrylovResult r;
+  r.its = sweeps;
+  if (mode == "m3-fgs" || mode == "m3-sgs") {
+    require((bool)wall_, "momentum_solve m3 modes need the wall-function path (nodals_h8_wall_init)", kErrInvalid);
+    wallMomentum(sweeps, omega, mode == "m3-sgs" ? 1 : 0, &r.r0, &r.rel);
+    wallCounters(true);
+  } else {
+    require(!wall_, "the wall-function path's momentum is solved by the m3-fgs / m3-sgs modes", kErrInvalid);
+    const std::array<double, 3> a0 = momentumAudit();
+    momentumWork(mode, omega, 0, sweeps);
+    const std::array<double, 3> a1 = momentumAudit();
+    r.r0 = std::max(a0[0], std::max(a0[1], a0[2]));
+    r.rel = std::max(a1[0], std::max(a1[1], a1[2]));
+  }
+  r.ok = std::isfinite(r.rel);
+  return r;
+}
+
+KrylovResult H8Solver::pcgAmg(const double* rhs, double* x, double rtol, double atol, int maxit) {
+  requireDevice();
+  require(rhs != nullptr && x != nullptr && maxit >= 1, "pcg_amg: null vectors or maxit < 1", kErrInvalid);
+  DevOp A = [this](hipStream_t s, const double* v, double* y) {
+    if (wall_) wallSchur(s, v, y);
+    else fineApply(s, v, y);
+  };
+  DevOp M = [this](hipStream_t s, const double* v, double* z) { precondApply(s, v, z); };
+  NODALS_HIP(hipMemcpyAsync(cont_.data(), rhs, sizeof(double) * (std::size_t)nc_, hipMemcpyHostToDevice, stream_));
+  const KrylovResult r = pcg(stream_, red_, nc_, A, M, cont_.data(), rtol, atol, maxit, kw_);
+  NODALS_HIP(hipMemcpyAsync(x, kw_.x.data(), sizeof(double) * (std::size_t)nc_, hipMemcpyDeviceToHost, stream_));
+  sync();
+  return r;
 }
 
 }  // namespace nodals
