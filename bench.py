@@ -45,32 +45,47 @@ def solver_args(mesh: str, precision: str) -> list:
             "--amg-precision", precision, "--verbose", "0", "--tag", f"bench_{precision}"]
 
 
-def cpu_baseline(mesh: str, gpu_outer: int, iters: int) -> dict:
-    """Oracle SIMPLE iterations on the host; projected to the GPU's count."""
-    import numpy as np
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:  # pragma: no cover
-        threadpool_limits = None
+def cpu_baseline(mesh: str, gpu_outer: int, budget_s: float, dp_gpu: float) -> dict:
+    """The FP64 C++/OpenMP port of the same SIMPLE loop (oracle/cpu_h8.cpp)
+    on the host's cores, from rest, for the GPU's iteration count: chunks
+    of 50 iterations until the count or the time budget is reached; any
+    remainder is projected at the measured per-iteration rate."""
+    import os
+
     from nodals_amd._lib import H8
+    from oracle import cpubind
     from oracle import h8_oracle as O
 
     h = H8(solver_args(mesh, "fp64"), device=False)
     case = O.case_from_product(h)
     h.close()
-    O.fine_topology(case)  # setup part, not timed
-    ctx = threadpool_limits(1) if threadpool_limits else None
-    t0 = time.perf_counter()
-    O.simple(case, p_solver="fgmres", p_rtol=0.5, p_max=200, smoother=O.Smoother(kind="jacobi"),
-             run_mode="fixed10", max_outer=iters)
-    dt = time.perf_counter() - t0
-    if ctx is not None:
-        ctx.__exit__(None, None, None)
-    per_it = dt / iters
-    return {"value": per_it * gpu_outer, "unit": "s", "cores": 1, "kind": "port",
-            "sample": (f"{iters} SIMPLE iterations of the same 40,320-tet pipe by the FP64 numpy/scipy oracle "
-                       f"({dt:.1f} s, {per_it:.3f} s/iteration), projected to the GPU's {gpu_outer} iterations "
-                       f"to converge")}
+    cs = cpubind.CpuSimple(case)  # gather lists, P^T: setup, not timed
+    usable = cpubind.usable_cores()
+    threads = min(usable, int(os.environ.get("OMP_NUM_THREADS") or usable))
+    U, p = None, None
+    done, dt = 0, 0.0
+    while done < gpu_outer and dt < budget_s:
+        n = min(50, gpu_outer - done)
+        t0 = time.perf_counter()
+        r = cs.simple(n, threads=threads, U0=U, p0=p)
+        dt += time.perf_counter() - t0
+        U, p, done = r["U"], r["p"], done + n
+    cs.close()
+    per_it = dt / done
+    total = per_it * gpu_outer
+    dp_cpu = O.pressure_drop_fit(case, p)
+    full = done == gpu_outer
+    tail = (f"all {gpu_outer} iterations timed; its pressure drop {dp_cpu:.6f} vs the GPU FP64 run's {dp_gpu:.6f}"
+            if full else f"{done} of the GPU's {gpu_outer} iterations timed, the rest projected at that rate")
+    print(f"[bench] CPU port: {threads} threads, {done} its in {dt:.1f} s ({per_it * 1e3:.1f} ms/it)",
+          file=sys.stderr, flush=True)
+    return {"value": total, "unit": "s", "cores": threads, "kind": "port",
+            "cpu_model": cpubind.cpu_model(), "usable_cores": usable, "ms_per_iteration": per_it * 1e3,
+            "iterations_timed": done, "projected": not full,
+            "pressure_drop": dp_cpu if full else None,
+            "sample": (f"FP64 C++/OpenMP port of the same SIMPLE loop (FGS momentum, FGMRES(30) + SA V-cycle with "
+                       f"damped-Jacobi smoothing, p-rtol 0.5) on the same 40,320-tet pipe from rest, {threads} "
+                       f"threads of {cpubind.cpu_model()}; {tail}")}
 
 
 def poisson_leg(n: int, reps: int) -> dict:
@@ -471,7 +486,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="time budget of the CPU baseline leg; iterations beyond it are projected")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=500)
     ap.add_argument("--no-poisson", action="store_true")
@@ -596,7 +612,7 @@ def main() -> int:
         learned["speedup_vs_fp64"] = float(np.mean(fp64["times"])) / learned["s"]
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(mesh, int(fp64["runs"][-1]["outer"]), a.cpu_iters)
+        cpu = cpu_baseline(mesh, int(fp64["runs"][-1]["outer"]), a.cpu_seconds, dp64)
     traffic = traffic_per_launch()
     line = {
         "metric": METRIC,
