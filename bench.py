@@ -339,6 +339,10 @@ def config5_args(mesh: str, precision: str, max_outer: int, tol: float) -> list:
 
 
 CONFIG5_WALL = ["--convection-form", "advective"]
+# the spiral case's wall sampling: one Spalding point (the third), ray traced
+# with local shortening ([wall_sampling] mode = ray_trace, samples = 1)
+CONFIG5_SAMPLING = ["--sst-wall-sampling", "ray_trace", "--sst-wall-active-points", "1",
+                    "--sst-wall-ray-local-adapt", "1"]
 
 
 def yplus30_sample_scale(re: float) -> float:
@@ -355,7 +359,7 @@ def config5_context(mesh: str, precision: str, max_outer: int, tol: float):
     from nodals_amd._lib import H8
 
     h = H8(config5_args(mesh, precision, max_outer, tol))
-    h.turb_init(["--sst-wall-sample-scale", repr(yplus30_sample_scale(50000.0))])
+    h.turb_init(["--sst-wall-sample-scale", repr(yplus30_sample_scale(50000.0)), *CONFIG5_SAMPLING])
     h.wall_init(CONFIG5_WALL)
     return h
 

@@ -406,6 +406,24 @@ int nodals_wall_profile5_noalloc(int64_t n, const double* U, const double* U_sig
  * Gauss points with a non-positive Jacobian (their terms are left out). */
 int nodals_hex_element_blocks(int64_t n, const double* coords, const double* u, double nu, double* diff,
                               double* conv, int32_t* bad);
+/* Host only (no GPU): wall-normal segments traced through a tetrahedral mesh
+ * by face-connected traversal (the SST application's ray-traced wall
+ * sampling, turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/modules/
+ * wall/portable/tet_ray_trace.hpp TetRayTrace::trace, used by
+ * modules/wall/tet_wall_ray_cache.inc:3-205).  points[np][3], tets[nt][4];
+ * ray r starts at origin[r] inside cell start[r], runs along dir[r] and is
+ * sampled at the ndist increasing distances dist[r][ndist].  Per (ray,
+ * sample): cell (-1 for a failed ray), bary[4] (barycentric in that cell),
+ * crossings (cells left before the sample).  status[r] = 0, or 1 when the
+ * segment leaves the fluid or cannot be followed.  certified (may be null):
+ * for a failed ray the distance, to dist[r][0] * 2^-28, up to which its
+ * first sample could be traced (the local shortening's bisection); for a
+ * traced ray its last distance.  Returns non-zero on invalid input (bad
+ * indices, degenerate or non-manifold tetrahedra). */
+int nodals_ray_trace(int64_t np, const double* points, int64_t nt, const int32_t* tets, int64_t nrays,
+                     const int32_t* start, const double* origin, const double* dir, int32_t ndist, const double* dist,
+                     int32_t max_crossings, double tol, int32_t* cell, double* bary, int32_t* crossings,
+                     int32_t* status, double* certified);
 /* OpenFOAM-style omega wall target (evaluate_of_auto_host,
  * modules/wall/portable/wall_host.hpp:5-9): in[n][15] = y nu Ut uTau k omega
  * kFloor omegaFloor kappa E beta1 reBlend prodLimit pk nut; shared[n];

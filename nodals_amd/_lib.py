@@ -57,7 +57,7 @@ EXPORTS = [
     "nodals_sys_spmv", "nodals_sys_precond",
     "nodals_sys_solve", "nodals_sys_solve_timed", "nodals_sys_set_rhs", "nodals_sys_time_spmv",
     "nodals_sst_local", "nodals_wall_spalding", "nodals_wall_profile5", "nodals_wall_omega_target",
-    "nodals_wall_profile5_noalloc", "nodals_wall_fit_tangential", "nodals_hex_element_blocks",
+    "nodals_wall_profile5_noalloc", "nodals_wall_fit_tangential", "nodals_hex_element_blocks", "nodals_ray_trace",
     "nodals_gnn_create_for_h8", "nodals_gnn_destroy", "nodals_gnn_info", "nodals_gnn_get_params",
     "nodals_gnn_set_params", "nodals_gnn_reserve", "nodals_gnn_sample", "nodals_gnn_forward", "nodals_gnn_gradient",
     "nodals_gnn_train", "nodals_gnn_evaluate", "nodals_h8_attach_gnn", "nodals_h8_sst_volume",
@@ -110,6 +110,8 @@ def lib() -> C.CDLL:
     L.nodals_wall_profile5_noalloc.argtypes = [C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.nodals_wall_fit_tangential.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp]
     L.nodals_hex_element_blocks.argtypes = [C.c_int64, vp, vp, C.c_double, vp, vp, vp]
+    L.nodals_ray_trace.argtypes = [C.c_int64, vp, C.c_int64, vp, C.c_int64, vp, vp, vp, C.c_int32, vp, C.c_int32,
+                                   C.c_double, vp, vp, vp, vp, vp]
     L.nodals_gnn_create_for_h8.argtypes = [vp, C.c_uint32, C.POINTER(vp)]
     L.nodals_gnn_destroy.argtypes = [vp]
     L.nodals_gnn_info.argtypes = [vp, vp]
@@ -542,6 +544,30 @@ def hex_element_blocks(coords, u, nu):
     bad = C.c_int32(0)
     _check(lib().nodals_hex_element_blocks(n, _ptr(X), _ptr(U), float(nu), _ptr(D), _ptr(Cv), C.byref(bad)))
     return D, Cv, int(bad.value)
+
+
+def ray_trace(points, tets, start, origin, direction, dist, max_crossings=4096, tol=1e-9):
+    """Host-only wall-normal segment traversal (no GPU): rays r from
+    origin[r] in cell start[r] along direction[r], sampled at dist[r][:].
+    Returns dict(cell (nr,nd), bary (nr,nd,4), crossings (nr,nd), status
+    (nr,), certified (nr,))."""
+    P = _f64(points, (-1, 3))
+    T = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    st = np.ascontiguousarray(start, dtype=np.int32).ravel()
+    nr = st.size
+    O, D = _f64(origin, (nr, 3)), _f64(direction, (nr, 3))
+    dist = np.asarray(dist, dtype=np.float64)
+    nd = dist.size // max(nr, 1) if nr else max(dist.size, 1)
+    dist = _f64(dist, (nr, nd)) if nr else np.zeros((0, nd))
+    cell = np.empty((nr, nd), np.int32)
+    bary = np.empty((nr, nd, 4))
+    cross = np.empty((nr, nd), np.int32)
+    status = np.empty(nr, np.int32)
+    cert = np.empty(nr)
+    _check(lib().nodals_ray_trace(P.shape[0], _ptr(P), T.shape[0], _ptr(T), nr, _ptr(st), _ptr(O), _ptr(D), nd,
+                                  _ptr(dist), int(max_crossings), float(tol), _ptr(cell), _ptr(bary), _ptr(cross),
+                                  _ptr(status), _ptr(cert)))
+    return {"cell": cell, "bary": bary, "crossings": cross, "status": status, "certified": cert}
 
 
 def wall_omega_target(inp, shared):

@@ -35,7 +35,7 @@ HOST_FLAGS = COMMON + ["-ffp-contract=off"]
 DEVICE_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                          "-munsafe-fp-atomics"]
 
-HOST_SOURCES = ["host/mesh.cpp", "host/h8_setup.cpp", "host/amg_setup.cpp", "host/turb_setup.cpp",
+HOST_SOURCES = ["host/mesh.cpp", "host/h8_setup.cpp", "host/amg_setup.cpp", "host/turb_setup.cpp", "host/ray_trace.cpp",
                 "host/wall_setup.cpp", "host/potential_start.cpp"]
 DEVICE_SOURCES = ["device/h8_solver.hip", "device/sys_solver.hip", "device/sst_kernels.hip", "device/gnn.hip", "device/turb_step.hip",
                    "device/wall_solver.hip", "device/hex_kernels.hip", "device/capi.hip"]

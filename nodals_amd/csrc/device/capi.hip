@@ -11,6 +11,7 @@
 #include "gnn.hpp"
 #include "sst_kernels.hpp"
 #include "sys_solver.hpp"
+#include "../host/ray_trace.hpp"
 #include "wall_state.hpp"
 
 struct nodals_h8 {
@@ -617,6 +618,48 @@ int nodals_hex_element_blocks(int64_t n, const double* coords, const double* u, 
     int b = 0;
     nodals::dev::hexElementBlocks(n, coords, u, nu, diff, conv, &b);
     if (bad) *bad = b;
+  });
+}
+
+int nodals_ray_trace(int64_t np, const double* points, int64_t nt, const int32_t* tets, int64_t nrays,
+                     const int32_t* start, const double* origin, const double* dir, int32_t ndist, const double* dist,
+                     int32_t max_crossings, double tol, int32_t* cell, double* bary, int32_t* crossings,
+                     int32_t* status, double* certified) {
+  return guarded([&]() {
+    nodals::require(np > 0 && nt > 0 && nrays >= 0 && ndist >= 1, "invalid ray-trace sizes");
+    nodals::require(points && tets && (nrays == 0 || (start && origin && dir && dist && cell && bary && crossings &&
+                                                     status)),
+                    "null arrays");
+    std::vector<std::array<double, 3>> P((std::size_t)np);
+    for (int64_t i = 0; i < np; ++i) P[(std::size_t)i] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    std::vector<std::array<int32_t, 4>> T((std::size_t)nt);
+    for (int64_t c = 0; c < nt; ++c) T[(std::size_t)c] = {tets[4 * c], tets[4 * c + 1], tets[4 * c + 2], tets[4 * c + 3]};
+    const nodals::TetRayTracer tracer(P, T);
+    for (int64_t r = 0; r < nrays; ++r) {
+      const std::array<double, 3> o = {origin[3 * r], origin[3 * r + 1], origin[3 * r + 2]};
+      const std::array<double, 3> d = {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+      const std::vector<double> ds(dist + r * ndist, dist + (r + 1) * ndist);
+      nodals::require(start[r] >= 0 && start[r] < nt, "ray start cell out of range");
+      try {
+        const auto hits = tracer.trace(start[r], o, d, ds, max_crossings, tol);
+        for (int m = 0; m < ndist; ++m) {
+          const auto& h = hits[(std::size_t)m];
+          cell[r * ndist + m] = h.cell;
+          crossings[r * ndist + m] = h.crossings;
+          for (int j = 0; j < 4; ++j) bary[(r * ndist + m) * 4 + j] = h.l[j];
+        }
+        status[r] = 0;
+        if (certified) certified[r] = ds.back();
+      } catch (const nodals::Error&) {
+        for (int m = 0; m < ndist; ++m) {
+          cell[r * ndist + m] = -1;
+          crossings[r * ndist + m] = 0;
+          for (int j = 0; j < 4; ++j) bary[(r * ndist + m) * 4 + j] = 0.0;
+        }
+        status[r] = 1;
+        if (certified) certified[r] = tracer.certifiedDistance(start[r], o, d, ds[0], max_crossings, tol);
+      }
+    }
   });
 }
 

@@ -148,7 +148,6 @@ static __global__ void __launch_bounds__(256) k_turb_inlet(
 
 struct WallConsts {
   double gamma, kappa, E, beta1, reBlend, prodLimit;
-  double y[5];
 };
 
 // Omega wall penalty coefficient and target of one wall quadrature point
@@ -159,15 +158,16 @@ struct WallConsts {
 // Invalid points get coeff 0 and are counted in *bad.
 static __global__ void __launch_bounds__(256) k_turb_wall_coef(
     int nq, const double* __restrict__ wgeom, const int* __restrict__ scell, const double* __restrict__ sl,
-    const double* __restrict__ sphi, const int* __restrict__ cref, const int* __restrict__ sref,
-    const double* __restrict__ cinvJ, const double* __restrict__ cxy, const double* __restrict__ u,
-    const double* __restrict__ ufix, const double* __restrict__ kf, const double* __restrict__ wf, TurbConsts K,
-    WallConsts W, Tri12 Q, double* __restrict__ coef, double* __restrict__ target, int* __restrict__ bad,
-    const double* __restrict__ vdist) {
+    const double* __restrict__ sphi, const double* __restrict__ sy, const int* __restrict__ cref,
+    const int* __restrict__ sref, const double* __restrict__ cinvJ, const double* __restrict__ cxy,
+    const double* __restrict__ u, const double* __restrict__ ufix, const double* __restrict__ kf,
+    const double* __restrict__ wf, TurbConsts K, WallConsts W, Tri12 Q, double* __restrict__ coef,
+    double* __restrict__ target, int* __restrict__ bad, const double* __restrict__ vdist) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nq) return;
   const double* G = wgeom + 4ll * (w / 12);
-  double U[5], Us[5];
+  double U[5], Us[5], Y[5];
+  for (int m = 0; m < 5; ++m) Y[m] = sy[(long long)w * 5 + m];
   bool ok = true;
   for (int m = 0; m < 5; ++m) {
     const long long s = (long long)w * 5 + m;
@@ -181,7 +181,7 @@ static __global__ void __launch_bounds__(256) k_turb_wall_coef(
     Us[m] = um;
     U[m] = fabs(um);
   }
-  const sst::Profile P = sst::fitProfile5(U, Us, W.y, K.nu);
+  const sst::Profile P = sst::fitProfile5(U, Us, Y, K.nu);
   const long long s3 = (long long)w * 5 + 2;
   const int c = scell[s3];
   const double l[4] = {sl[s3 * 4], sl[s3 * 4 + 1], sl[s3 * 4 + 2], sl[s3 * 4 + 3]};
@@ -191,9 +191,9 @@ static __global__ void __launch_bounds__(256) k_turb_wall_coef(
                                    K.kFloor, K.omFloor, K.deviatoric != 0);
   const double normal = us[0] * G[0] + us[1] * G[1] + us[2] * G[2];
   const double Ut = sqrt(fmax(0.0, us[0] * us[0] + us[1] * us[1] + us[2] * us[2] - normal * normal));
-  const sst::OmegaWall oa = sst::omegaWall(W.y[2], K.nu, Ut, P.uTau, kv, wvv, ss.pk, K.kFloor, K.omFloor, W.kappa,
+  const sst::OmegaWall oa = sst::omegaWall(Y[2], K.nu, Ut, P.uTau, kv, wvv, ss.pk, K.kFloor, K.omFloor, W.kappa,
                                            W.E, W.beta1, W.reBlend, W.prodLimit, true);
-  const double ell = W.y[4] - W.y[0];
+  const double ell = Y[4] - Y[0];
   const double nuEff = K.nu + ss.sigmaW * ss.nut;
   const double vweight = Q.q[w % 12][3] * G[3] * ell;
   const double cf = W.gamma * nuEff / ell * vweight / ell;

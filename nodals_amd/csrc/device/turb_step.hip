@@ -49,9 +49,20 @@ TurbParams parseTurbArgs(int argc, const char* const* argv) {
     else if (f == "--sst-of-beta1") P.ofBeta1 = num(f, v);
     else if (f == "--sst-of-re-blend") P.ofReBlend = num(f, v);
     else if (f == "--sst-of-production-limit") P.ofProdLimit = num(f, v);
+    else if (f == "--sst-wall-sampling") {
+      const std::string m = v;
+      require(m == "ray_trace" || m == "legacy_endpoint", "--sst-wall-sampling must be ray_trace or legacy_endpoint");
+      P.sampling = m == "ray_trace" ? 1 : 0;
+    } else if (f == "--sst-wall-ray-max-crossings") P.rayMaxCrossings = (int)num(f, v);
+    else if (f == "--sst-wall-ray-bary-tolerance") P.rayTol = num(f, v);
+    else if (f == "--sst-wall-ray-local-adapt") P.rayLocalAdapt = num(f, v) != 0.0;
+    else if (f == "--sst-wall-active-points") P.activePoints = (int)num(f, v);
     else fail("unknown k-omega option " + f);
   }
   require(P.gsOmega > 0 && P.gsOmega < 2, "--sst-gs-omega must be in (0, 2)");
+  require(P.rayMaxCrossings >= 1, "--sst-wall-ray-max-crossings must be positive");
+  require(P.rayTol > 0 && P.rayTol <= 1e-5, "--sst-wall-ray-bary-tolerance must be in (0, 1e-5]");
+  require(P.activePoints >= 1 && P.activePoints <= 5, "--sst-wall-active-points must be in 1..5");
   require(P.wallGamma > 0, "--sst-wall-penalty-gamma must be positive");
   require(P.intensity > 0 && P.lengthRatio > 0, "--sst-intensity and --sst-length-ratio must be positive");
   const bool anyY = P.heights[0] > 0 || P.heights[1] > 0 || P.heights[2] > 0 || P.heights[3] > 0 || P.heights[4] > 0;
@@ -109,24 +120,27 @@ void H8Solver::turbInit(const TurbParams& P) {
   st->fcell.upload(fc);
   st->flf.upload(fl);
   st->fgeom.upload(fg);
-  std::vector<double> wg, sl, sp;
+  std::vector<double> wg, sl, sp, sy;
   std::vector<int> sc;
   for (const TurbFace& f : T.wall) wg.insert(wg.end(), {f.n[0], f.n[1], f.n[2], f.area});
   for (const TurbSample& s : T.samples) {
     sc.push_back(s.cell);
     sl.insert(sl.end(), s.l, s.l + 4);
     sp.insert(sp.end(), s.phi, s.phi + 8);
+    sy.push_back(s.y);
   }
   if (sc.empty()) {
     wg.assign(4, 0.0);
     sc.push_back(0);
     sl.assign(4, 0.0);
     sp.assign(8, 0.0);
+    sy.assign(1, 0.0);
   }
   st->wgeom.upload(wg);
   st->scell.upload(sc);
   st->sl.upload(sl);
   st->sphi.upload(sp);
+  st->sy.upload(sy);
   st->coef.alloc((std::size_t)std::max(T.nWallQ, 1));
   st->target.alloc((std::size_t)std::max(T.nWallQ, 1));
   st->conv.alloc((std::size_t)nc_ * 64);
@@ -172,10 +186,9 @@ int H8Solver::turbEquation(int eq, double* rel0, double* rel1) {
   }
   int nb = nc_ + T.nInlet;
   if (eq == 1 && T.nWallQ > 0) {
-    WallConsts W{P.wallGamma, P.ofKappa, P.ofE, P.ofBeta1, P.ofReBlend, P.ofProdLimit, {0, 0, 0, 0, 0}};
-    for (int m = 0; m < 5; ++m) W.y[m] = T.heights[(std::size_t)m];
+    const WallConsts W{P.wallGamma, P.ofKappa, P.ofE, P.ofBeta1, P.ofReBlend, P.ofProdLimit};
     hipLaunchKernelGGL(k_turb_wall_coef, blocksFor(T.nWallQ), 256, 0, s, T.nWallQ, st.wgeom.data(), st.scell.data(),
-                       st.sl.data(), st.sphi.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(),
+                       st.sl.data(), st.sphi.data(), st.sy.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(),
                        u_.data(), fixedVal_.data(), st.k.data(), st.w.data(), K, W, tri12(), st.coef.data(),
                        st.target.data(), st.counters.data() + 2, st.vdistPtr());
     NODALS_LAUNCHED();
@@ -336,6 +349,14 @@ std::vector<double> H8Solver::turbField(const std::string& n) {
     return v;
   }
   if (n == "turb_heights") return std::vector<double>(T.heights.begin(), T.heights.end());
+  if (n == "turb_sample_y") {
+    std::vector<double> v;
+    for (const TurbSample& s : T.samples) v.push_back(s.y);
+    return v;
+  }
+  if (n == "turb_sample_stats")
+    return {(double)T.placed, (double)T.crossed, (double)T.maxCrossings, (double)T.shortened, T.minKeptFraction,
+            (double)T.rayMask, (double)st.P.sampling};
   if (n == "turb_inlet_state") return {T.kIn, T.omegaIn, (double)T.nInlet, (double)T.wall.size(), (double)T.nS,
                                        (double)T.nvert, (double)T.coloring.ncolors};
   fail("unknown k-omega field " + n);

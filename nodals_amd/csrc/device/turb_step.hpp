@@ -41,7 +41,7 @@ struct TurbState {
   TurbParams P;
   DBuf<int> sref, row, col, diagPos, entOff, termOff, term, rhsOff, rhsTerm, crows;
   DBuf<int> fcell, flf, scell, counters;
-  DBuf<double> fgeom, wgeom, sl, sphi, coef, target, conv, blk, brhs, val, diag, rhs, k, w;
+  DBuf<double> fgeom, wgeom, sl, sphi, sy, coef, target, conv, blk, brhs, val, diag, rhs, k, w;
   DBuf<double> vdist;  // vertex wall distances (duct mode), empty for the straight pipe
   DBuf<double> resid;  // residual of a scalar_solve norm
   const double* vdistPtr() const { return vdist.size() ? vdist.data() : nullptr; }

@@ -28,14 +28,16 @@ namespace nodals {
 namespace dev {
 
 // Face-averaged Spalding fit, one thread per wall face: the face-quadrature
-// means of the Picard functional and of the velocity at each active
-// wall-normal sample slot, then one tangential fit.  fit[f] = (beta, u_tau,
+// means of the Picard functional, of the velocity and of the wall distance
+// (sdist, per sample) at each active wall-normal sample slot, then one
+// tangential fit.  fit[f] = (beta, u_tau,
 // y+); beta = -1 and a raised counter for a face without a valid fit.
 static __global__ void k_wall_fit(int nf, const double* __restrict__ wn, const double* __restrict__ qw,
                                   const double* __restrict__ bp, const int* __restrict__ wcell,
                                   const int* __restrict__ cref, const double* __restrict__ u,
                                   const double* __restrict__ ufix, const int* __restrict__ scell,
-                                  const double* __restrict__ sphi, const double* __restrict__ heights, double nu,
+                                  const double* __restrict__ sphi, const double* __restrict__ heights,
+                                  const double* __restrict__ sdist, double nu,
                                   int activePoints, int zeroLimit, double* __restrict__ fit, int* __restrict__ bad) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
@@ -62,7 +64,7 @@ static __global__ void k_wall_fit(int nf, const double* __restrict__ wn, const d
     for (int m = 0; m < 5; ++m) {
       if (!(mask & (1u << m))) continue;
       const long long si = ((long long)f * 12 + q) * 5 + m;
-      yy[m] += w * heights[m];
+      yy[m] += w * sdist[si];
       const int sc = scell[si];
       for (int b = 0; b < 8; ++b) {
         const int r = cref[sc * 8 + b];

@@ -68,6 +68,14 @@ void H8Solver::wallInit(const WallParams& P) {
   require((bool)turb_, "the wall-function path needs the k-omega model (nodals_h8_turb_init) first", kErrInvalid);
   require(turb_->P.wall, "the wall-function path needs the k-omega wall faces (--sst-omega-wall-mode of_auto)",
           kErrInvalid);
+  {
+    // ray-traced placement covers only the slots that were traced
+    const unsigned fitMask = sst::activeMask(P.activePoints);
+    require((fitMask & ~turb_->T.rayMask) == 0,
+            "the Spalding fit uses wall samples that were not ray-traced: give the k-omega setup the same "
+            "active point count (--sst-wall-active-points)",
+            kErrInvalid);
+  }
   auto st = std::make_unique<WallState>();
   st->P = P;
   st->W = buildWallTopo(M_, S_, turb_->T);
@@ -253,7 +261,7 @@ void H8Solver::wallAssemble() {
   if (nf > 0) {
     hipLaunchKernelGGL(k_wall_fit, blocksFor(nf), 256, 0, s, nf, w.wn.data(), w.qw.data(), w.bp.data(),
                        w.wcell.data(), cref_.data(), u_.data(), fixedVal_.data(), st.scell.data(), st.sphi.data(),
-                       w.heights.data(), S_.pipe.nu, w.P.activePoints, w.P.zeroLimit ? 1 : 0, w.fit.data(),
+                       w.heights.data(), st.sy.data(), S_.pipe.nu, w.P.activePoints, w.P.zeroLimit ? 1 : 0, w.fit.data(),
                        w.counters.data());
     NODALS_LAUNCHED();
     hipLaunchKernelGGL(k_wall_nitsche, (nf + 3) / 4, 256, 0, s, nf, w.wcell.data(), w.wn.data(), w.wh.data(),

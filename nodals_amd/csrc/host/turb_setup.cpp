@@ -6,6 +6,7 @@
 #include <string>
 
 #include "error.hpp"
+#include "ray_trace.hpp"
 
 namespace nodals {
 
@@ -118,51 +119,123 @@ TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) 
   T.nInlet = (int)T.inlet.size();
   T.nWallQ = (int)T.wall.size() * 12;
 
-  // Wall-normal samples by barycentric adjacency walk from the wall cell.
-  auto across = [&](int c, int j) {
-    const int f = M.oppFace[(std::size_t)c][(std::size_t)j];
-    if (M.owner[(std::size_t)f] != c) return (int)M.owner[(std::size_t)f];
-    return f < M.nInternal() ? (int)M.neighbour[(std::size_t)f] : -1;
-  };
+  // Wall-normal samples.
+  require(P.sampling == 0 || P.sampling == 1, "wall sampling must be legacy_endpoint or ray_trace");
+  require(P.activePoints >= 1 && P.activePoints <= 5, "active wall points must be in 1..5");
   T.samples.resize((std::size_t)T.nWallQ * 5);
-  std::vector<int> seen((std::size_t)nc, 0);
-  int stamp = 0;
-  for (std::size_t fi = 0; fi < T.wall.size(); ++fi) {
-    const TurbFace& F = T.wall[fi];
+  auto quadPoint = [&](const TurbFace& F, int q) {
     const auto& t = M.tets[(std::size_t)F.cell];
-    for (int q = 0; q < 12; ++q) {
-      V3 X = {0, 0, 0};
-      for (int j = 0; j < 3; ++j) {
-        const V3& v = M.points[(std::size_t)t[(std::size_t)kTetFace[F.lf][j]]];
-        for (int d = 0; d < 3; ++d) X[(std::size_t)d] += kTri12[q][j] * v[(std::size_t)d];
-      }
-      for (int m = 0; m < 5; ++m) {
-        const double y = T.heights[(std::size_t)m];
-        const V3 p = {X[0] - y * F.n[0], X[1] - y * F.n[1], X[2] - y * F.n[2]};
-        int c = F.cell, found = -1, steps = 0;
-        double l[4] = {0, 0, 0, 0};
-        ++stamp;
-        while (c >= 0 && steps < 4096 && seen[(std::size_t)c] != stamp) {
-          seen[(std::size_t)c] = stamp;
-          ++steps;
-          if (!barycentric(M, c, p, l)) break;
-          int neg = 0;
-          for (int j = 1; j < 4; ++j)
-            if (l[j] < l[neg]) neg = j;
-          if (l[neg] >= -1e-8) {
-            found = c;
-            break;
+    V3 X = {0, 0, 0};
+    for (int j = 0; j < 3; ++j) {
+      const V3& v = M.points[(std::size_t)t[(std::size_t)kTetFace[F.lf][j]]];
+      for (int d = 0; d < 3; ++d) X[(std::size_t)d] += kTri12[q][j] * v[(std::size_t)d];
+    }
+    return X;
+  };
+  auto place = [&](TurbSample& s, int cell, const double l[4], double y) {
+    s.cell = cell;
+    s.y = y;
+    for (int j = 0; j < 4; ++j) s.l[j] = l[j];
+    basisAt(s.l, s.phi);
+  };
+  if (P.sampling == 0) {
+    // Adjacency walk from the wall cell towards each endpoint.
+    auto across = [&](int c, int j) {
+      const int f = M.oppFace[(std::size_t)c][(std::size_t)j];
+      if (M.owner[(std::size_t)f] != c) return (int)M.owner[(std::size_t)f];
+      return f < M.nInternal() ? (int)M.neighbour[(std::size_t)f] : -1;
+    };
+    std::vector<int> seen((std::size_t)nc, 0);
+    int stamp = 0;
+    for (std::size_t fi = 0; fi < T.wall.size(); ++fi) {
+      const TurbFace& F = T.wall[fi];
+      for (int q = 0; q < 12; ++q) {
+        const V3 X = quadPoint(F, q);
+        for (int m = 0; m < 5; ++m) {
+          const double y = T.heights[(std::size_t)m];
+          const V3 p = {X[0] - y * F.n[0], X[1] - y * F.n[1], X[2] - y * F.n[2]};
+          int c = F.cell, found = -1, steps = 0;
+          double l[4] = {0, 0, 0, 0};
+          ++stamp;
+          while (c >= 0 && steps < 4096 && seen[(std::size_t)c] != stamp) {
+            seen[(std::size_t)c] = stamp;
+            ++steps;
+            if (!barycentric(M, c, p, l)) break;
+            int neg = 0;
+            for (int j = 1; j < 4; ++j)
+              if (l[j] < l[neg]) neg = j;
+            if (l[neg] >= -1e-8) {
+              found = c;
+              break;
+            }
+            c = across(c, neg);
           }
-          c = across(c, neg);
+          require(found >= 0, "wall sample outside the mesh (face " + std::to_string(fi) + ", point " +
+                                  std::to_string(q) + ", height " + std::to_string(y) + ")");
+          place(T.samples[((std::size_t)fi * 12 + q) * 5 + m], found, l, y);
+          ++T.placed;
+          if (found != F.cell) ++T.crossed;
+          T.maxCrossings = std::max(T.maxCrossings, steps - 1);
         }
-        require(found >= 0, "wall sample outside the mesh (face " + std::to_string(fi) + ", point " +
-                                std::to_string(q) + ", height " + std::to_string(y) + ")");
-        TurbSample& s = T.samples[((std::size_t)fi * 12 + q) * 5 + m];
-        s.cell = found;
-        for (int j = 0; j < 4; ++j) s.l[j] = l[j];
-        basisAt(l, s.phi);
       }
     }
+    T.rayMask = 0x1f;
+  } else {
+    // Ray-traced active slots: the Spalding fit's points and the omega wall
+    // point (slot 3); inactive slots alias the first active slot's hit and
+    // keep their own height.
+    static const unsigned kFitMask[5] = {0x04u, 0x11u, 0x15u, 0x1bu, 0x1fu};
+    const unsigned mask = P.activePoints == 5 ? 0x1fu : (kFitMask[P.activePoints - 1] | 0x04u);
+    std::vector<int> slots;
+    std::vector<double> base;
+    for (int m = 0; m < 5; ++m)
+      if (mask & (1u << m)) {
+        slots.push_back(m);
+        base.push_back(T.heights[(std::size_t)m]);
+      }
+    require(!P.rayLocalAdapt || slots.size() == 1,
+            "ray local shortening needs exactly one active wall sample (one Spalding point at the omega slot)");
+    TetRayTracer tracer(M.points, M.tets);
+    for (std::size_t fi = 0; fi < T.wall.size(); ++fi) {
+      const TurbFace& F = T.wall[fi];
+      const std::array<double, 3> inward = {-F.n[0], -F.n[1], -F.n[2]};
+      for (int q = 0; q < 12; ++q) {
+        const V3 X = quadPoint(F, q);
+        std::vector<double> dist = base;
+        std::vector<RayHit> hits;
+        try {
+          hits = tracer.trace(F.cell, X, inward, dist, P.rayMaxCrossings, P.rayTol);
+        } catch (const Error& e) {
+          if (!P.rayLocalAdapt)
+            fail("wall sample ray (face " + std::to_string(fi) + ", point " + std::to_string(q) + ", slot " +
+                 std::to_string(slots[0] + 1) + "): " + e.what());
+          const double want = dist[0];
+          const double ok = tracer.certifiedDistance(F.cell, X, inward, want, P.rayMaxCrossings, P.rayTol);
+          const double chosen = std::min(want, 0.90 * ok);
+          require(chosen >= 0.25 * want && chosen < want,
+                  "wall sample ray cannot be shortened safely (face " + std::to_string(fi) + ", point " +
+                      std::to_string(q) + ", requested " + std::to_string(want) + ", certified " +
+                      std::to_string(ok) + "): " + e.what());
+          dist[0] = chosen;
+          hits = tracer.trace(F.cell, X, inward, dist, P.rayMaxCrossings, P.rayTol);
+          ++T.shortened;
+          T.minKeptFraction = std::min(T.minKeptFraction, chosen / want);
+        }
+        for (std::size_t k = 0; k < slots.size(); ++k) {
+          const RayHit& h = hits[k];
+          place(T.samples[((std::size_t)fi * 12 + q) * 5 + slots[k]], h.cell, h.l, dist[k]);
+          ++T.placed;
+          if (h.cell != F.cell) ++T.crossed;
+          T.maxCrossings = std::max(T.maxCrossings, (int)h.crossings);
+        }
+        for (int m = 0; m < 5; ++m)
+          if (!(mask & (1u << m))) {
+            const RayHit& h = hits[0];
+            place(T.samples[((std::size_t)fi * 12 + q) * 5 + m], h.cell, h.l, T.heights[(std::size_t)m]);
+          }
+      }
+    }
+    T.rayMask = mask;
   }
 
   // Blocks and their cells.

@@ -11,8 +11,9 @@
 // modules/transport/transport_preflight.inc + transport_finalize.inc
 // (assembly and row-L1 relaxation), scalar_linear_solver.inc:33-67 (fixed
 // coloured sweeps), tet_inlet_flux.inc (inlet flux), puretet_of_auto_point3.cuh
-// (omega wall penalty) and modules/wall/tet_wall_match_cache.inc:3-67 (sample
-// location by adjacency walk).
+// (omega wall penalty), modules/wall/tet_wall_match_cache.inc:3-67 (sample
+// location by adjacency walk) and modules/wall/tet_wall_ray_cache.inc:3-205
+// (ray-traced sample placement).
 #pragma once
 
 #include <array>
@@ -38,6 +39,19 @@ struct TurbParams {
   std::array<double, 5> heights{{0, 0, 0, 0, 0}};
   double sampleScale = 0.5;
   double ofKappa = 0.41, ofE = 9.8, ofBeta1 = 0.075, ofReBlend = 11.0, ofProdLimit = 10.0;
+  // Sample placement (SST_WALL_SAMPLING_MODE): 0 legacy_endpoint, the
+  // adjacency walk to each endpoint (tet_wall_match_cache.inc:3-67); 1
+  // ray_trace, the face-connected traversal of the whole wall-normal segment
+  // (tet_wall_ray_cache.inc:3-205) for the active slots only: the Spalding
+  // fit's points plus the omega wall point (the third), the other slots
+  // aliasing the first active one.  rayLocalAdapt (STAGE27_LOCAL_RAY_ADAPT,
+  // one active slot) shortens a segment that leaves the fluid to 0.90 of its
+  // certified length, at least 0.25 of the height.
+  int sampling = 0;
+  int rayMaxCrossings = 4096;
+  double rayTol = 1e-9;
+  bool rayLocalAdapt = false;
+  int activePoints = 5;  // Spalding fit points (SST24B_ACTIVE_POINTS)
 };
 
 // A boundary face seen from its tetrahedron: local face lf is the face
@@ -48,9 +62,11 @@ struct TurbFace {
   double area = 0;
 };
 
-// One wall-normal sample: containing cell and its P1+BF3 basis values there.
+// One wall-normal sample: containing cell and its P1+BF3 basis values there,
+// and its wall distance (the slot's height unless the ray was shortened).
 struct TurbSample {
   std::int32_t cell = -1;
+  double y = 0;
   double l[4] = {0, 0, 0, 0};
   double phi[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -77,6 +93,13 @@ struct TurbTopo {
   std::vector<std::int32_t> blockCell;  // cell of every block
   double kIn = 0, omegaIn = 0;
   std::array<double, 5> heights{{0, 0, 0, 0, 0}};  // sample heights in use
+  // Placement statistics: traced (or walked) samples, those whose cell is
+  // not the wall cell, the largest crossing count, rays shortened and the
+  // smallest kept fraction of a shortened ray's height.
+  std::int64_t placed = 0, crossed = 0, shortened = 0;
+  int maxCrossings = 0;
+  double minKeptFraction = 1.0;
+  unsigned rayMask = 0x1f;
 };
 
 // Builds the topology for the H8 pipe (inlet and wall patches of S.pipe).

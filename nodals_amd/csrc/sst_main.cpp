@@ -251,9 +251,27 @@ int run(int argc, char** argv) {
       tu.push_back(kv.first);
       tu.push_back(kv.second);
     }
+  // wall sample placement (the reference binary's default is the endpoint
+  // walk; the launcher asks for ray tracing)
+  const std::string sampling = envOr("SST_WALL_SAMPLING_MODE", "legacy_endpoint");
+  require(sampling == "ray_trace" || sampling == "legacy_endpoint",
+          "SST_WALL_SAMPLING_MODE=" + sampling + " is not supported (ray_trace or legacy_endpoint)");
+  tu.insert(tu.end(), {"--sst-wall-sampling", sampling, "--sst-wall-active-points", envOr("SST24B_ACTIVE_POINTS", "1"),
+                       "--sst-wall-ray-max-crossings", envOr("SST_WALL_RAY_MAX_CROSSINGS", "4096"),
+                       "--sst-wall-ray-bary-tolerance", envOr("SST_WALL_RAY_BARY_TOLERANCE", "1e-9"),
+                       "--sst-wall-ray-local-adapt", envOr("STAGE27_LOCAL_RAY_ADAPT", "0")});
   std::vector<const char*> tv;
   for (const auto& s : tu) tv.push_back(s.c_str());
   check(nodals_h8_turb_init(c.h, (int)tv.size(), tv.empty() ? nullptr : tv.data()));
+  {
+    double rs[7];
+    check(nodals_h8_get(c.h, "turb_sample_stats", rs, 7));
+    std::printf("SST_WALL_SAMPLING_RUNTIME method=%s rayContainment=%s activeMask=0x%x status=PASS\n",
+                sampling.c_str(), sampling == "ray_trace" ? "VERIFIED" : "NOT_CHECKED", (unsigned)rs[5]);
+    std::printf("SST_WALL_RAY_CACHE samplePoints=%lld crossedTet=%lld maxCrossings=%d shortened=%lld "
+                "minRetainedFrac=%.9g status=PASS\n",
+                (long long)rs[0], (long long)rs[1], (int)rs[2], (long long)rs[3], rs[4]);
+  }
 
   // wall closure from the launcher's environment
   const std::string conv = envOr("NODALS_MOMENTUM_CONVECTION_FORM", "advective");

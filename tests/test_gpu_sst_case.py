@@ -65,6 +65,9 @@ def test_sst_case_runs_checkpoints_and_restarts(helix_case):
     final = re.search(r"NODALS_SST_G2_FINAL .*wallShearMean=(\S+) yPlusMean=(\S+)", p.stdout)
     assert final and float(final.group(1)) > 0 and float(final.group(2)) > 0
     assert "NODALS_SST_RESIDUALS it=10" in p.stdout and "NODALS_SST_RESIDUALS it=20" in p.stdout
+    # the launcher's default [wall_sampling] mode = ray_trace reaches the solver
+    assert "SST_WALL_SAMPLING_RUNTIME method=ray_trace rayContainment=VERIFIED" in log
+    assert re.search(r"SST_WALL_RAY_CACHE samplePoints=\d+ .*shortened=0 ", log)
     for it in (10, 20):
         assert (out1 / f"NODALS_STAGE31_STATE_it{it}.ckpt").is_file()
         assert (out1 / f"flow_sst_it{it}.vtu").is_file()
