@@ -57,6 +57,12 @@ TurbParams parseTurbArgs(int argc, const char* const* argv) {
     else if (f == "--sst-wall-ray-bary-tolerance") P.rayTol = num(f, v);
     else if (f == "--sst-wall-ray-local-adapt") P.rayLocalAdapt = num(f, v) != 0.0;
     else if (f == "--sst-wall-active-points") P.activePoints = (int)num(f, v);
+    else if (f == "--sst-face-utau-csv") P.faceUtauCsv = v;
+    else if (f == "--sst-face-yplus-sp") P.faceTargetSp = num(f, v);
+    else if (f == "--sst-face-yplus-omega") P.faceTargetOmega = num(f, v);
+    else if (f == "--sst-face-yplus-min-scale") P.faceMinScale = num(f, v);
+    else if (f == "--sst-face-yplus-max-scale") P.faceMaxScale = num(f, v);
+    else if (f == "--sst-spalding-sample-scale") P.baseScale = num(f, v);
     else fail("unknown k-omega option " + f);
   }
   require(P.gsOmega > 0 && P.gsOmega < 2, "--sst-gs-omega must be in (0, 2)");
@@ -352,6 +358,13 @@ std::vector<double> H8Solver::turbField(const std::string& n) {
   if (n == "turb_sample_y") {
     std::vector<double> v;
     for (const TurbSample& s : T.samples) v.push_back(s.y);
+    return v;
+  }
+  if (n == "turb_face_y3") return T.faceY3;
+  if (n == "turb_face_yplus_stats") return std::vector<double>(T.faceYplusStats.begin(), T.faceYplusStats.end());
+  if (n == "turb_wall_faces") {
+    std::vector<double> v;
+    for (const TurbFace& f : T.wall) v.insert(v.end(), {(double)f.cell, (double)f.lf});
     return v;
   }
   if (n == "turb_sample_stats")

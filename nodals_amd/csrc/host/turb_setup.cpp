@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <fstream>
 #include <string>
 
 #include "error.hpp"
@@ -81,6 +83,56 @@ TurbFace facePlan(const TetMesh& M, int f) {
   return F;
 }
 
+// Per-face third-sample heights from fitted friction velocities
+// (sst_application.inc:1039-1097, NODALS_FACE_YPLUS_UTAU_CSV).
+std::vector<double> loadFaceY3(const TurbParams& P, TurbTopo& T, double nu) {
+  require(P.activePoints == 1, "per-face y+ heights need one active Spalding point (the third)");
+  require(P.faceTargetSp > 0 && P.faceTargetOmega > 0 && P.faceTargetSp <= 500 && P.faceTargetOmega <= 500 &&
+              P.faceMinScale > 0 && P.faceMinScale <= P.faceMaxScale && P.faceMaxScale <= 1,
+          "per-face y+ target or scale range invalid");
+  require(P.faceTargetOmega == P.faceTargetSp,
+          "per-face y+ heights: the omega target must equal the Spalding target (one third-sample point serves both)");
+  require(P.baseScale > 0, "per-face y+ heights need the Spalding sample scale of the given heights");
+  const double y3 = T.heights[2], full = y3 / P.baseScale;
+  const double yMin = full * P.faceMinScale, yMax = full * P.faceMaxScale;
+  std::ifstream in(P.faceUtauCsv);
+  require((bool)in, "cannot open the face friction-velocity file " + P.faceUtauCsv);
+  std::vector<double> out;
+  out.reserve(T.wall.size());
+  std::size_t fallback = 0, clampSp = 0, clampOm = 0;
+  double spMin = 1e300, spMax = 0, spSum = 0, omMin = 1e300, omMax = 0, omSum = 0;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    long long fi = -1, cell = -1, lf = -1;
+    double ut = 0, yp = 0;
+    const std::size_t n = out.size();
+    require(std::sscanf(line.c_str(), "%lld,%lld,%lld,%lf,%lf", &fi, &cell, &lf, &ut, &yp) == 5 &&
+                fi == (long long)n && n < T.wall.size() && cell == T.wall[n].cell && lf == T.wall[n].lf &&
+                std::isfinite(ut) && ut >= 0.0,
+            "face friction-velocity file does not match the wall faces at row " + std::to_string(n));
+    const bool fit = ut > 1e-8;
+    if (!fit) ++fallback;
+    const double sp0 = fit ? P.faceTargetSp * nu / ut : y3;
+    const double om0 = fit ? P.faceTargetOmega * nu / ut : y3 * (P.faceTargetOmega / P.faceTargetSp);
+    const double sp = std::min(yMax, std::max(yMin, sp0)), om = std::min(yMax, std::max(yMin, om0));
+    clampSp += std::abs(sp - sp0) > 1e-13;
+    clampOm += std::abs(om - om0) > 1e-13;
+    spMin = std::min(spMin, sp);
+    spMax = std::max(spMax, sp);
+    spSum += sp;
+    omMin = std::min(omMin, om);
+    omMax = std::max(omMax, om);
+    omSum += om;
+    out.push_back(sp);
+  }
+  require(!out.empty() && out.size() == T.wall.size(), "face friction-velocity file has the wrong number of faces");
+  const double nf = (double)out.size();
+  T.faceYplusStats = {nf, (double)fallback, (double)clampSp, (double)clampOm, spMin, spSum / nf, spMax,
+                      omMin, omSum / nf, omMax};
+  return out;
+}
+
 }  // namespace
 
 TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) {
@@ -121,6 +173,7 @@ TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) 
 
   // Wall-normal samples.
   require(P.sampling == 0 || P.sampling == 1, "wall sampling must be legacy_endpoint or ray_trace");
+  require(P.faceUtauCsv.empty() || P.sampling == 1, "per-face y+ sample heights need ray_trace wall sampling");
   require(P.activePoints >= 1 && P.activePoints <= 5, "active wall points must be in 1..5");
   T.samples.resize((std::size_t)T.nWallQ * 5);
   auto quadPoint = [&](const TurbFace& F, int q) {
@@ -181,6 +234,7 @@ TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) 
     }
     T.rayMask = 0x1f;
   } else {
+    if (!P.faceUtauCsv.empty()) T.faceY3 = loadFaceY3(P, T, S.pipe.nu);
     // Ray-traced active slots: the Spalding fit's points and the omega wall
     // point (slot 3); inactive slots alias the first active slot's hit and
     // keep their own height.
@@ -193,7 +247,8 @@ TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) 
         slots.push_back(m);
         base.push_back(T.heights[(std::size_t)m]);
       }
-    require(!P.rayLocalAdapt || slots.size() == 1,
+    const bool adapt = P.rayLocalAdapt || !T.faceY3.empty();
+    require(!adapt || slots.size() == 1,
             "ray local shortening needs exactly one active wall sample (one Spalding point at the omega slot)");
     TetRayTracer tracer(M.points, M.tets);
     for (std::size_t fi = 0; fi < T.wall.size(); ++fi) {
@@ -202,11 +257,15 @@ TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) 
       for (int q = 0; q < 12; ++q) {
         const V3 X = quadPoint(F, q);
         std::vector<double> dist = base;
+        if (!T.faceY3.empty()) {
+          const double ratio = T.faceY3[fi] / T.heights[2];
+          for (double& v : dist) v *= ratio;
+        }
         std::vector<RayHit> hits;
         try {
           hits = tracer.trace(F.cell, X, inward, dist, P.rayMaxCrossings, P.rayTol);
         } catch (const Error& e) {
-          if (!P.rayLocalAdapt)
+          if (!adapt)
             fail("wall sample ray (face " + std::to_string(fi) + ", point " + std::to_string(q) + ", slot " +
                  std::to_string(slots[0] + 1) + "): " + e.what());
           const double want = dist[0];

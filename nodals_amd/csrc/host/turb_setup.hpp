@@ -18,6 +18,7 @@
 
 #include <array>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "h8_setup.hpp"
@@ -52,6 +53,14 @@ struct TurbParams {
   double rayTol = 1e-9;
   bool rayLocalAdapt = false;
   int activePoints = 5;  // Spalding fit points (SST24B_ACTIVE_POINTS)
+  // Per-face third-sample heights from a previous segment's fitted friction
+  // velocities (NODALS_FACE_YPLUS_UTAU_CSV, rows "face,cell,lf,u_tau,y+"):
+  // y3 = target y+ nu / u_tau, clamped to [minScale, maxScale] times the
+  // full-scale third height (the given third height / baseScale,
+  // STAGE32_SPALDING_SAMPLE_SCALE); u_tau <= 1e-8 keeps the given height.
+  // Ray tracing with one active slot and local shortening.
+  std::string faceUtauCsv;
+  double faceTargetSp = 30, faceTargetOmega = 30, faceMinScale = 0.02, faceMaxScale = 0.35, baseScale = 0;
 };
 
 // A boundary face seen from its tetrahedron: local face lf is the face
@@ -100,6 +109,12 @@ struct TurbTopo {
   int maxCrossings = 0;
   double minKeptFraction = 1.0;
   unsigned rayMask = 0x1f;
+  // Per-face third-sample heights when loaded from fitted friction
+  // velocities (empty otherwise), and the load summary: faces, fallbacks
+  // (u_tau <= 1e-8), clamped Spalding and omega heights, then min, mean and
+  // max of the Spalding and of the omega heights.
+  std::vector<double> faceY3;
+  std::array<double, 10> faceYplusStats{};
 };
 
 // Builds the topology for the H8 pipe (inlet and wall patches of S.pipe).

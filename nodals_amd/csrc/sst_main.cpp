@@ -143,7 +143,36 @@ void saveCheckpoint(Context& c, const std::string& path, int it, const std::stri
        std::fwrite(k.data(), 8, k.size(), f) == k.size() && std::fwrite(w.data(), 8, w.size(), f) == w.size();
   ok = (std::fclose(f) == 0) && ok;
   require(ok, "error writing checkpoint " + path);
-  std::printf("STAGE31_RESTART_SAVE=%s it=%d\n", path.c_str(), it);
+  std::printf("STAGE31_RESTART_SAVE=PASS iteration=%d path=%s fields=U_P_K_OMEGA\n", it, path.c_str());
+  std::fflush(stdout);
+}
+
+// The wall path's face fits (friction velocity and y+ per wall face, the fit
+// of the last assembly, one iteration behind the saved state) for the next
+// segment's per-face sample heights: NODALS_FACE_UTAU_it<N>.csv, written to a
+// temporary name and renamed.
+void exportFaceUtau(Context& c, const std::string& dir, int it) {
+  const std::vector<double> fit = getD(c.h, "wall_fit", 3 * c.nf), faces = getD(c.h, "turb_wall_faces", 2 * c.nf);
+  const std::string path = dir + "/NODALS_FACE_UTAU_it" + std::to_string(it) + ".csv", tmp = path + ".tmp";
+  std::FILE* f = std::fopen(tmp.c_str(), "w");
+  require(f != nullptr, "cannot write " + path);
+  std::fprintf(f, "# NODALS_FACE_UTAU_V1 iteration=%d faces=%lld fit=F2_lagged\n", it, (long long)c.nf);
+  int64_t bad = 0;
+  for (int64_t i = 0; i < c.nf; ++i) {
+    const double beta = fit[3 * (std::size_t)i], ut = fit[3 * (std::size_t)i + 1], yp = fit[3 * (std::size_t)i + 2];
+    if (!(beta > 0 && std::isfinite(beta) && std::isfinite(ut) && ut >= 0 && std::isfinite(yp))) ++bad;
+    std::fprintf(f, "%lld,%d,%d,%.17g,%.17g\n", (long long)i, (int)faces[2 * (std::size_t)i],
+                 (int)faces[2 * (std::size_t)i + 1], ut, yp);
+  }
+  const bool closed = std::fclose(f) == 0;
+  if (bad || !closed) {
+    std::remove(tmp.c_str());
+    throw Fail(bad ? "invalid wall fits on " + std::to_string(bad) + " faces; no friction-velocity export"
+                   : "error writing " + path);
+  }
+  require(std::rename(tmp.c_str(), path.c_str()) == 0, "cannot rename " + tmp);
+  std::printf("NODALS_FACE_UTAU_EXPORT=PASS iteration=%d faces=%lld path=%s status=PASS\n", it, (long long)c.nf,
+              path.c_str());
   std::fflush(stdout);
 }
 
@@ -186,7 +215,7 @@ int loadCheckpoint(Context& c, const std::string& path, const std::string& finge
   check(nodals_h8_set(c.h, "p", p.data(), nc));
   check(nodals_h8_set(c.h, "k", k.data(), ns));
   check(nodals_h8_set(c.h, "omega", w.data(), ns));
-  std::printf("STAGE31_RESTART_LOAD=%s it=%d\n", path.c_str(), it);
+  std::printf("STAGE31_RESTART_LOAD=PASS path=%s iteration=%d next=%d fields=U_P_K_OMEGA\n", path.c_str(), it, it + 1);
   std::fflush(stdout);
   return it;
 }
@@ -260,6 +289,18 @@ int run(int argc, char** argv) {
                        "--sst-wall-ray-max-crossings", envOr("SST_WALL_RAY_MAX_CROSSINGS", "4096"),
                        "--sst-wall-ray-bary-tolerance", envOr("SST_WALL_RAY_BARY_TOLERANCE", "1e-9"),
                        "--sst-wall-ray-local-adapt", envOr("STAGE27_LOCAL_RAY_ADAPT", "0")});
+  // per-face sample heights from the previous segment's wall fits
+  const std::string faceCsv = envOr("NODALS_FACE_YPLUS_UTAU_CSV", "");
+  if (!faceCsv.empty()) {
+    require(sampling == "ray_trace", "NODALS_FACE_YPLUS_UTAU_CSV requires SST_WALL_SAMPLING_MODE=ray_trace");
+    const std::string base = envOr("STAGE32_SPALDING_SAMPLE_SCALE", "");
+    require(!base.empty(), "NODALS_FACE_YPLUS_UTAU_CSV requires STAGE32_SPALDING_SAMPLE_SCALE");
+    tu.insert(tu.end(), {"--sst-face-utau-csv", faceCsv, "--sst-spalding-sample-scale", base,
+                         "--sst-face-yplus-sp", envOr("NODALS_FACE_YPLUS_SP", "30"),
+                         "--sst-face-yplus-omega", envOr("NODALS_FACE_YPLUS_OMEGA", "30"),
+                         "--sst-face-yplus-min-scale", envOr("NODALS_FACE_YPLUS_MIN_SCALE", "0.02"),
+                         "--sst-face-yplus-max-scale", envOr("NODALS_FACE_YPLUS_MAX_SCALE", "0.35")});
+  }
   std::vector<const char*> tv;
   for (const auto& s : tu) tv.push_back(s.c_str());
   check(nodals_h8_turb_init(c.h, (int)tv.size(), tv.empty() ? nullptr : tv.data()));
@@ -271,6 +312,18 @@ int run(int argc, char** argv) {
     std::printf("SST_WALL_RAY_CACHE samplePoints=%lld crossedTet=%lld maxCrossings=%d shortened=%lld "
                 "minRetainedFrac=%.9g status=PASS\n",
                 (long long)rs[0], (long long)rs[1], (int)rs[2], (long long)rs[3], rs[4]);
+    if (!faceCsv.empty()) {
+      double fs[10];
+      check(nodals_h8_get(c.h, "turb_face_yplus_stats", fs, 10));
+      std::printf("NODALS_FACE_YPLUS_LOADED faces=%lld targetSp=%s targetOmega=%s minScale=%s maxScale=%s "
+                  "fallback=%lld clampSp=%lld clampOmega=%lld spY3Min=%.12e spY3Mean=%.12e spY3Max=%.12e "
+                  "omegaY3Min=%.12e omegaY3Mean=%.12e omegaY3Max=%.12e input=%s source=LAGGED_F2_FACE_UTAU "
+                  "status=PASS\n",
+                  (long long)fs[0], envOr("NODALS_FACE_YPLUS_SP", "30").c_str(),
+                  envOr("NODALS_FACE_YPLUS_OMEGA", "30").c_str(), envOr("NODALS_FACE_YPLUS_MIN_SCALE", "0.02").c_str(),
+                  envOr("NODALS_FACE_YPLUS_MAX_SCALE", "0.35").c_str(), (long long)fs[1], (long long)fs[2],
+                  (long long)fs[3], fs[4], fs[5], fs[6], fs[7], fs[8], fs[9], faceCsv.c_str());
+    }
   }
 
   // wall closure from the launcher's environment
@@ -315,6 +368,13 @@ int run(int argc, char** argv) {
   const std::string ckDir = envOr("STAGE31_CHECKPOINT_DIR", ".");
   const int vtuEvery = std::atoi(envOr("SST_G2_VTU_EVERY", "0").c_str());
   const std::string vtuDir = envOr("SST_G2_VTU_DIR", ".");
+  const std::string exportFlag = envOr("NODALS_FACE_YPLUS_EXPORT", "0");
+  require(exportFlag == "0" || exportFlag == "1", "NODALS_FACE_YPLUS_EXPORT must be 0 or 1");
+  const bool exportFaces = exportFlag == "1";
+  auto save = [&](int at) {
+    saveCheckpoint(c, ckptPath(ckDir, at), at, fingerprint);
+    if (exportFaces) exportFaceUtau(c, ckDir, at);
+  };
 
   nodals_h8_run_result r{};
   int last = it0;
@@ -328,7 +388,7 @@ int run(int argc, char** argv) {
     loopMs += r.loop_ms;
     last = r.outer;
     converged = r.converged != 0;
-    if (ckEvery > 0 && last % ckEvery == 0 && !converged) saveCheckpoint(c, ckptPath(ckDir, last), last, fingerprint);
+    if (ckEvery > 0 && last % ckEvery == 0 && !converged) save(last);
     if (vtuEvery > 0 && last % vtuEvery == 0) {
       const std::string p = vtuDir + "/flow_sst_it" + std::to_string(last) + ".vtu";
       check(nodals_h8_write_vtu(c.h, p.c_str()));
@@ -338,8 +398,7 @@ int run(int argc, char** argv) {
     it = last + 1;
   }
   // the final (or converged) state, unless the cadence has just written it
-  if (ckEvery > 0 && last > it0 && (converged || last % ckEvery != 0))
-    saveCheckpoint(c, ckptPath(ckDir, last), last, fingerprint);
+  if (ckEvery > 0 && last > it0 && (converged || last % ckEvery != 0)) save(last);
   const std::string vtu = A.get("--vtu");
   if (!vtu.empty()) {
     check(nodals_h8_write_vtu(c.h, vtu.c_str()));

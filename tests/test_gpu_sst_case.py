@@ -83,3 +83,39 @@ def test_sst_case_runs_checkpoints_and_restarts(helix_case):
     a = _payload(out1 / "NODALS_STAGE31_STATE_it20.ckpt")
     b = _payload(out2 / "NODALS_STAGE31_STATE_it20.ckpt")
     assert a == b
+
+
+def test_adaptive_runner_moves_wall_samples(helix_case):
+    """Three 10-iteration segments (refreshes at 10 and 20) through the
+    launcher and nodals_sst: each segment after the first restarts from the
+    previous checkpoint and places every wall face's third sample at
+    30 nu / u_tau of the previous segment's fit, clamped to [0.02, 0.35] of
+    the full-scale height (run_adaptive_case.py; sst_application.inc:
+    1039-1097)."""
+    d, sets = helix_case
+    out = d / "adaptive"
+    sets = [s for s in sets if not s.startswith(("simple.max_iterations", "output.checkpoint_every",
+                                                  "output.vtu_every"))]
+    sets += ["simple.max_iterations=30", "adaptive_sampling.first_refresh=10", "adaptive_sampling.refresh_every=10",
+             "output.checkpoint_every=10", "output.vtu_every=0"]
+    args = [sys.executable, "-m", "nodals_amd.adaptive_case", "--case", str(d / "case.case"), "--output", str(out),
+            "--cache", str(d / "cache")]
+    for s in sets:
+        args += ["--set", s]
+    print("[gpu] adaptive runner, small helix, 3 x 10 iterations", flush=True)
+    p = subprocess.run(args, capture_output=True, text=True, cwd=ROOT, timeout=900)
+    print(p.stdout[-3000:], p.stderr[-2000:], flush=True)
+    assert p.returncode == 0
+    assert p.stdout.count("NODALS_ADAPTIVE_PASS") == 3 and "NODALS_ADAPTIVE_COMPLETE iteration=30" in p.stdout
+    face = out / "stage_0000_0010" / "NODALS_FACE_UTAU_it10.csv"
+    rows = [l.split(",") for l in face.read_text().splitlines() if not l.startswith("#")]
+    ut = np.array([float(r[3]) for r in rows])
+    assert np.all(ut > 0)
+    log = (out / "stage_0010_0020" / "terminal.log").read_text()
+    assert "STAGE31_RESTART_LOAD=PASS" in log
+    m = re.search(r"NODALS_FACE_YPLUS_LOADED faces=(\d+) .*spY3Mean=(\S+) ", log)
+    assert m and int(m.group(1)) == len(rows)
+    nu, y3 = 0.0002, 4.4084029504620439e-4 / 0.004 * 0.5  # [wall_function] spalding_sample_scale 0.5, D = 1
+    want = np.clip(30 * nu / ut, 0.02 * y3 / 0.5, 0.35 * y3 / 0.5).mean()
+    assert abs(float(m.group(2)) - want) <= 1e-9 * want
+    assert "SST_WALL_SAMPLING_RUNTIME method=ray_trace" in log

@@ -193,3 +193,85 @@ def test_local_shortening_across_the_pipe(mesh_dir):
     # towards the axis is at least the radius) and D
     assert np.all(y3 / 0.9 > 0.5) and np.all(y3 / 0.9 <= 1.0 + 1e-6)
     assert np.all(s["y"][:, [0, 1, 3, 4]] == s["heights"][[0, 1, 3, 4]])
+
+
+def _face_csv(path, faces, utau, header=True):
+    with open(path, "w") as f:
+        if header:
+            f.write(f"# NODALS_FACE_UTAU_V1 iteration=10 faces={len(faces)} fit=F2_lagged\n")
+        for i, ((c, lf), u) in enumerate(zip(faces, utau)):
+            f.write(f"{i},{int(c)},{int(lf)},{float(u)!r},{30.0!r}\n")
+
+
+def _face_context(mesh, extra):
+    from nodals_amd._lib import H8
+
+    h = H8(["--mesh", mesh, "--max-outer", "10", "--verbose", "0", "--re", "5000"], device=False)
+    h.turb_init(extra)
+    return h
+
+
+def test_per_face_heights_from_friction_velocities(mesh_dir, tmp_path):
+    """The adaptive runner's hand-over (sst_application.inc:1039-1097): the
+    third sample of each wall face moves to y = 30 nu / u_tau, clamped to
+    [0.02, 0.35] of the full-scale third height (given height / sample
+    scale); u_tau = 0 falls back to the given height, clamped the same way.  Every placed sample carries
+    its height, and the rays are traced (locally shortened if needed)."""
+    from nodals_amd._lib import H8
+
+    scale = 0.5
+    base = ["--sst-wall-sampling", "ray_trace", "--sst-wall-active-points", "1", "--sst-wall-sample-scale", str(scale)]
+    h0 = _face_context(mesh_dir["small"], base)
+    nf = h0.turb["wall_faces"]
+    faces = h0.get("turb_wall_faces", 2 * nf).reshape(nf, 2)
+    y3 = h0.get("turb_heights", 5)[2]
+    h0.close()
+    nu = 1.0 / 5000.0
+    rng = np.random.default_rng(4)
+    utau = 30 * nu / (y3 / scale * rng.uniform(0.005, 0.5, nf))  # y+ 30 heights from 0.005 to 0.5 full scale
+    utau[:5] = 0.0
+    csv = tmp_path / "NODALS_FACE_UTAU_it10.csv"
+    _face_csv(csv, faces, utau)
+    h = _face_context(mesh_dir["small"], base + ["--sst-face-utau-csv", str(csv), "--sst-spalding-sample-scale",
+                                                 str(scale)])
+    got = h.get("turb_face_y3", nf)
+    full = y3 / scale
+    # the fallback height is clamped like the others
+    unbounded = np.where(utau > 1e-8, 30 * nu / np.where(utau > 0, utau, 1.0), y3)
+    want = np.clip(unbounded, 0.02 * full, 0.35 * full)
+    assert np.allclose(got, want, rtol=1e-14, atol=0)
+    st = h.get("turb_face_yplus_stats", 10)
+    assert st[0] == nf and st[1] == 5
+    assert st[2] == np.sum(np.abs(want - unbounded) > 1e-13) and 0 < st[2] < nf
+    nq = 12 * nf
+    sy = h.get("turb_sample_y", nq * 5).reshape(nf, 12, 5)
+    stats = h.get("turb_sample_stats", 7)
+    shortened = int(stats[3])
+    # unshortened third samples sit at the face's height
+    at = np.isclose(sy[:, :, 2], got[:, None], rtol=1e-15, atol=0)
+    assert at.sum() == nq - shortened
+    assert np.all(sy[:, :, 2] <= got[:, None] * (1 + 1e-15))
+    h.close()
+
+
+def test_per_face_heights_reject_mismatched_files(mesh_dir, tmp_path):
+    base = ["--sst-wall-sampling", "ray_trace", "--sst-wall-active-points", "1"]
+    h0 = _face_context(mesh_dir["small"], base)
+    nf = h0.turb["wall_faces"]
+    faces = h0.get("turb_wall_faces", 2 * nf).reshape(nf, 2)
+    h0.close()
+    csv = tmp_path / "f.csv"
+    face = ["--sst-face-utau-csv", str(csv), "--sst-spalding-sample-scale", "0.5"]
+    bad = faces.copy()
+    bad[3, 0] += 1
+    _face_csv(csv, bad, np.full(nf, 0.05))
+    with pytest.raises(RuntimeError, match="does not match the wall faces at row 3"):
+        _face_context(mesh_dir["small"], base + face)
+    _face_csv(csv, faces[:-1], np.full(nf - 1, 0.05))
+    with pytest.raises(RuntimeError, match="wrong number of faces"):
+        _face_context(mesh_dir["small"], base + face)
+    _face_csv(csv, faces, np.full(nf, 0.05))
+    with pytest.raises(RuntimeError, match="need ray_trace"):
+        _face_context(mesh_dir["small"], ["--sst-wall-active-points", "1"] + face)
+    with pytest.raises(RuntimeError, match="omega target must equal"):
+        _face_context(mesh_dir["small"], base + face + ["--sst-face-yplus-omega", "40"])
