@@ -323,7 +323,14 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
     } else if (n == "k" || n == "omega" || n.rfind("turb_", 0) == 0) {
       copyOut(s.turbField(n), host, count);
     }
-    else if (n == "cell_ref") {
+    else if (n == "mesh_dof") {
+      std::vector<int32_t> v;
+      for (const auto& d : s.mesh().fileDof) v.insert(v.end(), d.begin(), d.end());
+      nodals::require(!v.empty(), "the mesh has no file DOF numbering");
+      copyOut(v, host, count);
+    } else if (n == "mesh_dof_counts") {
+      copyOut(std::vector<int64_t>{s.mesh().fileNvel, s.mesh().fileNvert}, host, count);
+    } else if (n == "cell_ref") {
       std::vector<int32_t> v;
       for (const auto& cp : s.setup().cells) v.insert(v.end(), cp.ref, cp.ref + 8);
       copyOut(v, host, count);

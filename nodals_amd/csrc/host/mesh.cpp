@@ -489,9 +489,39 @@ TetMesh readHxt1(const std::string& path) {
   require(M.patches[1].count == (std::int32_t)nintet && M.patches[2].count == (std::int32_t)nouttet,
           "HXT1 inlet/outlet records do not match the mesh's boundary faces");
   finishTetMesh(M, path);
+  // the file's velocity DOFs in this mesh's (cell, basis) order
+  const std::uint64_t nvel = h[5];
+  for (std::size_t c = 0; c < (std::size_t)ntet; ++c)
+    require(tpres[c] == (std::int32_t)c, "HXT1 pressure ids other than the cell index are not supported (" + path + ")");
+  M.fileDof.resize((std::size_t)ntet);
+  for (std::size_t c = 0; c < (std::size_t)ntet; ++c)
+    for (int a = 0; a < 4; ++a) {
+      int j = -1;
+      for (int q = 0; q < 4; ++q)
+        if (tets[c * 4 + (std::size_t)q] == M.tets[c][(std::size_t)a]) j = q;
+      require(j >= 0, "HXT1 cell vertices changed while reading");
+      const std::int32_t dv = tvel[c * 8 + (std::size_t)j], df = tvel[c * 8 + 4 + (std::size_t)j];
+      require(dv >= 0 && (std::uint64_t)dv < nvel && df >= 0 && (std::uint64_t)df < nvel, "HXT1 velocity DOF out of range");
+      M.fileDof[c][(std::size_t)a] = dv;
+      M.fileDof[c][(std::size_t)a + 4] = df;
+    }
+  M.fileNvel = (std::int64_t)nvel;
+  M.fileNvert = (std::int64_t)nv;
   return M;
 }
 
-TetMesh readMesh(const std::string& path) { return isHxt1File(path) ? readHxt1(path) : readPolyMesh(path); }
+TetMesh readMesh(const std::string& path) {
+  if (isHxt1File(path)) return readHxt1(path);
+  TetMesh M = readPolyMesh(path);
+  M.fileDof.resize(M.tets.size());
+  for (std::size_t c = 0; c < M.tets.size(); ++c)
+    for (int a = 0; a < 4; ++a) {
+      M.fileDof[c][(std::size_t)a] = M.tets[c][(std::size_t)a];
+      M.fileDof[c][(std::size_t)a + 4] = M.nPoints() + M.oppFace[c][(std::size_t)a];
+    }
+  M.fileNvel = (std::int64_t)M.nPoints() + M.nFaces();
+  M.fileNvert = M.nPoints();
+  return M;
+}
 
 }  // namespace nodals

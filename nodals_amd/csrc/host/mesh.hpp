@@ -30,6 +30,13 @@ struct TetMesh {
   std::vector<std::int32_t> facePatch;  // -1 for internal faces
   std::vector<std::array<std::int32_t, 4>> tets;
   std::vector<std::array<std::int32_t, 4>> oppFace;
+  // Velocity DOF of every (cell, basis) in the numbering of the mesh file
+  // (basis a < 4: vertex tets[c][a]; a >= 4: the face opposite vertex
+  // tets[c][a - 4]): an HXT1 file's own tet velocity DOFs, or for a polyMesh
+  // vertices then faces in this mesh's face order.  fileNvel DOFs, the first
+  // fileNvert of them the vertices.  Checkpoints store fields this way.
+  std::vector<std::array<std::int32_t, 8>> fileDof;
+  std::int64_t fileNvel = 0, fileNvert = 0;
 
   int nPoints() const { return (int)points.size(); }
   int nFaces() const { return (int)faces.size(); }

@@ -13,13 +13,12 @@
 // follows NODALS_SOFTWARE_ERROR).
 //
 // Checkpoints (STAGE31_CHECKPOINT_EVERY / _DIR, restart from
-// STAGE31_RESTART_FROM) are this build's own format, named
-// NODALS_STAGE31_STATE_it<N>.ckpt as the launcher expects: the magic
-// "NODALSCK", uint32 version 1, int32 iteration, int64 free velocity DOFs,
-// cells and k-omega entities, the 64-character input fingerprint
-// (STAGE31_INPUT_FINGERPRINT, or blanks), then u [nv][3], p [nc], k [ns] and
-// omega [ns] as float64.  A restart checks the sizes and, when both carry
-// one, the fingerprint.
+// STAGE31_RESTART_FROM) are the reference's Stage31 files
+// NODALS_STAGE31_STATE_it<N>.ckpt (layout at saveCheckpoint), with the
+// fields in the mesh file's velocity DOF numbering; a restart checks the
+// format, sizes, fingerprint (when STAGE31_INPUT_FINGERPRINT is set),
+// payload checksum and finiteness.  With NODALS_FACE_YPLUS_EXPORT=1 each
+// checkpoint also writes the wall faces' fitted friction velocities.
 //
 // Everything goes through the C ABI of libnodals_hip.so.
 #include <algorithm>
@@ -122,28 +121,92 @@ std::string ckptPath(const std::string& dir, int it) {
   return dir + "/NODALS_STAGE31_STATE_it" + std::to_string(it) + ".ckpt";
 }
 
-void saveCheckpoint(Context& c, const std::string& path, int it, const std::string& fingerprint) {
-  std::vector<double> u((std::size_t)c.nv * 3);
-  for (int d = 0; d < 3; ++d) {
-    const std::vector<double> ud = getD(c.h, ("u" + std::to_string(d)).c_str(), c.nv);
-    for (int64_t i = 0; i < c.nv; ++i) u[3 * (std::size_t)i + d] = ud[(std::size_t)i];
+// Stage31 warm-restart file (modules/coupling/stage31_restart.hpp:38-117):
+// a 144-byte header (magic "NODALS_S31_R1" in 16 bytes, uint32 format 1,
+// uint32 value size 8, uint64 nv np nvert iteration, double prevMeanNut,
+// 65-byte SHA-256 hex fingerprint, padding to 8, uint64 FNV-1a 64 hash of the
+// payload), then FP64 u0 u1 u2 [nv], p [np], k omega [nv].  nv counts the
+// mesh file's velocity DOFs (vertices first), so a state is stored in the
+// file's own numbering; fixed DOFs carry their boundary values.
+constexpr std::size_t kS31Header = 144;
+
+std::uint64_t fnv1a(std::uint64_t h, const std::vector<double>& v) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(v.data());
+  for (std::size_t i = 0; i < v.size() * sizeof(double); ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
   }
-  const std::vector<double> p = getD(c.h, "p", c.nc), k = getD(c.h, "k", c.ns), w = getD(c.h, "omega", c.ns);
-  std::FILE* f = std::fopen(path.c_str(), "wb");
-  require(f != nullptr, "cannot write checkpoint " + path);
-  const uint32_t version = 1;
-  const int32_t it32 = it;
-  char fp[64];
-  std::memset(fp, ' ', 64);
-  std::memcpy(fp, fingerprint.data(), std::min<std::size_t>(64, fingerprint.size()));
-  bool ok = std::fwrite("NODALSCK", 1, 8, f) == 8 && std::fwrite(&version, 4, 1, f) == 1 &&
-            std::fwrite(&it32, 4, 1, f) == 1 && std::fwrite(&c.nv, 8, 1, f) == 1 && std::fwrite(&c.nc, 8, 1, f) == 1 &&
-            std::fwrite(&c.ns, 8, 1, f) == 1 && std::fwrite(fp, 1, 64, f) == 64;
-  ok = ok && std::fwrite(u.data(), 8, u.size(), f) == u.size() && std::fwrite(p.data(), 8, p.size(), f) == p.size() &&
-       std::fwrite(k.data(), 8, k.size(), f) == k.size() && std::fwrite(w.data(), 8, w.size(), f) == w.size();
+  return h;
+}
+
+// (cell, basis) -> file DOF, free DOF (or -(fixed slot + 1)) and scalar entity
+struct DofMaps {
+  std::vector<int32_t> file, ref;
+  std::vector<double> ent;
+  int64_t nvel = 0, nvert = 0;
+};
+
+DofMaps dofMaps(Context& c) {
+  DofMaps m;
+  int64_t counts[2];
+  check(nodals_h8_get(c.h, "mesh_dof_counts", counts, 2));
+  m.nvel = counts[0];
+  m.nvert = counts[1];
+  m.file.resize((std::size_t)c.nc * 8);
+  m.ref.resize((std::size_t)c.nc * 8);
+  check(nodals_h8_get(c.h, "mesh_dof", m.file.data(), c.nc * 8));
+  check(nodals_h8_get(c.h, "cell_ref", m.ref.data(), c.nc * 8));
+  m.ent = getD(c.h, "turb_cell_ent", c.nc * 8);
+  return m;
+}
+
+void saveCheckpoint(Context& c, const std::string& path, int it, const std::string& fingerprint) {
+  const DofMaps m = dofMaps(c);
+  std::vector<double> uf[3];
+  for (int d = 0; d < 3; ++d) uf[d] = getD(c.h, ("u" + std::to_string(d)).c_str(), c.nv);
+  int64_t info[8];
+  check(nodals_h8_info(c.h, info));
+  const std::vector<double> fixed = getD(c.h, "fixed_value", 3 * info[7]);
+  const std::vector<double> p = getD(c.h, "p", c.nc), ks = getD(c.h, "k", c.ns), ws = getD(c.h, "omega", c.ns);
+  std::vector<double> U[3], K((std::size_t)m.nvel), W((std::size_t)m.nvel);
+  std::vector<char> seen((std::size_t)m.nvel, 0);
+  for (int d = 0; d < 3; ++d) U[d].assign((std::size_t)m.nvel, 0.0);
+  for (std::size_t q = 0; q < m.file.size(); ++q) {
+    const std::size_t f = (std::size_t)m.file[q], e = (std::size_t)m.ent[q];
+    const int32_t r = m.ref[q];
+    for (int d = 0; d < 3; ++d) U[d][f] = r >= 0 ? uf[d][(std::size_t)r] : fixed[3 * (std::size_t)(-r - 1) + d];
+    K[f] = ks[e];
+    W[f] = ws[e];
+    seen[f] = 1;
+  }
+  for (char x : seen) require(x != 0, "a mesh velocity DOF belongs to no cell; cannot write " + path);
+  const std::vector<double>* const fields[6] = {&U[0], &U[1], &U[2], &p, &K, &W};
+  std::uint64_t hash = 14695981039346656037ull;
+  for (const auto* v : fields) hash = fnv1a(hash, *v);
+  unsigned char hdr[kS31Header] = {};
+  std::memcpy(hdr, "NODALS_S31_R1", 13);
+  const uint32_t format = 1, bytes = 8;
+  const uint64_t nv = (uint64_t)m.nvel, np = (uint64_t)c.nc, nvert = (uint64_t)m.nvert, iter = (uint64_t)it;
+  const double prevMeanNut = -1.0;  // no plateau history is kept here
+  std::memcpy(hdr + 16, &format, 4);
+  std::memcpy(hdr + 20, &bytes, 4);
+  std::memcpy(hdr + 24, &nv, 8);
+  std::memcpy(hdr + 32, &np, 8);
+  std::memcpy(hdr + 40, &nvert, 8);
+  std::memcpy(hdr + 48, &iter, 8);
+  std::memcpy(hdr + 56, &prevMeanNut, 8);
+  std::memcpy(hdr + 64, fingerprint.data(), std::min<std::size_t>(64, fingerprint.size()));
+  std::memcpy(hdr + 136, &hash, 8);
+  const std::string tmp = path + ".partial";
+  std::FILE* f = std::fopen(tmp.c_str(), "wb");
+  require(f != nullptr, "cannot write checkpoint " + tmp);
+  bool ok = std::fwrite(hdr, 1, kS31Header, f) == kS31Header;
+  for (const auto* v : fields) ok = ok && std::fwrite(v->data(), 8, v->size(), f) == v->size();
   ok = (std::fclose(f) == 0) && ok;
-  require(ok, "error writing checkpoint " + path);
-  std::printf("STAGE31_RESTART_SAVE=PASS iteration=%d path=%s fields=U_P_K_OMEGA\n", it, path.c_str());
+  require(ok, "error writing checkpoint " + tmp);
+  require(std::rename(tmp.c_str(), path.c_str()) == 0, "cannot rename " + tmp);
+  std::printf("STAGE31_RESTART_SAVE=PASS iteration=%d path=%s fields=FULL_BF3_U_P_K_OMEGA checksum=PASS\n", it,
+              path.c_str());
   std::fflush(stdout);
 }
 
@@ -177,47 +240,72 @@ void exportFaceUtau(Context& c, const std::string& dir, int it) {
 }
 
 int loadCheckpoint(Context& c, const std::string& path, const std::string& fingerprint) {
+  const DofMaps m = dofMaps(c);
   std::FILE* f = std::fopen(path.c_str(), "rb");
-  require(f != nullptr, "cannot open checkpoint " + path);
-  char magic[8];
-  uint32_t version = 0;
-  int32_t it = 0;
-  int64_t nv = 0, nc = 0, ns = 0;
-  char fp[64];
-  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "NODALSCK", 8) == 0 &&
-            std::fread(&version, 4, 1, f) == 1 && version == 1 && std::fread(&it, 4, 1, f) == 1 &&
-            std::fread(&nv, 8, 1, f) == 1 && std::fread(&nc, 8, 1, f) == 1 && std::fread(&ns, 8, 1, f) == 1 &&
-            std::fread(fp, 1, 64, f) == 64;
+  require(f != nullptr, "Stage31 checkpoint not found: " + path);
+  unsigned char hdr[kS31Header];
+  uint32_t format = 0, bytes = 0;
+  uint64_t nv = 0, np = 0, nvert = 0, iter = 0, hash = 0;
+  double prevMeanNut = 0;
+  bool ok = std::fread(hdr, 1, kS31Header, f) == kS31Header && std::memcmp(hdr, "NODALS_S31_R1", 13) == 0;
+  if (ok) {
+    std::memcpy(&format, hdr + 16, 4);
+    std::memcpy(&bytes, hdr + 20, 4);
+    std::memcpy(&nv, hdr + 24, 8);
+    std::memcpy(&np, hdr + 32, 8);
+    std::memcpy(&nvert, hdr + 40, 8);
+    std::memcpy(&iter, hdr + 48, 8);
+    std::memcpy(&prevMeanNut, hdr + 56, 8);
+    std::memcpy(&hash, hdr + 136, 8);
+    ok = format == 1 && bytes == 8;
+  }
   if (!ok) {
     std::fclose(f);
-    throw Fail("not a checkpoint of this solver: " + path);
+    throw Fail("Stage31 checkpoint header/format mismatch: " + path);
   }
-  if (nv != c.nv || nc != c.nc || ns != c.ns) {
+  if (nv != (uint64_t)m.nvel || np != (uint64_t)c.nc || nvert != (uint64_t)m.nvert) {
     std::fclose(f);
-    throw Fail("checkpoint sizes do not match the mesh: " + path);
+    throw Fail("Stage31 checkpoint DOF dimensions mismatch: refusing restart (" + path + ")");
   }
-  const std::string stored(fp, 64);
-  if (!fingerprint.empty() && stored.find_first_not_of(' ') != std::string::npos &&
-      stored.substr(0, fingerprint.size()) != fingerprint) {
+  if (!fingerprint.empty() && std::string(reinterpret_cast<const char*>(hdr + 64), 64) != fingerprint) {
     std::fclose(f);
-    throw Fail("checkpoint input fingerprint does not match STAGE31_INPUT_FINGERPRINT: " + path);
+    throw Fail("Stage31 checkpoint mesh/inlet/physical-parameter SHA256 mismatch: " + path);
   }
-  std::vector<double> u((std::size_t)nv * 3), p((std::size_t)nc), k((std::size_t)ns), w((std::size_t)ns);
-  ok = std::fread(u.data(), 8, u.size(), f) == u.size() && std::fread(p.data(), 8, p.size(), f) == p.size() &&
-       std::fread(k.data(), 8, k.size(), f) == k.size() && std::fread(w.data(), 8, w.size(), f) == w.size();
+  if (iter < 1 || iter > 100000000 || !std::isfinite(prevMeanNut)) {
+    std::fclose(f);
+    throw Fail("Stage31 checkpoint iteration/state invalid: " + path);
+  }
+  std::vector<double> U[3], p((std::size_t)np), K((std::size_t)nv), W((std::size_t)nv);
+  for (int d = 0; d < 3; ++d) U[d].resize((std::size_t)nv);
+  for (auto* v : {&U[0], &U[1], &U[2], &p, &K, &W}) ok = ok && std::fread(v->data(), 8, v->size(), f) == v->size();
+  char extra = 0;
+  const bool trailing = std::fread(&extra, 1, 1, f) == 1;
   std::fclose(f);
-  require(ok, "short checkpoint " + path);
-  for (int d = 0; d < 3; ++d) {
-    std::vector<double> ud((std::size_t)nv);
-    for (int64_t i = 0; i < nv; ++i) ud[(std::size_t)i] = u[3 * (std::size_t)i + d];
-    check(nodals_h8_set(c.h, ("u" + std::to_string(d)).c_str(), ud.data(), nv));
+  require(ok && !trailing, "Stage31 checkpoint truncated or trailing data: " + path);
+  std::uint64_t h = 14695981039346656037ull;
+  for (const auto* v : {&U[0], &U[1], &U[2], &p, &K, &W}) h = fnv1a(h, *v);
+  require(h == hash, "Stage31 checkpoint payload checksum mismatch: " + path);
+  for (const auto* v : {&U[0], &U[1], &U[2], &p, &K, &W})
+    for (double x : *v) require(std::isfinite(x), "Stage31 checkpoint contains nonfinite values: " + path);
+  std::vector<double> uf[3], ks((std::size_t)c.ns, 0.0), ws((std::size_t)c.ns, 0.0);
+  for (int d = 0; d < 3; ++d) uf[d] = getD(c.h, ("u" + std::to_string(d)).c_str(), c.nv);
+  for (std::size_t q = 0; q < m.file.size(); ++q) {
+    const std::size_t fd = (std::size_t)m.file[q], e = (std::size_t)m.ent[q];
+    const int32_t r = m.ref[q];
+    if (r >= 0)
+      for (int d = 0; d < 3; ++d) uf[d][(std::size_t)r] = U[d][fd];
+    ks[e] = K[fd];
+    ws[e] = W[fd];
   }
-  check(nodals_h8_set(c.h, "p", p.data(), nc));
-  check(nodals_h8_set(c.h, "k", k.data(), ns));
-  check(nodals_h8_set(c.h, "omega", w.data(), ns));
-  std::printf("STAGE31_RESTART_LOAD=PASS path=%s iteration=%d next=%d fields=U_P_K_OMEGA\n", path.c_str(), it, it + 1);
+  for (int d = 0; d < 3; ++d) check(nodals_h8_set(c.h, ("u" + std::to_string(d)).c_str(), uf[d].data(), c.nv));
+  check(nodals_h8_set(c.h, "p", p.data(), c.nc));
+  check(nodals_h8_set(c.h, "k", ks.data(), c.ns));
+  check(nodals_h8_set(c.h, "omega", ws.data(), c.ns));
+  std::printf("STAGE31_RESTART_LOAD=PASS path=%s iteration=%llu next=%llu mode=WARM_RESTART "
+              "fields=FULL_BF3_U_P_K_OMEGA checksum=PASS\n",
+              path.c_str(), (unsigned long long)iter, (unsigned long long)iter + 1);
   std::fflush(stdout);
-  return it;
+  return (int)iter;
 }
 
 int run(int argc, char** argv) {

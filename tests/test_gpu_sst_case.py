@@ -4,6 +4,7 @@ into this build's native solver ``nodals_amd/bin/nodals_sst`` (HXT1 reader,
 wall-function SST path, checkpoints, VTU), then a restart from the
 iteration-10 checkpoint that must reproduce the uninterrupted run's
 iteration-20 state bit for bit."""
+import json
 import math
 import re
 import shutil
@@ -20,10 +21,32 @@ ROOT = Path(__file__).resolve().parents[1]
 CASE = ROOT / "tests" / "golden" / "sst_ubend_reference.case"
 
 
-def _payload(path: Path) -> bytes:
+def _stage31(path: Path) -> dict:
+    """Header and fields of a Stage31 checkpoint (stage31_restart.hpp:38-117:
+    144-byte header, FP64 u0 u1 u2 [nv], p [np], k omega [nv], FNV-1a 64
+    over the payload)."""
     data = path.read_bytes()
-    assert data[:8] == b"NODALSCK"
-    return data[8 + 4 + 4 + 24 + 64:]
+    assert data[:13] == b"NODALS_S31_R1" and data[13:16] == b"\0\0\0"
+    fmt_, nbytes = np.frombuffer(data[16:24], "<u4")
+    nv, npres, nvert, it = (int(x) for x in np.frombuffer(data[24:56], "<u8"))
+    (prev,) = np.frombuffer(data[56:64], "<f8")
+    fp = data[64:128].decode()
+    (h,) = np.frombuffer(data[136:144], "<u8")
+    assert fmt_ == 1 and nbytes == 8 and data[128] == 0 and np.isfinite(prev)
+    payload = data[144:]
+    assert len(payload) == 8 * (5 * nv + npres)
+    acc = 14695981039346656037
+    for b in payload:
+        acc = ((acc ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    assert acc == int(h)
+    f = np.frombuffer(payload, "<f8")
+    return {"nv": nv, "np": npres, "nvert": nvert, "it": it, "fingerprint": fp, "payload": payload,
+            "u": f[:3 * nv].reshape(3, nv), "p": f[3 * nv:3 * nv + npres], "k": f[3 * nv + npres:4 * nv + npres],
+            "omega": f[4 * nv + npres:]}
+
+
+def _payload(path: Path) -> bytes:
+    return _stage31(path)["payload"]
 
 
 @pytest.fixture(scope="module")
@@ -83,6 +106,21 @@ def test_sst_case_runs_checkpoints_and_restarts(helix_case):
     a = _payload(out1 / "NODALS_STAGE31_STATE_it20.ckpt")
     b = _payload(out2 / "NODALS_STAGE31_STATE_it20.ckpt")
     assert a == b
+    # the Stage31 header carries the HXT1 file's DOF counts and the input
+    # fingerprint; fields are in the file's velocity DOF numbering
+    from nodals_amd import hxt1
+
+    ck = _stage31(out1 / "NODALS_STAGE31_STATE_it10.ckpt")
+    mesh = next((d / "cache").rglob("HXT1_mesh.bin"))
+    counts = hxt1.read_hxt1(mesh)["counts"]
+    assert (ck["nv"], ck["np"], ck["nvert"], ck["it"]) == (counts["nvel"], counts["ntet"], counts["nv"], 10)
+    inv = json.loads((out1 / "invocation.json").read_text())
+    assert ck["fingerprint"] == inv["environment_overrides"]["STAGE31_INPUT_FINGERPRINT"]
+    assert np.all(np.isfinite(ck["u"]))
+    # masked-P1 k-omega: vertex values positive, face-bubble coefficients zero
+    nvert = ck["nvert"]
+    assert np.all(ck["k"][:nvert] > 0) and np.all(ck["omega"][:nvert] > 0)
+    assert np.all(ck["k"][nvert:] == 0) and np.all(ck["omega"][nvert:] == 0)
 
 
 def test_adaptive_runner_moves_wall_samples(helix_case):
