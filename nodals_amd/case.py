@@ -88,7 +88,7 @@ H8_SCHEMA = Schema(
         Key("momentum.relative_drop", "0.1", "--mom-drop", number),
         Key("momentum.omega", "1.0", "--mom-omega", number),
         Key("momentum.max_iterations", "20000", "--mom-max", integer),
-        # [supg]: translated and validated; the native parser rejects it on.
+        # [supg]: the implicit 64-point SUPG of the native solver.
         Key("supg.enabled", "false", "--supg", boolean),
         Key("supg.tau_scale", "0.05", "--supg-tau-scale", number),
         Key("supg.magic", "9.0", "--supg-magic", number),

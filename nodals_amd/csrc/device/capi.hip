@@ -323,7 +323,17 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
     } else if (n == "k" || n == "omega" || n.rfind("turb_", 0) == 0) {
       copyOut(s.turbField(n), host, count);
     }
-    else if (n == "mesh_dof") {
+    else if (n == "rans_stats") copyOut(s.ransStats(), host, count);
+    else if (n == "cell_points") {
+      const auto& M = s.mesh();
+      std::vector<double> v;
+      for (const auto& t : M.tets)
+        for (int a = 0; a < 4; ++a) {
+          const auto& X = M.points[(std::size_t)t[(std::size_t)a]];
+          v.insert(v.end(), X.begin(), X.end());
+        }
+      copyOut(v, host, count);
+    } else if (n == "mesh_dof") {
       std::vector<int32_t> v;
       for (const auto& d : s.mesh().fileDof) v.insert(v.end(), d.begin(), d.end());
       nodals::require(!v.empty(), "the mesh has no file DOF numbering");

@@ -10,6 +10,7 @@
 #include <thread>
 
 #include "h8_kernels.hpp"
+#include "rans_kernels.hpp"
 #include "sst_volume.hpp"
 #include "wall_kernels.hpp"
 #include "wall_state.hpp"
@@ -261,11 +262,18 @@ H8Options parseH8Args(int argc, const char* const* argv) {
   require(o.amgPrecision == "fp64" || o.amgPrecision == "fp32" || o.amgPrecision == "bf16",
           "amg-precision must be fp64, fp32 or bf16");
   require(o.pressureRestart >= 1 && o.pressureRestart <= 31, "pressure-restart must be in [1,31]");
+  // the reference binary's checks of the optional momentum terms
+  // (h8_main_base.cu:1853-1865)
+  require(o.mixlenScale >= 0.0 && std::isfinite(o.mixlenScale), "mixlen-scale must be finite and >=0");
+  require(!o.supg || o.supgTauScale > 0.0, "SUPG tau scale must be positive");
+  require(!o.supg || o.supgMagic > 0.0, "SUPG magic must be positive");
+  require(!o.supg || o.supgForm == "implicit", "SUPG currently supports form=implicit");
+  require(!o.supg || o.supgQuadPoints == 64, "SUPG currently supports quad-points=64");
+  require(!o.mixingLength || (!o.duct && o.lid.empty()),
+          "--mixing-length 1 is the straight-pipe Nikuradse profile (not for --duct or --lid)");
   // Features of the reference H8 binary that this build does not implement.
-  require(!o.mixingLength, "--mixing-length 1 is not supported by this build");
   require(!o.dgInlet, "--dg-inlet 1 is not supported by this build");
   require(!o.weakWall, "--weak-wall 1 is not supported by this build");
-  require(!o.supg, "--supg 1 is not supported by this build");
   return o;
 }
 
@@ -441,6 +449,12 @@ void H8Solver::upload() {
   cref_.upload(cref);
   cinvJ_.upload(invJ);
   cdet_.upload(det);
+  if (opt_.supg || opt_.mixingLength) {
+    std::vector<double> h2((std::size_t)nc_);
+    for (int c = 0; c < nc_; ++c) h2[(std::size_t)c] = S_.cells[(std::size_t)c].h2;
+    ch2_.upload(h2);
+    ransStats_.alloc((std::size_t)nc_ * 4);
+  }
   bcoef_.upload(S_.bcoef);
   std::vector<double> fv((std::size_t)nFix_ * 3);
   for (int s = 0; s < nFix_; ++s)
@@ -667,6 +681,7 @@ void H8Solver::assemble(double alphaU, double rauScale, bool convReady) {
   else
     convLocal(stream_, 8192);
   if (turbCouple_) eddyDiffusion();
+  if (opt_.mixingLength || opt_.supg) ransBlocks();
   MomRowsArgs a;
   a.mrow = mrow_.data();
   a.mdiagPos = mdiagPos_.data();
@@ -1058,6 +1073,46 @@ void H8Solver::sstGeometry() {
   sstQuad_.upload(keast24());
 }
 
+void H8Solver::ransBlocks() {
+  if (opt_.mixingLength) sstGeometry();
+  dev::RansArgs A;
+  A.nc = nc_;
+  A.cref = cref_.data();
+  A.cinvJ = cinvJ_.data();
+  A.cdet = cdet_.data();
+  A.ch2 = ch2_.data();
+  A.cxy = opt_.mixingLength ? sstXY_.data() : nullptr;
+  A.u = u_.data();
+  A.ufix = fixedVal_.data();
+  A.radius = S_.pipe.R;
+  A.nu = S_.pipe.nu;
+  A.mixing = opt_.mixingLength ? 1 : 0;
+  A.mixScale = opt_.mixlenScale;
+  A.supg = opt_.supg ? 1 : 0;
+  A.tauScale = opt_.supgTauScale;
+  A.magic = opt_.supgMagic;
+  A.cbuf = cbuf_.data();
+  A.stats = ransStats_.data();
+  // one wavefront per cell, eight per CU resident across the chip
+  const int grid = std::max(1, std::min(nc_, 8 * std::max(cuCount_, 1)));
+  hipLaunchKernelGGL(dev::k_rans64, grid, 64, 0, stream_, A, dev::collapsed64());
+  NODALS_LAUNCHED();
+}
+
+std::vector<double> H8Solver::ransStats() {
+  if (!opt_.mixingLength || ransStats_.size() == 0) return {0.0, 0.0, 0.0};
+  sync();
+  const std::vector<double> s = ransStats_.host();
+  double rw = 0, w = 0, rmax = 0, smax = 0;
+  for (int c = 0; c < nc_; ++c) {
+    rw += s[4 * (std::size_t)c];
+    w += s[4 * (std::size_t)c + 1];
+    rmax = std::max(rmax, s[4 * (std::size_t)c + 2]);
+    smax = std::max(smax, s[4 * (std::size_t)c + 3]);
+  }
+  return {w > 0 ? rw / w : 0.0, rmax, smax};
+}
+
 void H8Solver::sstVolume(int equation, const std::vector<double>& kFree, const std::vector<double>& kFixed,
                          const std::vector<double>& wFree, const std::vector<double>& wFixed, double kFloor,
                          double omegaFloor, int deviatoric, std::vector<double>& mat, std::vector<double>& rhs) {
@@ -1207,7 +1262,13 @@ H8RunResult H8Solver::run() {
       NODALS_HIP(hipStreamWaitEvent(stream2_, evAsm_, 0));
       refreshOn(stream2_);
       NODALS_HIP(hipEventRecord(evRef_, stream2_));
-      if (it == 1 && !turbCouple_) {  // the setup snapshot is of the laminar operator
+      if (it == 1 && !turbCouple_ && (o.mixingLength || o.supg)) {
+        // the live eddy viscosity / stabilisation changes rAU from the setup
+        // snapshot: the check is skipped (h8_main_base.cu:2052-2054)
+        if (verbose)
+          std::printf("NODALS_GPU_H8_SNAPSHOT_PARITY it=1 policy=SKIP_DYNAMIC_RANS reason=%s status=PASS\n",
+                      o.mixingLength ? "live_nut_changes_rAU" : "supg_changes_rAU");
+      } else if (it == 1 && !turbCouple_) {  // the setup snapshot is of the laminar operator
         const double rr = rauSnapshotRel();
         if (verbose)
           std::printf("NODALS_GPU_H8_SNAPSHOT_PARITY it=1 rAURel=%.3e tol=%.3e status=%s\n", rr, o.snapshotTol,

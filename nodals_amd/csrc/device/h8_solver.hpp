@@ -332,6 +332,16 @@ class H8Solver {
   double chebLambda_ = 1.0;
   DBuf<double> sstXY_, sstQuad_;
   void sstGeometry();
+  // Mixing-length and SUPG cell blocks (--mixing-length, --supg) added to
+  // cbuf_ in the assembly; per-cell largest squared edge and mixing-length
+  // statistics [nc][4].
+  DBuf<double> ch2_, ransStats_;
+  void ransBlocks();
+ public:
+  // Mixing-length summary of the last assembly: volume-weighted mean and
+  // maximum nu_t / nu, maximum strain rate (zeros when the term is off).
+  std::vector<double> ransStats();
+ private:
   // One k (equation 0) or omega (1) update of the transport step.
   int turbEquation(int equation, double* rel0 = nullptr, double* rel1 = nullptr);
   void turbAdvance();       // convection + k + omega, queued on the stream
