@@ -324,6 +324,12 @@ int nodals_h8_get(nodals_h8* h, const char* name, void* host, int64_t count) {
       copyOut(s.turbField(n), host, count);
     }
     else if (n == "rans_stats") copyOut(s.ransStats(), host, count);
+    else if (n == "dg_inlet_count") copyOut(std::vector<int64_t>{(int64_t)s.setup().dgFaces.size()}, host, count);
+    else if (n == "dg_inlet_faces") {
+      std::vector<int32_t> v;
+      for (const auto& F : s.setup().dgFaces) v.insert(v.end(), {F.cell, F.lf});
+      copyOut(v, host, count);
+    } else if (n == "plug") copyOut(s.setup().plug, host, count);
     else if (n == "cell_points") {
       const auto& M = s.mesh();
       std::vector<double> v;

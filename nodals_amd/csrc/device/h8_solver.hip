@@ -271,8 +271,8 @@ H8Options parseH8Args(int argc, const char* const* argv) {
   require(!o.supg || o.supgQuadPoints == 64, "SUPG currently supports quad-points=64");
   require(!o.mixingLength || (!o.duct && o.lid.empty()),
           "--mixing-length 1 is the straight-pipe Nikuradse profile (not for --duct or --lid)");
+  require(!o.dgInlet || (!o.duct && o.lid.empty()), "--dg-inlet 1 is a straight-pipe option (not for --duct or --lid)");
   // Features of the reference H8 binary that this build does not implement.
-  require(!o.dgInlet, "--dg-inlet 1 is not supported by this build");
   require(!o.weakWall, "--weak-wall 1 is not supported by this build");
   return o;
 }
@@ -294,6 +294,7 @@ H8Solver::H8Solver(const H8Options& o, bool withDevice) : opt_(o) {
   prm.rauScale = opt_.rauScale;
   prm.ductDiameter = opt_.ductDiameter;
   prm.inletVelocityGiven = opt_.inletVelocityGiven;
+  prm.dgInlet = opt_.dgInlet;
   for (int d = 0; d < 3; ++d) prm.inletVelocity[d] = opt_.inletVelocity[d];
   S_ = buildH8Setup(M_, prm);
   F_ = buildFineTopo(S_);
@@ -560,6 +561,53 @@ void H8Solver::upload() {
   srhs_.upload(aos(S_.staticRhs));
   u_.alloc((std::size_t)nv_ * 3);
   fill<double>(stream_, (long long)nv_ * 3, u_.data(), 0.0);
+  if (S_.dgInlet) {
+    // the plug start on every free vertex (h10_initialize_plug), the inlet
+    // faces, and each touched row's (face, basis) right-hand-side terms
+    std::vector<double> u0((std::size_t)nv_ * 3);
+    for (int i = 0; i < nv_; ++i)
+      for (int d = 0; d < 3; ++d) u0[(std::size_t)i * 3 + d] = S_.plug[(std::size_t)perm_[(std::size_t)i] * 3 + d];
+    u_.upload(u0);
+    const int nf = (int)S_.dgFaces.size();
+    std::vector<int> fc, flf;
+    std::vector<double> fg;
+    std::vector<std::vector<int>> terms((std::size_t)nv_);
+    for (int k = 0; k < nf; ++k) {
+      const DgInletFace& F = S_.dgFaces[(std::size_t)k];
+      fc.push_back(F.cell);
+      flf.push_back(F.lf);
+      fg.insert(fg.end(), {F.area, F.n[0], F.n[1], F.n[2]});
+      for (int a = 0; a < 8; ++a) {
+        const int r = S_.cells[(std::size_t)F.cell].ref[a];
+        if (r >= 0) terms[(std::size_t)pinv_[(std::size_t)r]].push_back(k * 8 + a);
+      }
+    }
+    std::vector<int> rows, off{0}, list;
+    for (int i = 0; i < nv_; ++i)
+      if (!terms[(std::size_t)i].empty()) {
+        rows.push_back(i);
+        list.insert(list.end(), terms[(std::size_t)i].begin(), terms[(std::size_t)i].end());
+        off.push_back((int)list.size());
+      }
+    if (nf == 0) {
+      fc.push_back(0);
+      flf.push_back(0);
+      fg.assign(4, 0.0);
+    }
+    if (rows.empty()) {
+      rows.push_back(0);
+      list.push_back(0);
+    }
+    dgCell_.upload(fc);
+    dgLf_.upload(flf);
+    dgGeom_.upload(fg);
+    dgFaceRhs_.alloc((std::size_t)std::max(nf, 1) * 24);
+    dgRows_.upload(rows);
+    dgOff_.upload(off);
+    dgList_.upload(list);
+    dgNf_ = nf;
+    dgNrows_ = (int)off.size() - 1;
+  }
   brhs_.alloc((std::size_t)nv_ * 3);
   for (DBuf<double>* b : {&delta_, &diag_, &rau_}) b->alloc((std::size_t)nv_);
   std::vector<double> rs((std::size_t)nv_);
@@ -682,6 +730,7 @@ void H8Solver::assemble(double alphaU, double rauScale, bool convReady) {
     convLocal(stream_, 8192);
   if (turbCouple_) eddyDiffusion();
   if (opt_.mixingLength || opt_.supg) ransBlocks();
+  if (S_.dgInlet) dgInletBlocks();
   MomRowsArgs a;
   a.mrow = mrow_.data();
   a.mdiagPos = mdiagPos_.data();
@@ -708,6 +757,11 @@ void H8Solver::assemble(double alphaU, double rauScale, bool convReady) {
   hipLaunchKernelGGL((k_mom_rows<kLongLanes, kShortLanes>), nbL + nbS, 256, 0, stream_, a, nLong_, longRows_.data(),
                      nShort_, shortRows_.data(), nbL);
   NODALS_LAUNCHED();
+  if (S_.dgInlet && dgNrows_ > 0) {
+    hipLaunchKernelGGL(dev::k_dg_rhs, blocksFor(dgNrows_), 256, 0, stream_, dgNrows_, dgRows_.data(), dgOff_.data(),
+                       dgList_.data(), dgFaceRhs_.data(), brhs_.data());
+    NODALS_LAUNCHED();
+  }
   if (rauScale != 1.0) {
     hipLaunchKernelGGL(k_scale_into<double>, blocksFor(nv_), 256, 0, stream_, (long long)nv_, rauScale, rau_.data(),
                        rau_.data());
@@ -1099,6 +1153,33 @@ void H8Solver::ransBlocks() {
   NODALS_LAUNCHED();
 }
 
+void H8Solver::dgInletBlocks() {
+  if (dgNf_ == 0) return;
+  if (opt_.mixingLength) sstGeometry();
+  dev::DgInletArgs A;
+  A.nf = dgNf_;
+  A.cell = dgCell_.data();
+  A.lf = dgLf_.data();
+  A.geom = dgGeom_.data();
+  A.cref = cref_.data();
+  A.cinvJ = cinvJ_.data();
+  A.cxy = opt_.mixingLength ? sstXY_.data() : nullptr;
+  A.u = u_.data();
+  A.ufix = fixedVal_.data();
+  A.nu = S_.pipe.nu;
+  A.radius = S_.pipe.R;
+  A.mixScale = opt_.mixlenScale;
+  A.mixing = opt_.mixingLength ? 1 : 0;
+  for (int d = 0; d < 3; ++d) A.uh[d] = S_.pipe.inletVelocity[d];
+  A.cbuf = cbuf_.data();
+  A.faceRhs = dgFaceRhs_.data();
+  dev::Tri12Rule T;
+  for (int q = 0; q < 12; ++q)
+    for (int j = 0; j < 4; ++j) T.q[q][j] = kTri12[q][j];
+  hipLaunchKernelGGL(dev::k_dg_inlet, (dgNf_ + 63) / 64, 64, 0, stream_, A, T);
+  NODALS_LAUNCHED();
+}
+
 std::vector<double> H8Solver::ransStats() {
   if (!opt_.mixingLength || ransStats_.size() == 0) return {0.0, 0.0, 0.0};
   sync();
@@ -1262,12 +1343,13 @@ H8RunResult H8Solver::run() {
       NODALS_HIP(hipStreamWaitEvent(stream2_, evAsm_, 0));
       refreshOn(stream2_);
       NODALS_HIP(hipEventRecord(evRef_, stream2_));
-      if (it == 1 && !turbCouple_ && (o.mixingLength || o.supg)) {
-        // the live eddy viscosity / stabilisation changes rAU from the setup
-        // snapshot: the check is skipped (h8_main_base.cu:2052-2054)
+      if (it == 1 && !turbCouple_ && (o.mixingLength || o.supg || o.dgInlet)) {
+        // the live eddy viscosity, the DG inlet terms or the stabilisation
+        // change rAU from the setup snapshot: the check is skipped
+        // (h8_main_base.cu:2052-2054)
         if (verbose)
           std::printf("NODALS_GPU_H8_SNAPSHOT_PARITY it=1 policy=SKIP_DYNAMIC_RANS reason=%s status=PASS\n",
-                      o.mixingLength ? "live_nut_changes_rAU" : "supg_changes_rAU");
+                      o.mixingLength ? "live_nut_changes_rAU" : (o.dgInlet ? "dg_inlet_changes_rAU" : "supg_changes_rAU"));
       } else if (it == 1 && !turbCouple_) {  // the setup snapshot is of the laminar operator
         const double rr = rauSnapshotRel();
         if (verbose)

@@ -337,6 +337,12 @@ class H8Solver {
   // statistics [nc][4].
   DBuf<double> ch2_, ransStats_;
   void ransBlocks();
+  // DG weak inlet (--dg-inlet): inlet faces, their right-hand sides and the
+  // rows those reach
+  DBuf<int> dgCell_, dgLf_, dgRows_, dgOff_, dgList_;
+  DBuf<double> dgGeom_, dgFaceRhs_;
+  int dgNf_ = 0, dgNrows_ = 0;
+  void dgInletBlocks();
  public:
   // Mixing-length summary of the last assembly: volume-weighted mean and
   // maximum nu_t / nu, maximum strain rate (zeros when the term is off).

@@ -186,5 +186,141 @@ static __global__ void __launch_bounds__(64) k_rans64(RansArgs A, Collapsed64 Q)
   }
 }
 
+// Twelve-point triangle rule rows (barycentrics on the face's three
+// vertices in increasing local order, weight; weights sum to 1).
+struct Tri12Rule {
+  double q[12][4];
+};
+
+struct DgInletArgs {
+  int nf;
+  const int* cell;       // [nf] inlet cell (one inlet face per cell)
+  const int* lf;         // [nf] local inlet face (opposite vertex lf)
+  const double* geom;    // [nf][4] area, outward unit normal
+  const int* cref;
+  const double* cinvJ;
+  const double* cxy;     // mixing length: vertex x, y relative to the axis
+  const double* u;
+  const double* ufix;
+  double nu, radius, mixScale;
+  int mixing;
+  double uh[3];          // prescribed inlet velocity
+  double* cbuf;          // [nc][64] cell blocks (added to)
+  double* faceRhs;       // [nf][8][3]
+};
+
+// DG weak inlet, one thread per inlet face (gpu/serial_cuda/h10_dg_inlet.cuh:
+// 94-133): on the face's 12-point rule, with beta = max(0, -uh.n) and
+// nu_e = nu + nu_t (the mixing length when it is on),
+//   G_ab += w (beta phi_a phi_b + nu_e (dn(phi_a) phi_b - phi_a dn(phi_b)))
+//   R_a  += w (beta phi_a + nu_e dn(phi_a)) uh
+// G joins the cell block (matrix and Dirichlet columns), R the momentum
+// right-hand side through k_dg_rhs.
+static __global__ void __launch_bounds__(64) k_dg_inlet(DgInletArgs A, Tri12Rule T) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.nf) return;
+  const int c = A.cell[k], o = A.lf[k];
+  const double* g = A.geom + 4ll * k;
+  const double area = g[0], n0 = g[1], n1 = g[2], n2 = g[3];
+  const double* I = A.cinvJ + 9ll * c;
+  double GL[4][3];
+  for (int d = 0; d < 3; ++d) {
+    GL[1][d] = I[d];
+    GL[2][d] = I[3 + d];
+    GL[3][d] = I[6 + d];
+    GL[0][d] = -(I[d] + I[3 + d] + I[6 + d]);
+  }
+  double U[8][3];
+  for (int a = 0; a < 8; ++a) {
+    const int r = A.cref[8ll * c + a];
+    const double* v = r >= 0 ? A.u + 3ll * r : A.ufix + 3ll * (-r - 1);
+    U[a][0] = v[0];
+    U[a][1] = v[1];
+    U[a][2] = v[2];
+  }
+  int fv[3], m = 0;
+  for (int i = 0; i < 4; ++i)
+    if (i != o) fv[m++] = i;
+  const double bn = A.uh[0] * n0 + A.uh[1] * n1 + A.uh[2] * n2;
+  const double inflow = fmax(0.0, -bn);
+  double G[64], R[24];
+  for (int i = 0; i < 64; ++i) G[i] = 0.0;
+  for (int i = 0; i < 24; ++i) R[i] = 0.0;
+  for (int q = 0; q < 12; ++q) {
+    double lam[4] = {0.0, 0.0, 0.0, 0.0};
+    lam[fv[0]] = T.q[q][0];
+    lam[fv[1]] = T.q[q][1];
+    lam[fv[2]] = T.q[q][2];
+    double val[8], gr[8][3], dn[8];
+    for (int a = 0; a < 8; ++a) {
+      if (a < 4) {
+        val[a] = lam[a];
+        for (int d = 0; d < 3; ++d) gr[a][d] = GL[a][d];
+      } else {
+        const int b = a - 4;
+        const int i = b == 0 ? 1 : 0, j = b <= 1 ? 2 : 1, l = b <= 2 ? 3 : 2;
+        val[a] = 27.0 * lam[i] * lam[j] * lam[l];
+        for (int d = 0; d < 3; ++d)
+          gr[a][d] = 27.0 * (lam[j] * lam[l] * GL[i][d] + lam[i] * lam[l] * GL[j][d] + lam[i] * lam[j] * GL[l][d]);
+      }
+      dn[a] = gr[a][0] * n0 + gr[a][1] * n1 + gr[a][2] * n2;
+    }
+    double nut = 0.0;
+    if (A.mixing) {
+      double gu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) gu[i][j] += U[a][i] * gr[a][j];
+      double ss = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const double sij = 0.5 * (gu[i][j] + gu[j][i]);
+          ss += sij * sij;
+        }
+      const double strain = sqrt(fmax(0.0, 2.0 * ss));
+      const double* xy = A.cxy + 8ll * c;
+      double xq = 0.0, yq = 0.0;
+      for (int i = 0; i < 4; ++i) {
+        xq += lam[i] * xy[2 * i];
+        yq += lam[i] * xy[2 * i + 1];
+      }
+      const double eta = fmin(1.0, fmax(0.0, sqrt(xq * xq + yq * yq) / A.radius));
+      const double e2 = eta * eta;
+      const double ell = A.radius * fmax(0.0, 0.14 - 0.08 * e2 - 0.06 * e2 * e2);
+      nut = A.mixScale * ell * ell * strain;
+    }
+    const double nue = A.nu + nut, w = T.q[q][3] * area;
+    for (int a = 0; a < 8; ++a) {
+      for (int b = 0; b < 8; ++b) G[8 * a + b] += w * (inflow * val[a] * val[b] + nue * (-val[a] * dn[b] + dn[a] * val[b]));
+      for (int d = 0; d < 3; ++d) R[3 * a + d] += w * (inflow * val[a] * A.uh[d] + nue * dn[a] * A.uh[d]);
+    }
+  }
+  double* blk = A.cbuf + 64ll * c;
+  for (int i = 0; i < 64; ++i) blk[i] += G[i];
+  double* fr = A.faceRhs + 24ll * k;
+  for (int i = 0; i < 24; ++i) fr[i] = R[i];
+}
+
+// Adds the inlet faces' right-hand sides to their free rows, one thread per
+// row, terms in (face, basis) order: rows[r] (device numbering) sums
+// faceRhs over list[off[r] .. off[r+1]) (face * 8 + basis).
+static __global__ void k_dg_rhs(int nrows, const int* __restrict__ rows, const int* __restrict__ off,
+                                const int* __restrict__ list, const double* __restrict__ faceRhs,
+                                double* __restrict__ brhs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int t = off[r]; t < off[r + 1]; ++t) {
+    const double* f = faceRhs + 3ll * list[t];
+    s0 += f[0];
+    s1 += f[1];
+    s2 += f[2];
+  }
+  const long long i = rows[r];
+  brhs[3 * i] += s0;
+  brhs[3 * i + 1] += s1;
+  brhs[3 * i + 2] += s2;
+}
+
 }  // namespace dev
 }  // namespace nodals

@@ -422,8 +422,13 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
   // (g3_host.hpp:29-55).
   S.fixed.assign((std::size_t)np + nf, 0);
   S.wallFunction = S.duct && prm.wallFunction;
+  require(!prm.dgInlet || (!S.cavity && !S.duct), "the DG inlet is a straight-pipe option (not for --lid or --duct)");
+  S.dgInlet = prm.dgInlet;
   for (int f = ni; f < nf; ++f) {
     if (M.facePatch[(std::size_t)f] == S.pipe.outlet) continue;
+    // DG inlet: the inlet's entities stay free (the wall keeps the vertices
+    // it shares with the inlet)
+    if (S.dgInlet && M.facePatch[(std::size_t)f] == S.pipe.inlet) continue;
     // Wall-function walls are not Dirichlet: their vertex and face DOFs stay
     // free and carry the weak Nitsche/Spalding wall terms.
     if (S.wallFunction && M.facePatch[(std::size_t)f] == S.pipe.wall) continue;
@@ -499,13 +504,13 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
       }
     }
   }
-  for (int v = 0; v < np && !S.cavity && !S.duct; ++v)
+  for (int v = 0; v < np && !S.cavity && !S.duct && !S.dgInlet; ++v)
     if (onInlet[(std::size_t)v] && !onWall[(std::size_t)v]) {
       const int s = S.fixedSlot[(std::size_t)v];
       S.fixedValue[(std::size_t)s][2] =
           S.pipe.profileScale * idealUz(S.pipe, M.points[(std::size_t)v][0], M.points[(std::size_t)v][1]);
     }
-  if (!S.cavity && !S.duct) {
+  if (!S.cavity && !S.duct && !S.dgInlet) {
     const Patch& pin = M.patches[(std::size_t)S.pipe.inlet];
     for (int f = pin.start; f < pin.start + pin.count; ++f) {
       const auto& F = M.faces[(std::size_t)f];
@@ -676,16 +681,58 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
   S.fixedDiv.assign((std::size_t)nc, 0.0);
   S.volumes.assign((std::size_t)nc, 0.0);
   S.cellZ.assign((std::size_t)nc, 0.0);
+  // DG inlet: each cell's inlet face (one at most) and its outward area
+  // vector (cell_inlet_face_g2, g2_host.hpp:45-50)
+  std::vector<int> inletOpp;
+  std::vector<V3> inletSf;
+  if (S.dgInlet) {
+    inletOpp.assign((std::size_t)nc, -1);
+    inletSf.assign((std::size_t)nc, V3{0, 0, 0});
+    const Patch& pin = M.patches[(std::size_t)S.pipe.inlet];
+    for (int f = pin.start; f < pin.start + pin.count; ++f) {
+      const int c = M.owner[(std::size_t)f];
+      int lf = -1;
+      for (int i = 0; i < 4; ++i)
+        if (M.oppFace[(std::size_t)c][(std::size_t)i] == f) lf = i;
+      require(lf >= 0, "inlet face " + std::to_string(f) + " is not a face of its owner cell", kErrInternal);
+      require(inletOpp[(std::size_t)c] < 0, "tet has multiple DG inlet faces");
+      inletOpp[(std::size_t)c] = lf;
+      inletSf[(std::size_t)c] = outwardAreaVector(M, f);
+      DgInletFace q;
+      q.cell = c;
+      q.lf = lf;
+      const V3& sf = inletSf[(std::size_t)c];
+      q.area = std::sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+      require(q.area > 0.0, "degenerate DG inlet face");
+      for (int d = 0; d < 3; ++d) q.n[d] = sf[(std::size_t)d] / q.area;
+      S.dgFaces.push_back(q);
+    }
+  }
+  // Effective B: the DG inlet removes the inlet face's trace, -sf/3 on the
+  // face's vertices and -(9/20) sf on its bubble (g4_effective_bcoeff_host,
+  // g4_host.hpp:77-86, same operation order).
+  auto effB = [&](int c, const CellPlan& cp, int a, int d) {
+    double v = bcoefOf(cp, a, d);
+    if (S.dgInlet && inletOpp[(std::size_t)c] >= 0) {
+      const int o = inletOpp[(std::size_t)c];
+      if (a < 4 && a != o) v -= inletSf[(std::size_t)c][(std::size_t)d] / 3.0;
+      else if (a == 4 + o) v -= (9.0 / 20.0) * inletSf[(std::size_t)c][(std::size_t)d];
+    }
+    return v;
+  };
   for (int c = 0; c < nc; ++c) {
     const CellPlan& cp = S.cells[(std::size_t)c];
     for (int a = 0; a < 8; ++a)
-      for (int d = 0; d < 3; ++d) S.bcoef[(std::size_t)c * 24 + (std::size_t)(3 * a + d)] = bcoefOf(cp, a, d);
+      for (int d = 0; d < 3; ++d) S.bcoef[(std::size_t)c * 24 + (std::size_t)(3 * a + d)] = effB(c, cp, a, d);
     S.volumes[(std::size_t)c] = cp.det / 6.0;
     double q = 0;
+    if (S.dgInlet && inletOpp[(std::size_t)c] >= 0)  // the prescribed inflow flux (g4_host.hpp:313)
+      q += S.pipe.inletVelocity[0] * inletSf[(std::size_t)c][0] + S.pipe.inletVelocity[1] * inletSf[(std::size_t)c][1] +
+           S.pipe.inletVelocity[2] * inletSf[(std::size_t)c][2];
     for (int a = 0; a < 8; ++a) {
       if (cp.ref[a] >= 0) continue;
       const auto& fv = S.fixedValue[(std::size_t)(-cp.ref[a] - 1)];
-      for (int d = 0; d < 3; ++d) q += bcoefOf(cp, a, d) * fv[(std::size_t)d];
+      for (int d = 0; d < 3; ++d) q += effB(c, cp, a, d) * fv[(std::size_t)d];
     }
     S.fixedDiv[(std::size_t)c] = q;
     double z = 0;
@@ -723,9 +770,20 @@ H8Setup buildH8Setup(const TetMesh& M, const H8Params& prm) {
     S.zMax = std::max(S.zMax, x[2]);
   }
 
-  // Setup rAU from the zero-velocity assembly (h8_main_base.cu:1917-1922).
+  // Setup rAU from the zero-velocity assembly (h8_main_base.cu:1917-1922);
+  // with the DG inlet from the plug on every free vertex (:1918).
   std::array<std::vector<double>, 3> U0;
   for (auto& u : U0) u.assign((std::size_t)n, 0.0);
+  if (S.dgInlet) {
+    for (int v = 0; v < np; ++v) {
+      const int g = S.g2free[(std::size_t)v];
+      if (g < 0) continue;
+      for (int d = 0; d < 3; ++d) U0[(std::size_t)d][(std::size_t)g] = S.pipe.inletVelocity[d];
+    }
+    S.plug.assign((std::size_t)n * 3, 0.0);
+    for (int g = 0; g < n; ++g)
+      for (int d = 0; d < 3; ++d) S.plug[(std::size_t)g * 3 + d] = U0[(std::size_t)d][(std::size_t)g];
+  }
   std::vector<double> a0;
   std::array<std::vector<double>, 3> c0;
   hostAssembleCentral(S, U0, a0, c0);

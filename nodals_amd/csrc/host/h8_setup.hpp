@@ -93,10 +93,23 @@ struct H8Params {
   double ductDiameter = 0.0;
   bool inletVelocityGiven = false;
   double inletVelocity[3] = {0, 0, 0};
+  // Straight pipe only: the DG weak inlet (build_g4_setup's dgInlet,
+  // gpu/serial_cuda/g4_host.hpp:289-313): inlet DOFs free, the divergence
+  // without the inlet face trace and the inlet flux prescribed in the fixed
+  // divergence; the setup assembly starts from the plug.
+  bool dgInlet = false;
   double re = 20.0, bulk = 1.0;
   double alphaU = 0.5;
   double rauScale = 1.0;
   int threads = 0;  // 0: hardware concurrency (capped at 16)
+};
+
+// A DG inlet face seen from its cell: local face lf (opposite vertex lf),
+// area and outward unit normal.
+struct DgInletFace {
+  std::int32_t cell = -1, lf = -1;
+  double area = 0;
+  double n[3] = {0, 0, 0};
 };
 
 struct H8Setup {
@@ -104,6 +117,9 @@ struct H8Setup {
   bool cavity = false;
   bool duct = false;
   bool wallFunction = false;
+  bool dgInlet = false;
+  std::vector<DgInletFace> dgFaces;   // DG inlet faces, in inlet patch order
+  std::vector<double> plug;           // [nv][3] start velocity of the DG inlet: the plug on free vertices
   int pinCell = -1;  // pressure row pinned (cavity), -1 for none
   // duct mode: cells next to the inlet / outlet with their face areas, and
   // the distance of every vertex to the nearest wall face

@@ -713,12 +713,14 @@ def pressure_drop_fit(case: H8Case, p: np.ndarray) -> float:
 def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work="fgs1", p_solver="pcg",
            p_rtol=0.5, p_atol=1e-12, p_max=20, smoother: Optional[Smoother] = None, run_mode="fixed10",
            max_outer=10, simple_tol=1e-6, restart=30, U0=None, p0=None, callback=None, extra_blocks=None,
-           mom_sweeps=1, cheb_degree=3, cheb_power_its=6, cheb_low_fraction=0.05, cheb_power_safety=1.15):
+           mom_sweeps=1, cheb_degree=3, cheb_power_its=6, cheb_low_fraction=0.05, cheb_power_safety=1.15,
+           extra_rhs=None):
     """H8 SIMPLE loop (gpu/serial_cuda/h8_main_base.cu:2046-2119): assemble,
     B^T p + RHS, fixed momentum work, continuity, fine refresh, pressure
     Krylov + AMG, p += alpha_p dp; converge mode audits the momentum residual
     once the continuity target is met.  extra_blocks(U), when given, adds
-    cell blocks to the momentum assembly; callback(it, U, p) runs after the
+    cell blocks to the momentum assembly and extra_rhs(U) a (3,nv) term to
+    its right-hand side (the DG inlet); callback(it, U, p) runs after the
     pressure update (the coupled SST mode advances k and omega there).
     mom_sweeps repeats the momentum work unit that many times per
     iteration, as the SST path's fixed M3 sweep count
@@ -744,6 +746,8 @@ def simple(case: H8Case, alpha_u=0.5, alpha_p=0.5, mom_omega=1.0, momentum_work=
         delta, diag, rau = finalize_relax(case, aval, alpha_u)
         bt = bt_apply(case, p)
         b = momentum_rhs(case, conv, bt, delta, U)
+        if extra_rhs is not None:
+            b = b + extra_rhs(U)
         A = mom_matrix(case, aval)
         audit = None
         if run_mode == "converge" and it > 1 and cont_rel <= simple_tol:

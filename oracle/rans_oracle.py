@@ -1,5 +1,6 @@
 """CPU oracle of the H8 binary's optional momentum terms: the Nikuradse
-mixing-length eddy viscosity and the implicit SUPG stabilisation.
+mixing-length eddy viscosity, the implicit SUPG stabilisation and the DG
+weak inlet.
 
 TEST INFRASTRUCTURE ONLY.  Imported by ``tests/`` as the checker; never by
 the product package.
@@ -14,7 +15,11 @@ phi_a)(a.grad phi_b - nu lap phi_b)), both with the lagged velocity and the
 the quadrature loops at :141-149), assembled like the convection
 (h8_main_base.cu:1137-1148: added to the momentum matrix, Dirichlet columns
 to the right-hand side).  The rule here comes from scipy's Gauss-Jacobi
-and Gauss-Legendre roots.
+and Gauss-Legendre roots.  The DG inlet restates
+gpu/serial_cuda/h10_dg_inlet.cuh:94-133: per inlet face, beta = max(0,
+-uh.n), nu_e = nu + nu_t, G_ab = sum_q w (beta phi_a phi_b + nu_e (dn phi_a
+phi_b - phi_a dn phi_b)), R_a = sum_q w (beta phi_a + nu_e dn phi_a) uh on
+Dunavant's 12-point degree-6 triangle rule.
 """
 from __future__ import annotations
 
@@ -141,3 +146,100 @@ def extra_blocks_fn(case, points, nu, mixing=None, supg=None):
         return blocks(points, O.cell_values(case, U), nu, mixing, supg)[0]
 
     return f
+
+
+# Dunavant's 12-point degree-6 triangle rule: (l0, l1, l2, weight), weights
+# summing to 1 (Dunavant 1985, Int. J. Numer. Meth. Eng. 21, table entry 6).
+TRI12 = np.array([
+    [0.063089014491502, 0.063089014491502, 0.873821971016996, 0.050844906370207],
+    [0.063089014491502, 0.873821971016996, 0.063089014491502, 0.050844906370207],
+    [0.873821971016996, 0.063089014491502, 0.063089014491502, 0.050844906370207],
+    [0.249286745170910, 0.249286745170910, 0.501426509658180, 0.116786275726379],
+    [0.249286745170910, 0.501426509658180, 0.249286745170910, 0.116786275726379],
+    [0.501426509658180, 0.249286745170910, 0.249286745170910, 0.116786275726379],
+    [0.053145049844816, 0.310352451033785, 0.636502499121399, 0.082851075618374],
+    [0.053145049844816, 0.636502499121399, 0.310352451033785, 0.082851075618374],
+    [0.310352451033785, 0.053145049844816, 0.636502499121399, 0.082851075618374],
+    [0.310352451033785, 0.636502499121399, 0.053145049844816, 0.082851075618374],
+    [0.636502499121399, 0.053145049844816, 0.310352451033785, 0.082851075618374],
+    [0.636502499121399, 0.310352451033785, 0.053145049844816, 0.082851075618374]])
+
+
+def face_frame(points, lf):
+    """Area (nf,) and outward unit normal (nf,3) of local face lf (opposite
+    vertex lf) of cells with vertex coordinates points (nf,4,3)."""
+    idx = np.array([[j for j in range(4) if j != o] for o in range(4)])[lf]    # (nf,3)
+    V = np.take_along_axis(points, idx[:, :, None], axis=1)
+    sf = 0.5 * np.cross(V[:, 1] - V[:, 0], V[:, 2] - V[:, 0])
+    opp = points[np.arange(len(lf)), lf]
+    flip = np.einsum("fd,fd->f", sf, V[:, 0] - opp) < 0
+    sf[flip] *= -1
+    area = np.linalg.norm(sf, axis=1)
+    return area, sf / area[:, None]
+
+
+def dg_inlet(points, faces, Uc, nu, uh, mixing=None, rule=TRI12):
+    """G (nf,8,8) and R (nf,8,3) of the DG inlet faces (cell, lf) with the
+    cells' velocities Uc (nf,8,3) and vertex coordinates points (nf,4,3)."""
+    lf = np.asarray(faces)[:, 1]
+    nf = len(lf)
+    GL, _, _ = geometry(points)
+    area, n = face_frame(points, lf)
+    uh = np.asarray(uh, float)
+    inflow = np.maximum(0.0, -(n @ uh))
+    fv = np.array([[j for j in range(4) if j != o] for o in range(4)])[lf]
+    G = np.zeros((nf, 8, 8))
+    R = np.zeros((nf, 8, 3))
+    for q in range(len(rule)):
+        lam = np.zeros((nf, 4))
+        for j in range(3):
+            lam[np.arange(nf), fv[:, j]] = rule[q, j]
+        phi, grad, _ = basis(lam, GL)
+        dn = np.einsum("fad,fd->fa", grad, n)
+        nue = np.full(nf, nu)
+        if mixing is not None:
+            scale, Rr, cx, cy = mixing
+            gu = np.einsum("fmi,fmj->fij", Uc, grad)
+            S = 0.5 * (gu + np.swapaxes(gu, 1, 2))
+            strain = np.sqrt(np.maximum(0.0, 2 * (S * S).sum((1, 2))))
+            xq = (lam * points[:, :, 0]).sum(-1)
+            yq = (lam * points[:, :, 1]).sum(-1)
+            eta = np.clip(np.hypot(xq - cx, yq - cy) / Rr, 0.0, 1.0)
+            ell = Rr * np.maximum(0.0, 0.14 - 0.08 * eta ** 2 - 0.06 * eta ** 4)
+            nue = nue + scale * ell * ell * strain
+        w = rule[q, 3] * area
+        G += w[:, None, None] * (inflow[:, None, None] * phi[:, :, None] * phi[:, None, :]
+                                 + nue[:, None, None] * (dn[:, :, None] * phi[:, None, :] - phi[:, :, None] * dn[:, None, :]))
+        R += (w * inflow)[:, None, None] * phi[:, :, None] * uh[None, None, :] \
+            + (w * nue)[:, None, None] * dn[:, :, None] * uh[None, None, :]
+    return G, R
+
+
+def dg_inlet_fns(case, points_all, faces, nu, uh, mixing=None):
+    """(extra_blocks(U), extra_rhs(U)) for h8_oracle.simple: the DG inlet's
+    cell blocks and its right-hand side on the free rows."""
+    from . import h8_oracle as O
+
+    faces = np.asarray(faces)
+    cells = faces[:, 0]
+
+    def terms(U):
+        Uc = O.cell_values(case, U)[cells]
+        return dg_inlet(points_all[cells], faces, Uc, nu, uh, mixing)
+
+    def blocks_fn(U):
+        G, _ = terms(U)
+        out = np.zeros((case.nc, 8, 8))
+        np.add.at(out, cells, G)
+        return out
+
+    def rhs_fn(U):
+        _, R = terms(U)
+        b = np.zeros((3, case.nv))
+        ref = case.cell_ref[cells]                     # (nf,8)
+        free = ref >= 0
+        for d in range(3):
+            np.add.at(b[d], ref[free], R[:, :, d][free])
+        return b
+
+    return blocks_fn, rhs_fn

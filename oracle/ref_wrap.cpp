@@ -132,6 +132,35 @@ void* ref_open(const char* mesh_dir, double re, double bulk, const char* wall,
   }
 }
 
+// The same setup with the DG weak inlet (build_g4_setup's dgInlet): inlet
+// DOFs free, effective B and prescribed-flux fixed divergence; the setup
+// assembly starts from the plug on every free vertex (h8_main_base.cu:1918).
+void* ref_open_dg(const char* mesh_dir, double re, double bulk, const char* wall, const char* inlet,
+                  const char* outlet, double alpha_u) {
+  try {
+    auto R = std::make_unique<RefCase>();
+    R->mesh = load_foam_tet_mesh(mesh_dir);
+    R->setup = build_g4_setup(R->mesh, re, bulk, wall, inlet, outlet, true, false);
+    std::array<std::vector<double>, 3> U0;
+    for (auto& u : U0) u.assign((std::size_t)R->setup.topo.n, 0.0);
+    for (std::size_t v = 0; v < R->mesh.points.size(); ++v) {
+      const int g = R->setup.momMask.g2free[v];
+      if (g < 0) continue;
+      for (int d = 0; d < 3; ++d) U0[(std::size_t)d][(std::size_t)g] = R->setup.pipe.inletVelocity[d];
+    }
+    std::vector<double> hostA;
+    std::array<std::vector<double>, 3> conv;
+    host_assemble_central_g4(R->setup, U0, hostA, conv);
+    R->initRau = host_finalize_relax_g4(R->setup, hostA, alpha_u, &R->initDelta);
+    R->setup.pressure.rAU = R->initRau;
+    publish_setup(*R);
+    return R.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
 void ref_close(void* h) { delete static_cast<RefCase*>(h); }
 
 // Empty handle for the mesh-free entry points (ref_sa_from_csr).

@@ -20,10 +20,11 @@ def available() -> bool:
 
 class RefCase:
     def __init__(self, mesh: str, re: float = 20.0, bulk: float = 1.0, wall: str = "patch_0_0",
-                 inlet: str = "patch_2_0", outlet: str = "patch_1_0", alpha_u: float = 0.5):
+                 inlet: str = "patch_2_0", outlet: str = "patch_1_0", alpha_u: float = 0.5, dg_inlet: bool = False):
         L = C.CDLL(str(REF_LIB))
-        L.ref_open.restype = C.c_void_p
-        L.ref_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double]
+        for fn in (L.ref_open, L.ref_open_dg):
+            fn.restype = C.c_void_p
+            fn.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double]
         L.ref_close.argtypes = [C.c_void_p]
         L.ref_size.restype = C.c_longlong
         L.ref_size.argtypes = [C.c_void_p, C.c_char_p]
@@ -35,7 +36,8 @@ class RefCase:
         L.ref_pressure_drop.argtypes = [C.c_void_p, C.c_void_p]
         L.ref_last_error.restype = C.c_char_p
         self.L = L
-        self.h = L.ref_open(str(mesh).encode(), re, bulk, wall.encode(), inlet.encode(), outlet.encode(), alpha_u)
+        opener = L.ref_open_dg if dg_inlet else L.ref_open
+        self.h = opener(str(mesh).encode(), re, bulk, wall.encode(), inlet.encode(), outlet.encode(), alpha_u)
         if not self.h:
             raise RuntimeError("reference setup failed: " + L.ref_last_error().decode())
         self.nv = self.get("mom_row", np.int64).size - 1

@@ -46,7 +46,7 @@ def test_usage_errors_are_status_codes(mesh_dir):
     with pytest.raises(NodalsError, match="fixed10 requires maxOuter=10"):
         H8(["--mesh", mesh_dir["small"]], device=False)
     with pytest.raises(NodalsError, match="not supported by this build"):
-        H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--dg-inlet", "1"], device=False)
+        H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--weak-wall", "1"], device=False)
     with pytest.raises(NodalsError, match="cannot open"):
         H8(["--mesh", "/nonexistent/polyMesh", "--max-outer", "10"], device=False)
     for n in ("0", "61"):

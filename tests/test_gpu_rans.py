@@ -1,5 +1,6 @@
 """The H8 binary's optional momentum terms on the GPU (--mixing-length,
---supg; gpu/serial_cuda/h9_rans_mixlen.cuh, h7_momentum_assembly.cuh):
+--supg, --dg-inlet; gpu/serial_cuda/h9_rans_mixlen.cuh,
+h7_momentum_assembly.cuh, h10_dg_inlet.cuh):
 fixed-10 SIMPLE on the small pipe against the oracle's SIMPLE loop with the
 same cell blocks (oracle/rans_oracle.py, pinned in test_rans_oracle.py),
 fields to 1e-9, and the mixing-length statistics of the last assembly."""
@@ -13,6 +14,8 @@ CASES = {
     "supg": ["--supg", "1", "--supg-tau-scale", "0.05", "--supg-magic", "9.0"],
     "both": ["--mixing-length", "1", "--mixlen-scale", "0.7", "--supg", "1", "--supg-tau-scale", "0.1",
              "--supg-magic", "4.0"],
+    "dg-inlet": ["--dg-inlet", "1"],
+    "dg-inlet+mixing-length": ["--dg-inlet", "1", "--mixing-length", "1", "--mixlen-scale", "1.0"],
 }
 
 
@@ -45,8 +48,21 @@ def test_rans_terms_fixed10_match_oracle(mesh_dir, which):
     def keep(it, U, p):
         seen[it] = U.copy()
 
+    blocks = R.extra_blocks_fn(case, pts, nu, mixing, supg) if (mixing or supg) else None
+    rhs, U0 = None, None
+    if "--dg-inlet" in extra:
+        nf = int(h.get("dg_inlet_count", 1, np.int64)[0])
+        faces = h.get("dg_inlet_faces", 2 * nf, np.int32).reshape(nf, 2)
+        uh = h.get("pipe_geometry", 12)[6:9]
+        dg_blocks, rhs = R.dg_inlet_fns(case, pts, faces, nu, uh, mixing)
+        U0 = h.get("plug", 3 * case.nv).reshape(case.nv, 3).T.copy()
+        if blocks is None:
+            blocks = dg_blocks
+        else:
+            vol = blocks
+            blocks = lambda U: vol(U) + dg_blocks(U)  # noqa: E731
     ref = O.simple(case, p_solver="pcg", p_max=200, smoother=O.Smoother(kind="jacobi"), max_outer=10,
-                   extra_blocks=R.extra_blocks_fn(case, pts, nu, mixing, supg), callback=keep)
+                   extra_blocks=blocks, extra_rhs=rhs, U0=U0, callback=keep)
     worst = 0.0
     for d in range(3):
         u = h.get(f"u{d}", case.nv)
@@ -56,7 +72,7 @@ def test_rans_terms_fixed10_match_oracle(mesh_dir, which):
     print(f"[gpu]   max relative L2 difference vs oracle {worst:.3e}", flush=True)
     assert worst < 1e-9
     # the terms change the solution (against the plain run)
-    base = O.simple(case, p_solver="pcg", p_max=200, smoother=O.Smoother(kind="jacobi"), max_outer=10)
+    base = O.simple(case, p_solver="pcg", p_max=200, smoother=O.Smoother(kind="jacobi"), max_outer=10, U0=U0)
     assert np.linalg.norm(ref["U"][2] - base["U"][2]) > 1e-6 * np.linalg.norm(base["U"][2])
     if mixing is not None:
         stats = h.get("rans_stats", 3)

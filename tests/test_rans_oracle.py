@@ -189,3 +189,67 @@ def test_rans_options_are_checked(mesh_dir):
                      (["--mixing-length", "1", "--mixlen-scale", "-1"], "mixlen-scale")):
         with pytest.raises(NodalsError, match=msg):
             H8(base + bad, device=False)
+
+
+def test_triangle_rule_is_degree_six():
+    T = R.TRI12
+    assert abs(T[:, 3].sum() - 1) < 1e-14
+    for e in [(a, b, c) for a in range(7) for b in range(7) for c in range(7) if a + b + c <= 6]:
+        got = (T[:, 3] * T[:, 0] ** e[0] * T[:, 1] ** e[1] * T[:, 2] ** e[2]).sum()
+        exact = 2 * math.factorial(e[0]) * math.factorial(e[1]) * math.factorial(e[2]) / math.factorial(sum(e) + 2)
+        assert abs(got - exact) <= 1e-12 * exact, e
+
+
+def test_dg_inlet_terms_vanish_at_the_prescribed_plug():
+    """u = uh (vertices, zero bubbles) satisfies the weak inlet condition:
+    the inflow and Nitsche terms cancel, G U = R."""
+    P = np.stack([_tet(s, 0.2) for s in range(6)])
+    faces = np.array([[c, c % 4] for c in range(6)])
+    uh = np.array([0.2, -0.1, 1.3])
+    Uc = np.zeros((6, 8, 3))
+    Uc[:, :4] = uh
+    for mixing in (None, (1.0, 0.5, 0.0, 0.0)):
+        G, Rr = R.dg_inlet(P, faces, Uc, 0.01, uh, mixing)
+        assert np.abs(np.einsum("fab,fbd->fad", G, Uc) - Rr).max() < 1e-14 * np.abs(Rr).max()
+
+
+def test_dg_inlet_entries_are_face_integrals():
+    """G and R against adaptive cubature over the face (the integrands have
+    degree <= 6, which the 12-point rule integrates exactly); a flipped
+    Nitsche sign fails."""
+    P = _tet(9, 0.3)
+    GL, _, _ = R.geometry(P[None])
+    lf = 2
+    area, n = R.face_frame(P[None], np.array([lf]))
+    V = P[[j for j in range(4) if j != lf]]
+    uh = np.array([0.3, 0.1, -0.9])
+    beta = max(0.0, -(n[0] @ uh))
+    nu = 0.02
+    rng = np.random.default_rng(2)
+    Uc = rng.standard_normal((1, 8, 3))
+    G, Rr = R.dg_inlet(P[None], np.array([[0, lf]]), Uc, nu, uh)
+
+    def face_int(f):
+        def g(y, x):
+            lam = np.zeros(4)
+            fv = [j for j in range(4) if j != lf]
+            lam[fv[0]], lam[fv[1]], lam[fv[2]] = 1 - x - y, x, y
+            return f(lam)
+        val, _ = integrate.dblquad(g, 0, 1, 0, lambda x: 1 - x, epsabs=1e-14, epsrel=1e-12)
+        return 2 * area[0] * val
+
+    def at(lam):
+        phi, grad, _ = R.basis(lam[None], GL)
+        return phi[0], grad[0] @ n[0]
+
+    for a, b in ((0, 5), (6, 1), (4, 4), (7, 6)):
+        want = face_int(lambda lam: beta * at(lam)[0][a] * at(lam)[0][b]
+                        + nu * (at(lam)[1][a] * at(lam)[0][b] - at(lam)[0][a] * at(lam)[1][b]))
+        assert abs(G[0, a, b] - want) <= 1e-11 * max(abs(want), 1e-8)
+        bad = face_int(lambda lam: beta * at(lam)[0][a] * at(lam)[0][b]
+                       - nu * (at(lam)[1][a] * at(lam)[0][b] - at(lam)[0][a] * at(lam)[1][b]))
+        if abs(want - bad) > 1e-8:
+            assert abs(G[0, a, b] - bad) > 1e-9
+    a = 5
+    want = face_int(lambda lam: beta * at(lam)[0][a] + nu * at(lam)[1][a])
+    assert np.allclose(Rr[0, a], want * uh, rtol=1e-11, atol=1e-14)

@@ -24,10 +24,10 @@ def _product(mesh, extra=()):
     return H8(["--mesh", mesh, "--max-outer", "10", "--amg-smoother", "cheb2", *extra], device=False)
 
 
-def _compare_all(mesh):
-    R = refbind.RefCase(mesh)
+def _compare_all(mesh, dg_inlet=False):
+    R = refbind.RefCase(mesh, dg_inlet=dg_inlet)
     R.build_sa()
-    P = _product(mesh)
+    P = _product(mesh, ["--dg-inlet", "1"] if dg_inlet else [])
     I = P.info
     nv, nc, nnz = I["nv"], I["cells"], I["mom_nnz"]
     assert R.nv == nv and R.nc == nc
@@ -72,10 +72,30 @@ def test_medium_pipe_bitwise(mesh_dir):
     _compare_all(mesh_dir["medium"])
 
 
+@pytest.mark.parametrize("which", ["small", "medium"])
+def test_dg_inlet_setup_bitwise(mesh_dir, which):
+    """build_g4_setup with the DG weak inlet (g4_host.hpp:289-313): free
+    inlet DOFs, the effective B without the inlet face trace, the prescribed
+    inflow flux in the fixed divergence and the setup rAU of the plug start;
+    the inlet really changes the numbering and the B table."""
+    R, P = _compare_all(mesh_dir[which], dg_inlet=True)
+    R0 = refbind.RefCase(mesh_dir[which])
+    assert R.nv > R0.nv
+    plug = P.get("plug", 3 * R.nv).reshape(R.nv, 3)
+    pipe = P.get("pipe_geometry", 12)
+    g2 = R.get("g2free", np.int32)
+    npts = P.get("mesh_sizes", 4, np.int64)[0]
+    free_vert = g2[:npts][g2[:npts] >= 0]
+    assert np.array_equal(plug[free_vert], np.tile(pipe[6:9], (free_vert.size, 1)))
+    assert np.all(np.delete(plug, free_vert, axis=0) == 0)
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_MESH_40K), reason="reference 40k pipe mesh not present")
-def test_reference_40k_pipe_bitwise():
-    """The reference's own shipped mesh (40,620 unstructured tets)."""
-    _compare_all(REF_MESH_40K)
+@pytest.mark.parametrize("dg_inlet", [False, True])
+def test_reference_40k_pipe_bitwise(dg_inlet):
+    """The reference's own shipped mesh (40,620 unstructured tets), with and
+    without the DG inlet."""
+    _compare_all(REF_MESH_40K, dg_inlet)
 
 
 def test_unsmoothed_hierarchy_when_no_spectrum(mesh_dir):
