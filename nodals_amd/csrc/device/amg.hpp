@@ -30,7 +30,22 @@ struct SmootherOptions {
   double mcgsOmega = 1.0;
   int mcgsFineSweeps = 1, mcgsCoarseSweeps = 1;
   std::string mcgsOrder = "symmetric";  // forward | backward | symmetric
+  // Coarse levels with at most this many nonzeros (and vectors that fit
+  // kFuseLdsBytes of LDS) run each MCGS smoothing call as one workgroup
+  // (k_mcgs_level_wg) instead of one launch per colour; 0 turns it off.
+  // One CU streams a level's matrix once per colour pass: on the 207k helix's
+  // CF hierarchy the 108k-nonzero level gains and the 873k-nonzero level
+  // loses against launches spread over the chip.
+  long long mcgsFuseNnz = 250000;
 };
+
+// LDS of the fused level smoother: iterate, right-hand side, diagonal and
+// row pointers.
+constexpr std::size_t kFuseLdsBytes = 96 * 1024;
+template <class TV>
+constexpr std::size_t fuseLdsBytes(int n) {
+  return (std::size_t)n * (3 * sizeof(TV) + sizeof(int)) + sizeof(int);
+}
 
 // View of the finest operator, owned elsewhere (refreshed each SIMPLE step).
 template <class TF, class TV>
@@ -75,6 +90,8 @@ class DevAmg {
     DBuf<TV> val, diag, l1, b, x, r, tmp, corr;
     double lambda = 1.0;
     DevColoring<TV> color;
+    bool fused = false;          // MCGS smoothing in one workgroup
+    DBuf<int> colorOff, colorLanes;
   };
   struct Xfer {
     int nFine = 0, nCoarse = 0, lanesP = 8, lanesT = 8;
@@ -113,6 +130,14 @@ class DevAmg {
         inv[l].assign((std::size_t)A.n, 0);
         for (int i = 0; i < A.n; ++i) inv[l][(std::size_t)perm[l][(std::size_t)i]] = i;
         Lv.color.uploadContiguous(C);
+        if (o.mcgsFuseNnz > 0 && (long long)A.col.size() <= o.mcgsFuseNnz && fuseLdsBytes<TV>(A.n) <= kFuseLdsBytes) {
+          std::vector<int> lanes((std::size_t)C.ncolors);
+          for (int c = 0; c < C.ncolors; ++c)
+            lanes[(std::size_t)c] = lanesFor(Lv.meanRow, C.offset[(std::size_t)c + 1] - C.offset[(std::size_t)c]);
+          Lv.colorOff.upload(C.offset);
+          Lv.colorLanes.upload(lanes);
+          Lv.fused = true;
+        }
       }
       const std::vector<int>& pm = perm[l];
       const std::vector<int>& iv = inv[l];
@@ -365,6 +390,21 @@ class DevAmg {
   }
 
   void smoothLevel(hipStream_t s, Level& A, const TV* b, TV* x) {
+    if (A.fused) {
+      const int order = opt_.mcgsOrder == "forward" ? 0 : opt_.mcgsOrder == "backward" ? 1 : 2;
+      const std::size_t lds = fuseLdsBytes<TV>(A.n);
+      static const bool attr = [] {
+        NODALS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mcgs_level_wg<TV, TV>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFuseLdsBytes));
+        return true;
+      }();
+      (void)attr;
+      hipLaunchKernelGGL((k_mcgs_level_wg<TV, TV>), 1, 1024, lds, s, A.n, A.row.data(), A.col.data(), A.val.data(),
+                         A.diag.data(), b, x, A.color.ncolors, A.colorOff.data(), A.colorLanes.data(),
+                         opt_.mcgsCoarseSweeps, order, opt_.mcgsOmega);
+      NODALS_LAUNCHED();
+      return;
+    }
     smooth<TV>(s, A.n, A.lanes, A.meanRow, A.row.data(), A.col.data(), A.val.data(), A.diag.data(), A.l1.data(), &A.color,
                A.lambda, opt_.jacobiCoarseSweeps, opt_.mcgsCoarseSweeps, b, x, A.tmp.data(), A.tmp.data());
   }

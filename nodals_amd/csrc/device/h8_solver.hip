@@ -184,6 +184,7 @@ H8Options parseH8Args(int argc, const char* const* argv) {
     else if (a == "--amg-mcgs-fine-sweeps") o.amgMcgsFineSweeps = inum(next());
     else if (a == "--amg-mcgs-coarse-sweeps") o.amgMcgsCoarseSweeps = inum(next());
     else if (a == "--amg-mcgs-order") o.amgMcgsOrder = next();
+    else if (a == "--amg-mcgs-fuse-nnz") o.amgMcgsFuseNnz = (long long)num(next());
     else if (a == "--momentum-work") o.momentumWork = next();
     else if (a == "--mom-sweeps") o.momSweeps = inum(next());
     else if (a == "--run-mode") o.runMode = next();
@@ -666,6 +667,7 @@ void H8Solver::setupAmg() {
   so.mcgsFineSweeps = opt_.amgMcgsFineSweeps;
   so.mcgsCoarseSweeps = opt_.amgMcgsCoarseSweeps;
   so.mcgsOrder = opt_.amgMcgsOrder;
+  so.mcgsFuseNnz = opt_.amgMcgsFuseNnz;
   Coloring fineColor;
   if (so.kind == "mcgs") fineColor = greedyColoring32(nc_, F_.row.data(), F_.col.data());
   const Coloring* fc = so.kind == "mcgs" ? &fineColor : nullptr;

@@ -45,6 +45,7 @@ struct H8Options {
   std::string wallTractionMode = "spalding", wallDiagCsv;
   double wallReferenceF = 0.0284003, mixlenScale = 1.0, supgTauScale = 0.05, supgMagic = 9.0;
   int supgQuadPoints = 64;
+  long long amgMcgsFuseNnz = 250000;  // extension: coarse MCGS levels smoothed in one workgroup (0: off)
   std::string supgForm = "implicit";
   int maxOuter = 2500, momMax = 20000, pMax = 20;
   int fineCsrRefreshEvery = 1;

@@ -369,6 +369,71 @@ __global__ void __launch_bounds__(256) k_mcgs_color(int begin, int count, const 
   }
 }
 
+// A whole MCGS smoothing call of one small level in one workgroup, from x =
+// 0: `sweeps` sweeps of forward (order 0), backward (1) or forward then
+// backward (2) colour passes over the level numbered in colour order (colour
+// c is rows [coff[c], coff[c+1])), with a barrier between colours instead of
+// a launch per colour.  The iterate, the row pointers, the right-hand side
+// and the diagonal are staged in LDS, so a row's chain is one global load
+// (its columns and values) between LDS reads.  Colour c's rows use
+// clanes[c] lanes each, the lanes of k_mcgs_color's launch for that colour,
+// and the same rowDot/groupSum, so the result is bit-identical to the
+// launch-per-colour passes.  Dynamic LDS: n (3 sizeof(TV) + 4) + 4 bytes.
+template <int L, class TM, class TV>
+__device__ __forceinline__ void mcgsRowsLds(int begin, int count, const int* rs, const int* __restrict__ col,
+                                            const TM* __restrict__ val, const TV* ds, const TV* bs, TV* xs,
+                                            double omega) {
+  const int lane = threadIdx.x % L;
+  for (int q = threadIdx.x / L; q < count; q += blockDim.x / L) {
+    const int i = begin + q;
+    const TV bi = bs[i], di = ds[i], old = xs[i];
+    TV s = rowDot<L, TV>(
+        rs[i] + lane, rs[i + 1], col, i, [&](int k) { return ld<TV>(val, k); }, [&](int j) { return xs[j]; });
+    s = groupSum(s, L);
+    if (lane == 0) {
+      const TV gs = (bi - s) / di;
+      xs[i] = old + (TV)omega * (gs - old);
+    }
+  }
+}
+
+template <class TM, class TV>
+__global__ void __launch_bounds__(1024) k_mcgs_level_wg(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                                        const TM* __restrict__ val, const TV* __restrict__ diag,
+                                                        const TV* __restrict__ b, TV* __restrict__ x, int ncolors,
+                                                        const int* __restrict__ coff, const int* __restrict__ clanes,
+                                                        int sweeps, int order, double omega) {
+  extern __shared__ unsigned char ldsRaw[];
+  TV* xs = reinterpret_cast<TV*>(ldsRaw);
+  TV* bs = xs + n;
+  TV* ds = bs + n;
+  int* rs = reinterpret_cast<int*>(ds + n);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    xs[i] = TV(0);
+    bs[i] = b[i];
+    ds[i] = diag[i];
+  }
+  for (int i = threadIdx.x; i <= n; i += blockDim.x) rs[i] = row[i];
+  __syncthreads();
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int pass = 0; pass < 2; ++pass) {
+      if ((pass == 0 && order == 1) || (pass == 1 && order == 0)) continue;
+      for (int q = 0; q < ncolors; ++q) {
+        const int c = pass == 1 ? ncolors - 1 - q : q;
+        const int begin = coff[c], count = coff[c + 1] - begin;
+        switch (clanes[c]) {
+          case 4: mcgsRowsLds<4, TM, TV>(begin, count, rs, col, val, ds, bs, xs, omega); break;
+          case 8: mcgsRowsLds<8, TM, TV>(begin, count, rs, col, val, ds, bs, xs, omega); break;
+          case 16: mcgsRowsLds<16, TM, TV>(begin, count, rs, col, val, ds, bs, xs, omega); break;
+          case 32: mcgsRowsLds<32, TM, TV>(begin, count, rs, col, val, ds, bs, xs, omega); break;
+          default: mcgsRowsLds<64, TM, TV>(begin, count, rs, col, val, ds, bs, xs, omega); break;
+        }
+        __syncthreads();
+      }
+    }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xs[i];
+}
+
 // x = T b for a dense row-major n x n matrix (terminal AMG level).
 template <class TV>
 __global__ void __launch_bounds__(256) k_dense_mv(int n, const TV* __restrict__ T, const TV* __restrict__ b,
