@@ -342,7 +342,10 @@ CONFIG5_WALL = ["--convection-form", "advective"]
 # the spiral case's wall sampling: one Spalding point (the third), ray traced
 # with local shortening ([wall_sampling] mode = ray_trace, samples = 1)
 CONFIG5_SAMPLING = ["--sst-wall-sampling", "ray_trace", "--sst-wall-active-points", "1",
-                    "--sst-wall-ray-local-adapt", "1"]
+                    "--sst-wall-ray-local-adapt", "1",
+                    # [omega_wall] stage41_mode = trace_log: the log-layer omega
+                    # target imposed on the wall faces by symmetric Nitsche
+                    "--sst-omega-wall-target", "trace_log"]
 
 
 def yplus30_sample_scale(re: float) -> float:

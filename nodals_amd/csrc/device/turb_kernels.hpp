@@ -4,7 +4,8 @@
 //
 // Per equation: element blocks (volume terms + Picard convection, one
 // wavefront per cell: k_sst_volume), inlet flux blocks (k_turb_inlet), the
-// omega wall penalty (k_turb_wall_coef + k_turb_wall_block), then one
+// omega wall condition (k_turb_wall_coef, then k_turb_wall_block or
+// k_turb_wall_trace), then one
 // gather per vertex row that sums its entries over the blocks, forms the
 // row-L1 relaxation and writes the sweep operator, diagonal and right-hand
 // side (k_turb_gather); the coloured sweeps are k_mcgs_color; k_turb_floor
@@ -13,7 +14,9 @@
 //
 // Reference: turbulence/sst_gpu_v2/source/puretet_profile5_eq5_hxt5b/
 // modules/transport/tet_inlet_flux.inc:1-23 (sst_inlet_tet),
-// puretet_of_auto_point3.cuh:8-114 (sst_tet_of_auto_spalding5_point3),
+// modules/wall/of_auto_point3_kernel.inc:19-76 (sst_tet_of_auto_spalding5_point3
+// with spalding_profile5_tet_fit.inc:40-95, of_auto_point3_target.inc,
+// of_auto_point3_rank1_penalty.inc and of_auto_trace_log_nitsche.inc),
 // hxt5c_sst_g1.cuh:438 (combine_scalar), modules/transport/
 // scalar_relaxation.inc:8-33 (scalar_row_l1_relax_kernel) with row_l1_plan
 // (modules/transport/portable/transport_core.hpp:26-31), :176
@@ -148,40 +151,55 @@ static __global__ void __launch_bounds__(256) k_turb_inlet(
 
 struct WallConsts {
   double gamma, kappa, E, beta1, reBlend, prodLimit;
+  double ell;          // y5 - y1 of the nominal sample heights
+  int fitPoints, mode;  // active Spalding points; 0 of_auto, 1 sample_log, 2 trace_log
 };
 
-// Omega wall penalty coefficient and target of one wall quadrature point
-// (one thread each): five-point Spalding fit of u_tau from the axial
-// velocity at the five wall-normal samples, the SST state at the third
-// sample, the automatic omega target with the shared Spalding u_tau, and
-// coeff = gamma nu_e / ell * (w_q area ell) / ell with ell = y5 - y1.
-// Invalid points get coeff 0 and are counted in *bad.
+// Omega wall coefficient and target of one wall quadrature point (one
+// thread each): Spalding fit of u_tau over the active samples from the
+// magnitude of the velocity's component tangential to the wall face (the
+// reference's vector form; the speed is invariant under a rigid rotation of
+// velocity and normal), the SST state at the third sample, the automatic
+// omega target with the shared Spalding u_tau (its blended value for
+// of_auto, its log-layer value for sample_log and trace_log), the rank-one
+// coefficient coeff = gamma nu_e / ell * (w_q area ell) / ell and nu_e =
+// nu + sigma_w nu_t at the third sample (mu, the trace closure's
+// viscosity).  Invalid points get coeff 0 and are counted in *bad.
 static __global__ void __launch_bounds__(256) k_turb_wall_coef(
     int nq, const double* __restrict__ wgeom, const int* __restrict__ scell, const double* __restrict__ sl,
     const double* __restrict__ sphi, const double* __restrict__ sy, const int* __restrict__ cref,
     const int* __restrict__ sref, const double* __restrict__ cinvJ, const double* __restrict__ cxy,
     const double* __restrict__ u, const double* __restrict__ ufix, const double* __restrict__ kf,
     const double* __restrict__ wf, TurbConsts K, WallConsts W, Tri12 Q, double* __restrict__ coef,
-    double* __restrict__ target, int* __restrict__ bad, const double* __restrict__ vdist) {
+    double* __restrict__ target, int* __restrict__ bad, const double* __restrict__ vdist,
+    double* __restrict__ mu) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nq) return;
   const double* G = wgeom + 4ll * (w / 12);
-  double U[5], Us[5], Y[5];
+  const unsigned mask = sst::activeMask(W.fitPoints);
+  double U[5] = {0, 0, 0, 0, 0}, Y[5];
   for (int m = 0; m < 5; ++m) Y[m] = sy[(long long)w * 5 + m];
-  bool ok = true;
+  bool ok = fabs(G[0] * G[0] + G[1] * G[1] + G[2] * G[2] - 1.0) < 1.0e-8;
   for (int m = 0; m < 5; ++m) {
+    if (!(mask & (1u << m))) continue;
     const long long s = (long long)w * 5 + m;
     const int c = scell[s];
-    double um = 0.0;
+    double uv[3] = {0.0, 0.0, 0.0};
     for (int a = 0; a < 8; ++a) {
       const int r = cref[(long long)c * 8 + a];
-      um += (r >= 0 ? u[3ll * r + 2] : ufix[3ll * (-r - 1) + 2]) * sphi[s * 8 + a];
+      const double* Ua = r >= 0 ? u + 3ll * r : ufix + 3ll * (-r - 1);
+      const double ph = sphi[s * 8 + a];
+      uv[0] += Ua[0] * ph;
+      uv[1] += Ua[1] * ph;
+      uv[2] += Ua[2] * ph;
     }
-    ok = ok && isfinite(um);
-    Us[m] = um;
-    U[m] = fabs(um);
+    const double un = uv[0] * G[0] + uv[1] * G[1] + uv[2] * G[2];
+    const double t0 = uv[0] - un * G[0], t1 = uv[1] - un * G[1], t2 = uv[2] - un * G[2];
+    const double mag = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    ok = ok && isfinite(uv[0]) && isfinite(uv[1]) && isfinite(uv[2]) && isfinite(mag);
+    U[m] = mag;
   }
-  const sst::Profile P = sst::fitProfile5(U, Us, Y, K.nu);
+  const sst::Profile P = sst::fitActive(U, U, Y, W.fitPoints, K.nu);
   const long long s3 = (long long)w * 5 + 2;
   const int c = scell[s3];
   const double l[4] = {sl[s3 * 4], sl[s3 * 4 + 1], sl[s3 * 4 + 2], sl[s3 * 4 + 3]};
@@ -193,15 +211,66 @@ static __global__ void __launch_bounds__(256) k_turb_wall_coef(
   const double Ut = sqrt(fmax(0.0, us[0] * us[0] + us[1] * us[1] + us[2] * us[2] - normal * normal));
   const sst::OmegaWall oa = sst::omegaWall(Y[2], K.nu, Ut, P.uTau, kv, wvv, ss.pk, K.kFloor, K.omFloor, W.kappa,
                                            W.E, W.beta1, W.reBlend, W.prodLimit, true);
-  const double ell = Y[4] - Y[0];
+  const double ell = W.ell;
   const double nuEff = K.nu + ss.sigmaW * ss.nut;
   const double vweight = Q.q[w % 12][3] * G[3] * ell;
   const double cf = W.gamma * nuEff / ell * vweight / ell;
-  ok = ok && P.valid && isfinite(nuEff) && nuEff > 0.0 && isfinite(cf) && cf > 0.0 && isfinite(oa.omegaTarget) &&
-       oa.omegaTarget > 0.0;
+  const double tgt = W.mode == 0 ? oa.omegaTarget : oa.omegaLog;
+  ok = ok && P.valid && Y[2] > 0.0 && ell > 0.0 && isfinite(nuEff) && nuEff > 0.0 && isfinite(cf) && cf > 0.0 &&
+       isfinite(tgt) && tgt > 0.0;
   coef[w] = ok ? cf : 0.0;
-  target[w] = ok ? oa.omegaTarget : 0.0;
+  target[w] = ok ? tgt : 0.0;
+  mu[w] = ok ? nuEff : 0.0;
   if (!ok) atomicAdd(bad, 1);
+}
+
+// trace_log: the omega target g imposed on the wall face by symmetric
+// Nitsche, one wavefront per wall quadrature point, on the face's own cell:
+//   A_ab = w [ -mu dn phi_b phi_a - mu dn phi_a phi_b + tau phi_a phi_b ]
+//   b_a  = w [ -mu dn phi_a + tau phi_a ] g
+// at the face point, w = w_q area, tau = gammaT mu / h, h = det / (2 area)
+// (three times the volume over the face area), n the outward normal.
+// Points whose rank-one checks failed (coef 0) were counted already;
+// a failed trace check is counted here.  Both give a zero block.
+static __global__ void __launch_bounds__(256) k_turb_wall_trace(
+    int nq, const int* __restrict__ wcell, const int* __restrict__ wlf, const double* __restrict__ wgeom,
+    const double* __restrict__ cinvJ, const double* __restrict__ cdet, const double* __restrict__ coef,
+    const double* __restrict__ target, const double* __restrict__ mu, double gammaT, Tri12 Q,
+    double* __restrict__ blk, double* __restrict__ brhs, int* __restrict__ bad) {
+  __shared__ double PH[4][8], DN[4][8];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + wv;
+  const bool live = w < nq;
+  const int f = live ? w / 12 : 0, q = live ? w % 12 : 0;
+  const int c = wcell[f];
+  const double* G = wgeom + 4ll * f;
+  if (live && lane == 0) {
+    double l[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 3; ++j) l[kTurbFace[wlf[f]][j]] = Q.q[q][j];
+    double phi[8], gr[8][3];
+    p1bf3Basis(l, phi, gr);
+    const double* I = cinvJ + (long long)c * 9;
+    for (int a = 0; a < 8; ++a) {
+      double dn = 0.0;
+      for (int d = 0; d < 3; ++d) dn += (gr[a][0] * I[d] + gr[a][1] * I[3 + d] + gr[a][2] * I[6 + d]) * G[d];
+      PH[wv][a] = phi[a];
+      DN[wv][a] = dn;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const double h = cdet[c] / (2.0 * G[3]);
+  const double m = mu[w], g = target[w];
+  const double tau = gammaT * m / h;
+  const double ww = Q.q[q][3] * G[3];
+  const bool okT = isfinite(h) && h > 0.0 && isfinite(m) && m > 0.0 && isfinite(tau) && tau > 0.0 && isfinite(ww) &&
+                   ww > 0.0 && isfinite(g) && g > 0.0;
+  const bool ok = coef[w] > 0.0 && okT;
+  if (coef[w] > 0.0 && !okT && lane == 0) atomicAdd(bad, 1);
+  const int a = lane >> 3, b = lane & 7;
+  blk[(long long)w * 64 + lane] =
+      ok ? ww * (-m * DN[wv][b] * PH[wv][a] - m * DN[wv][a] * PH[wv][b] + tau * PH[wv][a] * PH[wv][b]) : 0.0;
+  if (lane < 8) brhs[(long long)w * 8 + lane] = ok ? ww * (-m * DN[wv][lane] + tau * PH[wv][lane]) * g : 0.0;
 }
 
 // Rank-one penalty block of each wall quadrature point on the cell of its

@@ -41,6 +41,12 @@ TurbParams parseTurbArgs(int argc, const char* const* argv) {
       require(m == "of_auto" || m == "none", "--sst-omega-wall-mode must be of_auto or none");
       P.wall = m == "of_auto";
     } else if (f == "--sst-wall-penalty-gamma") P.wallGamma = num(f, v);
+    else if (f == "--sst-omega-wall-target") {
+      const std::string m = v;
+      require(m == "of_auto" || m == "sample_log" || m == "trace_log",
+              "--sst-omega-wall-target must be of_auto, sample_log or trace_log");
+      P.omegaMode = m == "of_auto" ? 0 : m == "sample_log" ? 1 : 2;
+    } else if (f == "--sst-omega-trace-gamma") P.traceGamma = num(f, v);
     else if (f.size() == 20 && f.rfind("--sst-profile-fit-y", 0) == 0 && f[19] >= '1' && f[19] <= '5')
       P.heights[(std::size_t)(f[19] - '1')] = num(f, v);
     else if (f == "--sst-wall-sample-scale") P.sampleScale = num(f, v);
@@ -70,6 +76,8 @@ TurbParams parseTurbArgs(int argc, const char* const* argv) {
   require(P.rayTol > 0 && P.rayTol <= 1e-5, "--sst-wall-ray-bary-tolerance must be in (0, 1e-5]");
   require(P.activePoints >= 1 && P.activePoints <= 5, "--sst-wall-active-points must be in 1..5");
   require(P.wallGamma > 0, "--sst-wall-penalty-gamma must be positive");
+  require(P.omegaMode != 2 || (std::isfinite(P.traceGamma) && P.traceGamma >= 10.0 && P.traceGamma <= 1e5),
+          "trace_log trace_gamma must be in [10,100000]");
   require(P.intensity > 0 && P.lengthRatio > 0, "--sst-intensity and --sst-length-ratio must be positive");
   const bool anyY = P.heights[0] > 0 || P.heights[1] > 0 || P.heights[2] > 0 || P.heights[3] > 0 || P.heights[4] > 0;
   if (anyY)
@@ -127,13 +135,21 @@ void H8Solver::turbInit(const TurbParams& P) {
   st->flf.upload(fl);
   st->fgeom.upload(fg);
   std::vector<double> wg, sl, sp, sy;
-  std::vector<int> sc;
-  for (const TurbFace& f : T.wall) wg.insert(wg.end(), {f.n[0], f.n[1], f.n[2], f.area});
+  std::vector<int> sc, wc, wl;
+  for (const TurbFace& f : T.wall) {
+    wg.insert(wg.end(), {f.n[0], f.n[1], f.n[2], f.area});
+    wc.push_back(f.cell);
+    wl.push_back(f.lf);
+  }
   for (const TurbSample& s : T.samples) {
     sc.push_back(s.cell);
     sl.insert(sl.end(), s.l, s.l + 4);
     sp.insert(sp.end(), s.phi, s.phi + 8);
     sy.push_back(s.y);
+  }
+  if (wc.empty()) {
+    wc.push_back(0);
+    wl.push_back(0);
   }
   if (sc.empty()) {
     wg.assign(4, 0.0);
@@ -147,6 +163,10 @@ void H8Solver::turbInit(const TurbParams& P) {
   st->sl.upload(sl);
   st->sphi.upload(sp);
   st->sy.upload(sy);
+  st->wcell.upload(wc);
+  st->wlf.upload(wl);
+  st->omegaFitPoints = P.activePoints;
+  st->mu.alloc((std::size_t)std::max(T.nWallQ, 1));
   st->coef.alloc((std::size_t)std::max(T.nWallQ, 1));
   st->target.alloc((std::size_t)std::max(T.nWallQ, 1));
   st->conv.alloc((std::size_t)nc_ * 64);
@@ -192,14 +212,22 @@ int H8Solver::turbEquation(int eq, double* rel0, double* rel1) {
   }
   int nb = nc_ + T.nInlet;
   if (eq == 1 && T.nWallQ > 0) {
-    const WallConsts W{P.wallGamma, P.ofKappa, P.ofE, P.ofBeta1, P.ofReBlend, P.ofProdLimit};
+    const WallConsts W{P.wallGamma, P.ofKappa,     P.ofE, P.ofBeta1, P.ofReBlend, P.ofProdLimit,
+                       T.heights[4] - T.heights[0], st.omegaFitPoints, P.omegaMode};
     hipLaunchKernelGGL(k_turb_wall_coef, blocksFor(T.nWallQ), 256, 0, s, T.nWallQ, st.wgeom.data(), st.scell.data(),
                        st.sl.data(), st.sphi.data(), st.sy.data(), cref_.data(), st.sref.data(), cinvJ_.data(), sstXY_.data(),
                        u_.data(), fixedVal_.data(), st.k.data(), st.w.data(), K, W, tri12(), st.coef.data(),
-                       st.target.data(), st.counters.data() + 2, st.vdistPtr());
+                       st.target.data(), st.counters.data() + 2, st.vdistPtr(), st.mu.data());
     NODALS_LAUNCHED();
-    hipLaunchKernelGGL(k_turb_wall_block, (T.nWallQ + 3) / 4, 256, 0, s, T.nWallQ, st.sphi.data(), st.coef.data(),
-                       st.target.data(), st.blk.data() + (std::size_t)nb * 64, st.brhs.data() + (std::size_t)nb * 8);
+    if (P.omegaMode == 2)
+      hipLaunchKernelGGL(k_turb_wall_trace, (T.nWallQ + 3) / 4, 256, 0, s, T.nWallQ, st.wcell.data(), st.wlf.data(),
+                         st.wgeom.data(), cinvJ_.data(), cdet_.data(), st.coef.data(), st.target.data(),
+                         st.mu.data(), P.traceGamma, tri12(), st.blk.data() + (std::size_t)nb * 64,
+                         st.brhs.data() + (std::size_t)nb * 8, st.counters.data() + 2);
+    else
+      hipLaunchKernelGGL(k_turb_wall_block, (T.nWallQ + 3) / 4, 256, 0, s, T.nWallQ, st.sphi.data(),
+                         st.coef.data(), st.target.data(), st.blk.data() + (std::size_t)nb * 64,
+                         st.brhs.data() + (std::size_t)nb * 8);
     NODALS_LAUNCHED();
     nb += T.nWallQ;
   }
@@ -331,7 +359,8 @@ std::vector<double> H8Solver::turbField(const std::string& n) {
   require((bool)turb_, "k-omega step not initialised (nodals_h8_turb_init)", kErrInvalid);
   TurbState& st = *turb_;
   const TurbTopo& T = st.T;
-  const bool devField = n == "k" || n == "omega" || n == "turb_val" || n == "turb_diag" || n == "turb_rhs";
+  const bool devField = n == "k" || n == "omega" || n == "turb_val" || n == "turb_diag" || n == "turb_rhs" ||
+                        n == "turb_wall_target";
   if (devField) requireDevice();
   if (hasDevice()) sync();
   auto toD = [](const std::vector<int>& v) { return std::vector<double>(v.begin(), v.end()); };
@@ -340,6 +369,7 @@ std::vector<double> H8Solver::turbField(const std::string& n) {
   if (n == "turb_val") return st.val.host();
   if (n == "turb_diag") return st.diag.host();
   if (n == "turb_rhs") return st.rhs.host();
+  if (n == "turb_wall_target") return st.target.host();  // omega target per wall point, last omega assembly
   if (n == "turb_row") return toD(T.row);
   if (n == "turb_col") return toD(T.col);
   if (n == "turb_color") return toD(T.coloring.color);

@@ -40,8 +40,12 @@ struct TurbState {
   TurbTopo T;
   TurbParams P;
   DBuf<int> sref, row, col, diagPos, entOff, termOff, term, rhsOff, rhsTerm, crows;
-  DBuf<int> fcell, flf, scell, counters;
-  DBuf<double> fgeom, wgeom, sl, sphi, sy, coef, target, conv, blk, brhs, val, diag, rhs, k, w;
+  DBuf<int> fcell, flf, scell, counters, wcell, wlf;
+  DBuf<double> fgeom, wgeom, sl, sphi, sy, coef, target, mu, conv, blk, brhs, val, diag, rhs, k, w;
+  // Spalding points of the omega wall fit: the k-omega setup's active
+  // points, replaced by the wall path's own count when it is initialised
+  // (the reference holds one count for both, SST24B_ACTIVE_POINTS)
+  int omegaFitPoints = 5;
   DBuf<double> vdist;  // vertex wall distances (duct mode), empty for the straight pipe
   DBuf<double> resid;  // residual of a scalar_solve norm
   const double* vdistPtr() const { return vdist.size() ? vdist.data() : nullptr; }

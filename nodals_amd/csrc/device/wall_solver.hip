@@ -76,6 +76,7 @@ void H8Solver::wallInit(const WallParams& P) {
             "active point count (--sst-wall-active-points)",
             kErrInvalid);
   }
+  turb_->omegaFitPoints = P.activePoints;
   auto st = std::make_unique<WallState>();
   st->P = P;
   st->W = buildWallTopo(M_, S_, turb_->T);

@@ -302,7 +302,12 @@ TurbTopo buildTurbTopo(const TetMesh& M, const H8Setup& S, const TurbParams& P) 
   T.blockCell.resize((std::size_t)nb);
   for (int c = 0; c < nc; ++c) T.blockCell[(std::size_t)c] = c;
   for (int i = 0; i < T.nInlet; ++i) T.blockCell[(std::size_t)nc + i] = T.inlet[(std::size_t)i].cell;
-  for (int w = 0; w < T.nWallQ; ++w) T.blockCell[(std::size_t)nc + T.nInlet + w] = T.samples[(std::size_t)w * 5 + 2].cell;
+  require(P.omegaMode >= 0 && P.omegaMode <= 2, "omega wall mode must be of_auto, sample_log or trace_log");
+  require(P.omegaMode != 2 || (P.traceGamma >= 10.0 && P.traceGamma <= 1e5),
+          "trace_log trace_gamma must be in [10,100000]");
+  for (int w = 0; w < T.nWallQ; ++w)
+    T.blockCell[(std::size_t)nc + T.nInlet + w] =
+        P.omegaMode == 2 ? T.wall[(std::size_t)w / 12].cell : T.samples[(std::size_t)w * 5 + 2].cell;
 
   // Vertex rows: sorted vertex columns, then sorted bubble columns.
   std::vector<std::vector<int>> vc((std::size_t)np), bc((std::size_t)np);

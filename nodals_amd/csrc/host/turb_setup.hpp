@@ -33,8 +33,17 @@ struct TurbParams {
   double gsOmega = 0.7;                         // sweep relaxation
   double kFloor = 1e-8, omegaFloor = 1e-3;
   bool deviatoric = false;                      // dev(2S):grad(U) production
-  bool wall = true;                             // omega wall penalty at the third sample
+  bool wall = true;                             // omega wall condition at the third sample
   double wallGamma = 1e6;
+  // Omega wall closure (STAGE41_OMEGA_MODE, sst_application.inc:88-139;
+  // of_auto_point3_rank1_penalty.inc, of_auto_trace_log_nitsche.inc):
+  // 0 of_auto, the blended viscous/log target, and 1 sample_log, the pure
+  // log target, each as a rank-one penalty at the third sample; 2
+  // trace_log, the pure log target imposed on the wall face itself by
+  // symmetric Nitsche with tau = traceGamma mu / h, h = 3 volume / area.
+  // The reference binary's default is sample_log.
+  int omegaMode = 1;
+  double traceGamma = 50.0;
   // Wall-normal sample heights; zeros select the reference rule
   // d_i * D / 0.004 * sampleScale (turbulence/sst_gpu_v2/nodals.py:289-292).
   std::array<double, 5> heights{{0, 0, 0, 0, 0}};
@@ -93,7 +102,8 @@ struct TurbTopo {
   Coloring coloring;
   // Gather maps.  Blocks: cells [0, nc), inlet faces [nc, nc + nInlet) (on
   // the face's cell), wall quadrature points [nc + nInlet, nc + nInlet +
-  // nWallQ) (on the cell of their third sample).  Row i owns entries
+  // nWallQ) (on the cell of their third sample; on the wall face's own
+  // cell for the trace_log closure).  Row i owns entries
   // [entOff[i], entOff[i+1]): first its CSR columns in order, then its
   // bubble columns ascending; entry e sums block values term[termOff[e] ..
   // termOff[e+1]) (block*64 + 8a + b, ascending).  Its right-hand side sums

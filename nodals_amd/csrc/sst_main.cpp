@@ -389,9 +389,27 @@ int run(int argc, char** argv) {
                          "--sst-face-yplus-min-scale", envOr("NODALS_FACE_YPLUS_MIN_SCALE", "0.02"),
                          "--sst-face-yplus-max-scale", envOr("NODALS_FACE_YPLUS_MAX_SCALE", "0.35")});
   }
+  // omega wall closure (the reference binary's default is sample_log)
+  const std::string s41 = envOr("STAGE41_OMEGA_MODE", "sample_log");
+  require(s41 == "sample_log" || s41 == "of_auto" || s41 == "trace_log" || s41 == "couette_point3",
+          "STAGE41_OMEGA_MODE must be sample_log, of_auto or couette_point3");
+  require(s41 != "couette_point3", "STAGE41_OMEGA_MODE=couette_point3 is not supported by this build");
+  const std::string traceGamma = envOr("STAGE41_TRACE_GAMMA", "50");
+  tu.insert(tu.end(), {"--sst-omega-wall-target", s41, "--sst-omega-trace-gamma", traceGamma});
   std::vector<const char*> tv;
   for (const auto& s : tu) tv.push_back(s.c_str());
   check(nodals_h8_turb_init(c.h, (int)tv.size(), tv.empty() ? nullptr : tv.data()));
+  if (s41 == "trace_log")
+    std::printf("NODALS_OMEGA_TRACE_LOG enabled=1 target=PURE_LOG_AT_INTERIOR_POINT3 "
+                "placement=PHYSICAL_TET_WALL_TRACE operator=SYMMETRIC_NITSCHE mu=POINT3_NU_PLUS_SIGMAW_NUT "
+                "h=THREE_TET_VOLUME_OVER_WALL_AREA gamma=%.12e kWall=NATURAL_ZERO_GRADIENT production=0 "
+                "spaldingFit=SHARED_FULL_VECTOR status=EXPERIMENTAL\n",
+                std::stod(traceGamma));
+  std::printf("STAGE41_SAMPLE_LOG mode=%s omegaTarget=%s constraint=%s fullVectorSpalding=ON robin=OFF "
+              "traceNitsche=%s SSTvolume=UNCHANGED deviceConstants=SET status=PASS\n",
+              s41.c_str(), s41 == "of_auto" ? "OF_AUTO_BLEND" : "PURE_LOG",
+              s41 == "trace_log" ? "PHYSICAL_TRACE_SYMMETRIC_NITSCHE" : "ORIGINAL_INTERIOR_POINT_RANK1",
+              s41 == "trace_log" ? "ON" : "OFF");
   {
     double rs[7];
     check(nodals_h8_get(c.h, "turb_sample_stats", rs, 7));

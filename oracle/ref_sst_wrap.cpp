@@ -78,6 +78,22 @@ void ref_profile5(long long n, const double* U, const double* Us, const double* 
   }
 }
 
+// Spalding fit over the active points of a five-slot sample set
+// (active::fit_values, modules/wall/portable/spalding_active.hpp:59-112,
+// the fit of spalding_match_profile5_tet_vector): U (speeds) and y are
+// [n][5]; out[n][2] = uTau yPlus; valid[n].
+void ref_fit_values(long long n, const double* U, const double* y, int count, double nu, double* out, int* valid) {
+  for (long long i = 0; i < n; ++i) {
+    double ut = 0, yp = 0, beta = 0, uMid = 0, rms = 0, span = 0;
+    valid[i] = nodals::wall::active::fit_values(U + 5 * i, U + 5 * i, y + 5 * i, count, nu, ut, yp, beta, uMid, rms,
+                                                span)
+                   ? 1
+                   : 0;
+    out[2 * i] = ut;
+    out[2 * i + 1] = yp;
+  }
+}
+
 // OpenFOAM-style omega wall target.  in[n][15] = y nu Ut uTau k omega kFloor
 // omegaFloor kappa E beta1 reBlend prodLimit pk nut; shared[n];
 // out[n][8] = omegaTarget omegaVis omegaLog lamFrac rey yPlusK pkAuto

@@ -138,6 +138,7 @@ class RefSst:
         L.ref_profile5.argtypes = [C.c_longlong, vp, vp, vp, C.c_double, vp, vp]
         L.ref_of_auto.argtypes = [C.c_longlong, vp, vp, vp]
         L.ref_fit_tangential.argtypes = [C.c_longlong, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]
+        L.ref_fit_values.argtypes = [C.c_longlong, vp, vp, C.c_int, C.c_double, vp, vp]
         self.L = L
 
     @staticmethod
@@ -177,6 +178,16 @@ class RefSst:
         out = np.empty((n, 6))
         valid = np.empty(n, dtype=np.int32)
         self.L.ref_profile5(n, *(self._p(v) for v in a), nu, self._p(out), self._p(valid))
+        return out, valid
+
+    def fit_values(self, U, y, nu, count):
+        """Spalding fit over the active points (active::fit_values): U
+        speeds and y (n,5); out (n,2) = u_tau y+, valid."""
+        n = int(np.asarray(y).size // 5)
+        a = [np.ascontiguousarray(v, dtype=np.float64).reshape(n, 5) for v in (U, y)]
+        out = np.empty((n, 2))
+        valid = np.empty(n, dtype=np.int32)
+        self.L.ref_fit_values(n, *(self._p(v) for v in a), int(count), nu, self._p(out), self._p(valid))
         return out, valid
 
     def fit_tangential(self, samples, y, nrm, pic, nu, count, zero_limit=True):

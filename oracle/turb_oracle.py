@@ -13,7 +13,7 @@ zero (g2_state.inc:12-27):
 - matrix = base (nu diffusion, exact tensor) + Picard convection
   (hxt4b_foundation.inc:474, the H8 central tensor) + volume terms
   (tet_volume_kernel.inc, via sst_oracle) + inlet flux (tet_inlet_flux.inc)
-  + omega wall penalty (puretet_of_auto_point3.cuh), combined per entry
+  + omega wall condition (modules/wall/of_auto_point3_kernel.inc:19-76), combined per entry
   (hxt5c_sst_g1.cuh:438 combine_scalar) by sparse summation;
 - row-L1 relaxation over every stored entry of each row
   (scalar_relaxation.inc:8-33, transport_core.hpp:26-31);
@@ -22,8 +22,15 @@ zero (g2_state.inc:12-27):
   (vertex) rows, fixed rows set to zero (hxt5b_boundary.cuh:1820);
 - floor of the vertex values (scalar_relaxation.inc:176).
 
-The pointwise SST state, the five-point Spalding fit and the OF-auto omega
-target come from the REFERENCE's own compiled cores (refbind.RefSst).  Wall
+The pointwise SST state, the Spalding fit over the active points
+(active::fit_values, on the speed tangential to the wall face,
+spalding_profile5_tet_fit.inc:40-95) and the OF-auto omega target come from
+the REFERENCE's own compiled cores (refbind.RefSst).  The omega wall
+condition is one of the reference's three (STAGE41_OMEGA_MODE,
+sst_application.inc:88-139): of_auto (blended target) or sample_log (log
+target) as a rank-one penalty at the third sample
+(of_auto_point3_rank1_penalty.inc), or trace_log, the log target imposed on
+the wall face by symmetric Nitsche (of_auto_trace_log_nitsche.inc).  Wall
 samples are located by the barycentric adjacency walk of
 modules/wall/tet_wall_match_cache.inc:3-67.
 
@@ -31,7 +38,10 @@ Pins (tests/test_turb_oracle.py): the twelve-point triangle rule integrates
 every monomial of degree <= 6 exactly; the walk reconstructs each sample
 point; the inlet block reproduces its right-hand side on the constant inlet
 field (the flux terms cancel exactly there); the wall block vanishes on a
-field equal to the target at the sample; row-L1 relaxation keeps the fixed
+field equal to the target at the sample; the trace block is symmetric and
+its residual on a linear field equal to the target on the face, and on a
+constant field, are the closed-form face integrals of the basis and of its
+normal derivatives (with h = 3 volume / area from the vertices); row-L1 relaxation keeps the fixed
 point of the unrelaxed system; many sweeps converge to a direct solve; the
 colouring is proper.
 """
@@ -113,6 +123,9 @@ class Params:
     of_beta1: float = 0.075
     of_re_blend: float = 11.0
     of_prod_limit: float = 10.0
+    active_points: int = 5          # Spalding fit points (the wall path's count when it runs)
+    omega_target: str = "sample_log"  # of_auto | sample_log | trace_log
+    trace_gamma: float = 50.0
 
 
 def cell_entities(m: Mesh) -> np.ndarray:
@@ -257,7 +270,7 @@ class Step:
         if ib:
             blocks.append(np.array(ib))
             brhs.append(np.array(ir))
-        # omega wall penalty at the third sample
+        # omega wall condition
         if eq == 1 and self.wall:
             wb, wr, wc = self.wall_blocks(Uc)
             blocks.append(wb)
@@ -293,16 +306,22 @@ class Step:
         b = np.einsum("q,qa->a", wq * inflow * phi_in, phi) + np.einsum("q,qa->a", wq * nue * phi_in, dn)
         return A, b
 
-    def wall_data(self, Uc):
-        """Per wall quadrature point: u_tau, omega target and penalty coefficient."""
+    def wall_data(self, Uc, with_nue=False):
+        """Per wall quadrature point: penalty coefficient, omega target,
+        basis at the third sample, its cell and the invalid count (and nu_e
+        at the third sample)."""
         P = self.P
         nw = len(self.wall)
         cells = self.s_cell.reshape(-1)
         L = self.s_l.reshape(-1, 4)
         phi, _ = S.basis(L)
-        uz = np.einsum("na,na->n", Uc[cells][:, :, 2], phi).reshape(nw * 12, 5)
+        uv = np.einsum("nad,na->nd", Uc[cells], phi).reshape(nw * 12, 5, 3)
+        nrm = np.repeat(np.array([f[2] for f in self.wall]), 12, axis=0)
+        tang = uv - np.einsum("nmd,nd->nm", uv, nrm)[:, :, None] * nrm[:, None, :]
+        speed = np.linalg.norm(tang, axis=2)
+        self.last_speed = speed
         y = np.broadcast_to(self.heights, (nw * 12, 5))
-        fit, valid = self.ref.profile5(np.abs(uz), uz, y, self.nu)
+        fit, valid = self.ref.fit_values(speed, y, self.nu, P.active_points)
         c3 = self.s_cell[:, :, 2].reshape(-1)
         L3 = self.s_l[:, :, 2].reshape(-1, 4)
         out, phi3, _, us, kv, wv = self._state(c3, L3, Uc, self.k, self.w)
@@ -316,19 +335,51 @@ class Step:
                                np.full(q, P.of_E), np.full(q, P.of_beta1), np.full(q, P.of_re_blend),
                                np.full(q, P.of_prod_limit), out[:, 8], out[:, 0]])
         oa = self.ref.of_auto(inp, np.ones(q, dtype=np.int32))
+        tgt = oa[:, 0] if P.omega_target == "of_auto" else oa[:, 2]
         ell = self.heights[4] - self.heights[0]
         nue = self.nu + out[:, 4] * out[:, 0]
         vw = np.tile(TRI12[:, 3], nw) * area * ell
         coef = P.gamma * nue / ell * vw / ell
         ok = (valid != 0) & np.isfinite(nue) & (nue > 0) & np.isfinite(coef) & (coef > 0) & \
-            np.isfinite(oa[:, 0]) & (oa[:, 0] > 0)
-        return np.where(ok, coef, 0.0), np.where(ok, oa[:, 0], 0.0), phi3, c3, int((~ok).sum())
+            np.isfinite(tgt) & (tgt > 0)
+        res = np.where(ok, coef, 0.0), np.where(ok, tgt, 0.0), phi3, c3, int((~ok).sum())
+        return res + (np.where(ok, nue, 0.0),) if with_nue else res
+
+    def trace_block(self, face, q: int, mu: float, g: float):
+        """8x8 block and 8-vector of the trace_log condition at point q of a
+        wall face on the face's cell (of_auto_trace_log_nitsche.inc:6-45):
+        w [-mu dn phi_b phi_a - mu dn phi_a phi_b + tau phi_a phi_b] and
+        w [-mu dn phi_a + tau phi_a] g, w = w_q area, tau = gamma mu / h,
+        h = det / (2 area)."""
+        c, lf, n, area = face
+        L = np.zeros((1, 4))
+        L[0, TET_FACE[lf]] = TRI12[q, :3]
+        phi, gr = S.basis(L)
+        dn = (gr[0] @ self.case.cell_invJ[c]) @ n
+        phi = phi[0]
+        h = self.case.cell_det[c] / (2.0 * area)
+        tau = self.P.trace_gamma * mu / h
+        w = TRI12[q, 3] * area
+        A = w * (-mu * np.outer(phi, dn) - mu * np.outer(dn, phi) + tau * np.outer(phi, phi))
+        b = w * (-mu * dn + tau * phi) * g
+        ok = all(np.isfinite(v) and v > 0 for v in (h, mu, tau, w, g))
+        return A, b, ok
 
     def wall_blocks(self, Uc):
-        coef, target, phi3, c3, _ = self.wall_data(Uc)
-        wb = coef[:, None, None] * phi3[:, :, None] * phi3[:, None, :]
-        wr = (coef * target)[:, None] * phi3
-        return wb, wr, c3
+        if self.P.omega_target != "trace_log":
+            coef, target, phi3, c3, _ = self.wall_data(Uc)
+            wb = coef[:, None, None] * phi3[:, :, None] * phi3[:, None, :]
+            wr = (coef * target)[:, None] * phi3
+            return wb, wr, c3
+        coef, target, _, _, _, nue = self.wall_data(Uc, with_nue=True)
+        nq = coef.size
+        wb, wr = np.zeros((nq, 8, 8)), np.zeros((nq, 8))
+        for i in range(nq):
+            if coef[i] > 0:
+                A, b, ok = self.trace_block(self.wall[i // 12], i % 12, nue[i], target[i])
+                if ok:
+                    wb[i], wr[i] = A, b
+        return wb, wr, np.repeat(np.array([f[0] for f in self.wall]), 12)
 
     @staticmethod
     def relax(A: sp.csr_matrix, rhs: np.ndarray, x: np.ndarray, alpha: float):

@@ -63,7 +63,8 @@ def helix_case(tmp_path_factory):
     sets = ["mesh.diameter=1", "physics.reynolds=5000", "physics.kinematic_viscosity=0.0002",
             "inlet.main.velocity=(%r, %r, %r)" % tuple(float(x) for x in tan), "simple.max_iterations=20",
             "solver.p.rtol=1e-3", "output.checkpoint_every=10", "output.vtu_every=10", "simple.ur_u=0.5",
-            "simple.ur_p=0.3", "convection.form=advective", "stress.form=deviatoric", "solver.u.sweeps=20"]
+            "simple.ur_p=0.3", "convection.form=advective", "stress.form=deviatoric", "solver.u.sweeps=20",
+            "omega_wall.stage41_mode=trace_log"]
     return d, sets
 
 
@@ -91,6 +92,9 @@ def test_sst_case_runs_checkpoints_and_restarts(helix_case):
     # the launcher's default [wall_sampling] mode = ray_trace reaches the solver
     assert "SST_WALL_SAMPLING_RUNTIME method=ray_trace rayContainment=VERIFIED" in log
     assert re.search(r"SST_WALL_RAY_CACHE samplePoints=\d+ .*shortened=0 ", log)
+    # the spiral case's omega wall closure reaches the solver
+    assert "STAGE41_SAMPLE_LOG mode=trace_log omegaTarget=PURE_LOG " in log
+    assert "NODALS_OMEGA_TRACE_LOG enabled=1 " in log
     for it in (10, 20):
         assert (out1 / f"NODALS_STAGE31_STATE_it{it}.ckpt").is_file()
         assert (out1 / f"flow_sst_it{it}.vtu").is_file()

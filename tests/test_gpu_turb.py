@@ -10,10 +10,18 @@ last-bit difference to ~1e-11), so the assembled vertex operator agrees to
 1e-10 relative in norm (its norm is carried by the 1e8-sized wall-penalty
 rows) and k, omega after the sweeps to 1e-9 relative (a step is a fixed
 number of Gauss-Seidel sweeps, which propagate rounding without amplifying
-it).  The five-point wall fit runs the same Gauss-Newton
-iteration on both sides; its rare golden-section fallback agrees only to
-3e-5 (tests/test_gpu_sst.py), which would show here as a wall-point target
-difference, so the cases below are checked to have every wall point valid.
+it).
+
+The five-point wall fit runs the same Gauss-Newton iteration (12 steps, a
+halving line search, a golden-section fallback of resolution ~1e-5) on both
+sides.  Its objective is flat at the minimum, so whether a step still
+lowers it is decided by rounding, and device and host can take different
+branches: u_tau agrees to 3e-5 (tests/test_gpu_sst.py pins exactly that on
+the reference's fit).  With the log-layer omega target (proportional to
+u_tau) the five-point cases therefore compare the per-point targets to 3e-5
+and the right-hand side and fields after the sweeps to 1e-6.  The one-point
+fit (the spiral case's) is a scalar Spalding root with no such branch and is
+held to the tight bounds.
 """
 import numpy as np
 import pytest
@@ -68,13 +76,18 @@ def vertex_operator(h, st):
     (["--sst-alpha-k", "0.5", "--sst-alpha-omega", "0.6", "--sst-k-fgs-sweeps", "7", "--sst-omega-fgs-sweeps", "5",
       "--sst-gs-omega", "1.0", "--sst-omega-wall-mode", "none"],
      T.Params(alpha_k=0.5, alpha_omega=0.6, k_sweeps=7, omega_sweeps=5, gs_omega=1.0, wall=False)),
-], ids=["reference-defaults", "no-wall"])
+    (["--sst-omega-wall-target", "of_auto"], T.Params(omega_target="of_auto")),
+    (["--sst-omega-wall-target", "trace_log", "--sst-omega-trace-gamma", "80", "--sst-wall-active-points", "1"],
+     T.Params(omega_target="trace_log", trace_gamma=80.0, active_points=1)),
+], ids=["reference-defaults", "no-wall", "of-auto-target", "trace-log-one-point"])
 def test_turb_step_matches_oracle(flow, args, params):
     h, U = flow
     print(f"[gpu] k-omega step vs oracle, args={args}", flush=True)
     h.turb_init(args)
     st = oracle_for(h, U, params)
     ns = st.m.ns
+    five = params.wall and params.active_points == 5
+    tol = 1e-6 if five else 1e-9
     assert np.allclose(h.get("k", ns), st.k, rtol=1e-15, atol=0)
     for it in range(3):
         r = h.turb_step()
@@ -84,12 +97,19 @@ def test_turb_step_matches_oracle(flow, args, params):
         if it == 0:
             for name, (got, want) in zip(("values", "diagonal", "rhs"), vertex_operator(h, st)):
                 print(f"[gpu]   relaxed vertex operator {name}: rel {rel(got, want):.2e}", flush=True)
-                assert rel(got, want) < 1e-10, name
+                assert rel(got, want) < (tol if name == "rhs" and five else 1e-10), name
+            if params.wall:
+                _, want, _, _, _ = st.wall_data(H.cell_values(st.case, U))
+                got = h.get("turb_wall_target", want.size)
+                e = np.abs(got - want) / want
+                print(f"[gpu]   omega targets: max rel {e.max():.2e}, above 1e-9 at {int((e > 1e-9).sum())} "
+                      f"of {e.size} points", flush=True)
+                assert e.max() < (3e-5 if five else 1e-9)
         k, w = h.get("k", ns), h.get("omega", ns)
         print(f"[gpu]   step {it + 1}: k rel {rel(k, st.k):.2e} omega rel {rel(w, st.w):.2e} "
               f"({r['ms']:.3f} ms)", flush=True)
         assert np.all(k[st.m.nvert:] == 0) and np.all(w[st.m.nvert:] == 0)
-        assert rel(k, st.k) < 1e-9 and rel(w, st.w) < 1e-9
+        assert rel(k, st.k) < tol and rel(w, st.w) < tol
 
 
 def test_turb_step_set_state_and_floor(flow):
@@ -97,8 +117,10 @@ def test_turb_step_set_state_and_floor(flow):
     below the floor are clipped and counted like the oracle's."""
     h, U = flow
     print("[gpu] k-omega step from a set state with floor clips", flush=True)
-    h.turb_init(["--sst-k-floor", "1e-3"])
-    st = oracle_for(h, U, T.Params(k_floor=1e-3))
+    # the blended target keeps the five-point fit's branch differences out
+    # of the comparison (see the module docstring)
+    h.turb_init(["--sst-k-floor", "1e-3", "--sst-omega-wall-target", "of_auto"])
+    st = oracle_for(h, U, T.Params(k_floor=1e-3, omega_target="of_auto"))
     nv, ns = st.m.nvert, st.m.ns
     rng = np.random.default_rng(11)
     k0 = np.zeros(ns)
@@ -120,8 +142,8 @@ def test_coupled_momentum_assembly(flow):
     columns alike), as in the oracle's assembly with the same blocks."""
     h, U = flow
     print("[gpu] coupled momentum assembly (eddy viscosity) vs oracle", flush=True)
-    h.turb_init([])
-    st = oracle_for(h, U, T.Params())
+    h.turb_init(["--sst-omega-wall-target", "of_auto"])
+    st = oracle_for(h, U, T.Params(omega_target="of_auto"))
     nv, ns = st.m.nvert, st.m.ns
     rng = np.random.default_rng(5)
     st.k[:nv] = st.k_in * rng.uniform(1.0, 20.0, nv)
@@ -156,10 +178,10 @@ def test_coupled_simple_fixed10_matches_oracle(mesh_dir):
     print("[gpu] coupled SIMPLE fixed10 small pipe vs oracle", flush=True)
     h = H8(["--mesh", mesh_dir["small"], "--max-outer", "10", "--alpha-u", "0.5", "--alpha-p", "0.5", "--p-rtol",
             "0.5", "--p-max", "20", "--amg-smoother", "jacobi", "--pressure-solver", "pcg", "--verbose", "0"])
-    h.turb_init([])
+    h.turb_init(["--sst-wall-active-points", "1"])  # the one-point fit, log target
     h.turb_couple(True)
     case = H.case_from_product(h)
-    st = oracle_for(h, np.zeros((3, case.nv)), T.Params())
+    st = oracle_for(h, np.zeros((3, case.nv)), T.Params(active_points=1))
     st.case = case
     r = h.run()
     assert r["outer"] == 10 and r["turb_ms"] > 0

@@ -54,15 +54,16 @@ def helix(tmp_path_factory):
     return str(meshgen.write_helix_case(tmp_path_factory.mktemp("whelix") / "helix", core=4, ring=2, nslab=48))
 
 
-def _setup(helix, extra=()):
+def _setup(helix, extra=(), turb=(), params=None):
     from nodals_amd._lib import H8
 
     h = H8(["--mesh", helix, "--max-outer", "10", *ARGS, *extra])
-    h.turb_init([])
+    h.turb_init(list(turb))
     h.wall_init([])
     case = H.case_from_product(h)
     m = TO.mesh_from_product(h)
-    st = TO.Step(m, case, h.get("pipe_geometry", 12), TO.Params(), refbind.RefSst(),
+    # the omega wall fit takes the wall path's active point count (1 by default)
+    st = TO.Step(m, case, h.get("pipe_geometry", 12), params or TO.Params(active_points=1), refbind.RefSst(),
                  wall_distance=h.get("wall_distance", m.nvert))
     wp = WO.WallPath(case, st, refbind.RefSst(), fitter=DeviceFit())
     return h, case, st, wp
@@ -159,9 +160,11 @@ def test_wall_schur_and_correction_match_oracle(helix):
 
 
 @pytest.mark.timeout(400)
-def test_wall_simple_fixed10_matches_oracle(helix):
-    print("\n[gpu] wall-function SIMPLE fixed10, small helix Re=5000 (k, omega coupled)", flush=True)
-    h, case, st, wp = _setup(helix)
+@pytest.mark.parametrize("target", ["sample_log", "trace_log"])
+def test_wall_simple_fixed10_matches_oracle(helix, target):
+    print(f"\n[gpu] wall-function SIMPLE fixed10, small helix Re=5000 (k, omega coupled, omega {target})", flush=True)
+    h, case, st, wp = _setup(helix, turb=["--sst-omega-wall-target", target],
+                             params=TO.Params(active_points=1, omega_target=target))
     r = h.run()
     assert r["outer"] == 10 and r["pass_"] == 1
     hist = []

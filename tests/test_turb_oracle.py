@@ -97,6 +97,84 @@ def test_wall_block_vanishes_at_target(small):
         assert np.abs(A @ x - b).max() <= 1e-12 * np.abs(b).max()
 
 
+def _face_geometry(st, face):
+    """Barycentric gradients, 3 volume / area and the face's vertex
+    positions of a wall face's cell, from the vertex coordinates alone."""
+    c, lf, n, area = face
+    X = st.m.points[st.m.tets[c]]
+    M = np.column_stack([np.ones(4), X])             # [1, x_a]
+    grad = np.linalg.inv(M.T)[:, 1:]  # row a: grad lambda_a
+    vol = abs(np.linalg.det(X[1:] - X[0])) / 6.0
+    return grad, 3.0 * vol / area, X
+
+
+def test_trace_block_closed_forms(small):
+    """trace_log's symmetric Nitsche block on a wall face, summed over the
+    face's twelve points, against the face integrals of the P1+BF3 basis:
+    int lambda_a = A/3 on the face's vertices, int 27 l_j l_k l_l = 0.45 A
+    for the face's own bubble, int lambda_j lambda_k = A/12.  A linear field
+    equal to the target g on the face leaves mu s int phi_a (s its slope
+    into the fluid); a constant field c leaves (c - g)(tau int phi_a - mu
+    int dn phi_a), with tau = gamma mu / h and h = 3 volume / area."""
+    _, st = small
+    P = T.Params(omega_target="trace_log", trace_gamma=73.0)
+    st2 = T.Step.__new__(T.Step)
+    st2.__dict__.update(st.__dict__)
+    st2.P = P
+    mu, g, s, cval = 2.3e-3, 41.0, 5.5, 17.0
+    for face in st.wall[:: max(1, len(st.wall) // 9)]:
+        c, lf, n, area = face
+        grad, h, X = _face_geometry(st, face)
+        A = np.zeros((8, 8))
+        b = np.zeros(8)
+        for q in range(12):
+            Aq, bq, ok = st2.trace_block(face, q, mu, g)
+            assert ok
+            A += Aq
+            b += bq
+        assert np.abs(A - A.T).max() <= 1e-13 * np.abs(A).max()
+        fv = [j for j in range(4) if j != lf]
+        int_phi = np.zeros(8)
+        int_phi[fv] = area / 3.0
+        int_phi[4 + lf] = 0.45 * area
+        gn = grad @ n
+        int_dn = np.zeros(8)
+        int_dn[:4] = area * gn
+        int_dn[4 + lf] = 27.0 * area / 12.0 * gn[fv].sum()
+        for i in fv:
+            int_dn[4 + i] = 27.0 * area / 12.0 * gn[lf]
+        x0 = X[fv[0]]
+        lin = np.r_[g - s * (X - x0) @ n, np.zeros(4)]
+        assert np.allclose(A @ lin - b, mu * s * int_phi, rtol=0, atol=1e-11 * np.abs(b).max())
+        tau = 73.0 * mu / h
+        const = np.r_[np.full(4, cval), np.zeros(4)]
+        want = (cval - g) * (tau * int_phi - mu * int_dn)
+        assert np.allclose(A @ const - b, want, rtol=0, atol=1e-11 * np.abs(b).max())
+
+
+def test_vector_fit_uses_the_tangential_speed(small):
+    """The omega wall fit takes the speed tangential to the wall face.  On
+    the straight pipe every wall face is parallel to the axis, so for an
+    axial flow that speed is the axial speed |u_z| at each sample, and a
+    radial component (along the wall normals, which are horizontal) leaves
+    the speeds unchanged only where it is normal to the face."""
+    _, st = small
+    Uc = T.H.cell_values(st.case, plug(st.case))
+    st.wall_data(Uc)
+    cells = st.s_cell.reshape(-1)
+    phi, _ = T.S.basis(st.s_l.reshape(-1, 4))
+    uz = np.einsum("na,na->n", Uc[cells][:, :, 2], phi).reshape(-1, 5)
+    assert max(abs(f[2][2]) for f in st.wall) < 1e-14
+    assert np.allclose(st.last_speed, np.abs(uz), rtol=1e-14, atol=0)
+    # the reference core's fit over 5 and over 1 active point both succeed
+    out5, v5 = refbind.RefSst().fit_values(st.last_speed, np.broadcast_to(st.heights, uz.shape), st.nu, 5)
+    out1, v1 = refbind.RefSst().fit_values(st.last_speed, np.broadcast_to(st.heights, uz.shape), st.nu, 1)
+    assert v5.all() and v1.all() and (out5[:, 0] > 0).all() and (out1[:, 0] > 0).all()
+    # with one point the fit is the pointwise Spalding root at the third sample
+    root, ok = refbind.RefSst().spalding(st.last_speed[:, 2], np.full(uz.shape[0], st.heights[2]), st.nu)
+    assert ok.all() and np.allclose(out1[:, 0], root[:, 0], rtol=1e-10)
+
+
 def vertex_system(st, A, b):
     nv = st.m.nvert
     return A[:nv, :nv].tocsr(), b[:nv]
