@@ -229,23 +229,35 @@ def turb_leg(h, steps: int) -> dict:
     row-L1 relaxation, 20 + 20 coloured Gauss-Seidel sweeps, floors) on the
     converged pipe velocity of context h, with the reference's spiral-case
     settings (alpha 0.3, sweep omega 0.7, penalty 1e6)."""
-    t0 = time.perf_counter()
-    h.turb_init([])
-    setup_s = time.perf_counter() - t0
-    for _ in range(2):
-        h.turb_step()
-    dev, wall = [], []
-    for _ in range(max(1, steps)):
-        ts = time.perf_counter()
-        r = h.turb_step()
-        wall.append(time.perf_counter() - ts)
-        dev.append(r["ms"])
+    def run(args):
+        t0 = time.perf_counter()
+        h.turb_init(args)
+        setup_s = time.perf_counter() - t0
+        for _ in range(2):
+            h.turb_step()
+        dev, wall = [], []
+        for _ in range(max(1, steps)):
+            ts = time.perf_counter()
+            r = h.turb_step()
+            wall.append(time.perf_counter() - ts)
+            dev.append(r["ms"])
+        ns = h.turb["ns"]
+        return setup_s, dev, wall, h.get("k", ns), h.get("omega", ns)
+
+    setup_s, dev, wall, k64, w64 = run([])
     out = {"ms_per_step_device": float(np.mean(dev)), "ms_per_step_wall": 1e3 * float(np.mean(wall)),
            "steps": len(dev), "setup_s": setup_s, "scalar_dofs": h.turb["ns"], "vertex_rows": h.turb["nvert"],
            "colors": h.turb["colors"], "wall_faces": h.turb["wall_faces"], "inlet_faces": h.turb["inlet_faces"],
            "sweeps_per_equation": 20}
+    # the same steps with the sweep operator's values stored in FP32
+    _, dev32, _, k32, w32 = run(["--sst-sweep-precision", "fp32"])
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))  # noqa: E731
+    out["fp32_sweep_values"] = {"ms_per_step_device": float(np.mean(dev32)), "k_rel_l2_vs_fp64": rel(k32, k64),
+                                "omega_rel_l2_vs_fp64": rel(w32, w64), "steps_compared": 2 + len(dev32)}
     print(f"[bench] k-omega step: {out['ms_per_step_device']:.3f} ms device, {out['ms_per_step_wall']:.3f} ms wall "
-          f"({h.turb['colors']} colours, {h.turb['wall_faces']} wall faces)", file=sys.stderr, flush=True)
+          f"({h.turb['colors']} colours, {h.turb['wall_faces']} wall faces); FP32 sweep values "
+          f"{np.mean(dev32):.3f} ms, k/omega vs FP64 {rel(k32, k64):.1e}/{rel(w32, w64):.1e}",
+          file=sys.stderr, flush=True)
     return out
 
 

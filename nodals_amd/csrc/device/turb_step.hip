@@ -47,6 +47,11 @@ TurbParams parseTurbArgs(int argc, const char* const* argv) {
               "--sst-omega-wall-target must be of_auto, sample_log or trace_log");
       P.omegaMode = m == "of_auto" ? 0 : m == "sample_log" ? 1 : 2;
     } else if (f == "--sst-omega-trace-gamma") P.traceGamma = num(f, v);
+    else if (f == "--sst-sweep-precision") {
+      const std::string m = v;
+      require(m == "fp64" || m == "fp32", "--sst-sweep-precision must be fp64 or fp32");
+      P.sweepFp32 = m == "fp32";
+    }
     else if (f.size() == 20 && f.rfind("--sst-profile-fit-y", 0) == 0 && f[19] >= '1' && f[19] <= '5')
       P.heights[(std::size_t)(f[19] - '1')] = num(f, v);
     else if (f == "--sst-wall-sample-scale") P.sampleScale = num(f, v);
@@ -174,6 +179,7 @@ void H8Solver::turbInit(const TurbParams& P) {
   st->blk.alloc(nb * 64);
   st->brhs.alloc(nb * 8);
   st->val.alloc(T.col.size());
+  if (P.sweepFp32) st->valF.alloc(T.col.size());
   st->diag.alloc((std::size_t)T.nvert);
   st->rhs.alloc((std::size_t)T.nvert);
   std::vector<double> k0((std::size_t)T.nS, 0.0), w0((std::size_t)T.nS, 0.0);
@@ -261,12 +267,17 @@ int H8Solver::turbEquation(int eq, double* rel0, double* rel1) {
   // from one captured graph per equation
   const double meanRow = (double)T.col.size() / std::max(T.nvert, 1);
   const int sweeps = eq == 0 ? P.kSweeps : P.omegaSweeps;
+  if (P.sweepFp32) cast<float, double>(s, (long long)T.col.size(), st.val.data(), st.valF.data());
   st.sweepGraph[eq].run(s, graphKey({x, st.val.data(), st.rhs.data()}, sweeps * 4.0 + P.gsOmega), [&]() {
     for (int sw = 0; sw < sweeps; ++sw)
       for (int c = 0; c < T.coloring.ncolors; ++c) {
         const int b = st.colorOff[(std::size_t)c], n = st.colorOff[(std::size_t)c + 1] - b;
-        mcgsColor<double, double>(s, lanesFor(meanRow, n), b, n, st.crows.data(), st.row.data(), st.col.data(),
-                                  st.val.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
+        if (P.sweepFp32)
+          mcgsColor<float, double>(s, lanesFor(meanRow, n), b, n, st.crows.data(), st.row.data(), st.col.data(),
+                                   st.valF.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
+        else
+          mcgsColor<double, double>(s, lanesFor(meanRow, n), b, n, st.crows.data(), st.row.data(),
+                                    st.col.data(), st.val.data(), st.diag.data(), st.rhs.data(), x, P.gsOmega);
       }
   });
   if (rel1) *rel1 = relResid();

@@ -47,6 +47,7 @@ struct TurbState {
   // (the reference holds one count for both, SST24B_ACTIVE_POINTS)
   int omegaFitPoints = 5;
   DBuf<double> vdist;  // vertex wall distances (duct mode), empty for the straight pipe
+  DBuf<float> valF;    // FP32 copy of val for --sst-sweep-precision fp32
   DBuf<double> resid;  // residual of a scalar_solve norm
   const double* vdistPtr() const { return vdist.size() ? vdist.data() : nullptr; }
   std::vector<int> colorOff;

@@ -44,6 +44,10 @@ struct TurbParams {
   // The reference binary's default is sample_log.
   int omegaMode = 1;
   double traceGamma = 50.0;
+  // Storage of the relaxed sweep operator's off-diagonal values: false FP64;
+  // true FP32 (an extension of this build: the sweeps read half the value
+  // bytes; the state, diagonal, right-hand side and accumulation stay FP64).
+  bool sweepFp32 = false;
   // Wall-normal sample heights; zeros select the reference rule
   // d_i * D / 0.004 * sampleScale (turbulence/sst_gpu_v2/nodals.py:289-292).
   std::array<double, 5> heights{{0, 0, 0, 0, 0}};
